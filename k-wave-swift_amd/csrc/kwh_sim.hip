@@ -1,0 +1,1678 @@
+// kwh_sim.hip -- host side of libkwavehip: descriptor validation, operator setup, the step
+// schedule of the k-space first-order loop, sources, sensors, outputs, and the C ABI.
+//
+// Mirrors kspaceFirstOrder1D/2D/3D (Sources/KWave/Solver/KSpaceFirstOrder.swift:203-761). The
+// host owns no field data after setup: every array lives in HBM and every step is a fixed
+// sequence of launches on one HIP stream (optionally replayed from a captured hipGraph).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kwave_hip.h"
+#include "kwh_kernels.h"
+#include "kwh_setup.h"
+
+using namespace kwh;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct KwhError : std::runtime_error {
+    int code;
+    KwhError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define KWH_REQUIRE(cond, code, msg)                         \
+    do {                                                     \
+        if (!(cond)) throw KwhError((code), std::string(msg)); \
+    } while (0)
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory) ? KWH_ENOMEM : KWH_EDEVICE;
+        throw KwhError(code, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+#define HIPCHK(x) hip_check((x), #x)
+
+// Device buffer (RAII).
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        release();
+        if (b == 0) return;
+        HIPCHK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <typename U> U* as() const { return reinterpret_cast<U*>(p); }
+};
+
+int ilog2(int v) {
+    int r = 0;
+    while ((1 << r) < v) ++r;
+    return r;
+}
+
+FftPlan make_plan(int L) {
+    FftPlan pl{};
+    pl.L = L;
+    pl.nstages = 0;
+    int m = L;
+    auto push = [&](int r) {
+        if (pl.nstages >= kMaxStages) throw KwhError(KWH_EUNSUPPORTED, "FFT length has too many factors");
+        pl.radix[pl.nstages++] = r;
+        m /= r;
+    };
+    while (m % 16 == 0) push(16);
+    while (m % 8 == 0) push(8);
+    while (m % 4 == 0) push(4);
+    while (m % 2 == 0) push(2);
+    while (m % 3 == 0) push(3);
+    while (m % 5 == 0) push(5);
+    for (int f = 7; m > 1; f += 2)
+        while (m % f == 0) push(f);
+    return pl;
+}
+
+std::vector<double> fft_freq(int n, double d) {
+    std::vector<double> r(n, 0.0);
+    const double val = 1.0 / (n * d);
+    const int cutoff = (n - 1) / 2 + 1;
+    for (int i = 0; i < cutoff; ++i) r[i] = i * val;
+    int j = -(n / 2);
+    for (int i = cutoff; i < n; ++i) r[i] = (j++) * val;
+    return r;
+}
+
+// pmlProfile (Sources/KWave/PML/PML.swift:17-43).
+std::vector<double> pml_profile(int n, double dx, double dt, double c, int pml_size, double pml_alpha, bool staggered) {
+    const double a = pml_alpha * (c / dx);
+    const double m = pml_size;
+    std::vector<double> pml(n, 1.0);
+    for (int k = 0; k < std::min(pml_size, n); ++k) {
+        const double x = k + 1;
+        double pl, pr;
+        if (staggered) {
+            pl = (m + 0.5 - x) / m;
+            pr = (x + 0.5) / m;
+        } else {
+            pl = (m + 1 - x) / m;
+            pr = x / m;
+        }
+        pml[k] = std::exp(-a * dt / 2.0 * std::pow(pl, 4.0));
+        pml[n - pml_size + k] = std::exp(-a * dt / 2.0 * std::pow(pr, 4.0));
+    }
+    return pml;
+}
+
+double db2neper_unit(double y) { return 100.0 * std::pow(1e-6 / (2.0 * M_PI), y) / (20.0 * std::log10(M_E)); }
+
+int grid_blocks(int64_t n, int threads = 256) {
+    int64_t b = (n + threads - 1) / threads;
+    return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 65536);
+}
+
+}  // namespace
+
+struct kwh_ctx {
+    int device;
+};
+
+// Abstract sim: the typed implementation follows.
+struct kwh_sim {
+    virtual ~kwh_sim() = default;
+    virtual void launch(int t0, int t1) = 0;
+    virtual void sync() = 0;
+    virtual void run(int t0, int t1, kwh_step_cb cb, int interval, void* user) = 0;
+    virtual void output_shape(int field, int64_t shape[3], int* ndim) = 0;
+    virtual void read_output(int field, void* dst, size_t bytes) = 0;
+    virtual int steps_done() const = 0;
+    virtual size_t state_bytes() = 0;
+    virtual void save_state(void* dst, size_t bytes) = 0;
+    virtual void load_state(const void* src, size_t bytes) = 0;
+    virtual void set_timing(bool on) = 0;
+    virtual int kernel_stats(kwh_kernel_stat* st, int cap) = 0;
+    virtual void set_graph(bool on) = 0;
+};
+
+namespace {
+
+constexpr int kNames = 16;
+const char* kKernelNames[kNames] = {"strided_fwd", "strided_inv", "middle_grad", "middle_div", "middle_abs",
+                                    "z_velocity",  "z_density",   "z_eos",       "z_init",     "z_forward_real",
+                                    "z_inverse_real", "middle_table", "gather", "source_scatter", "fill", "other"};
+enum KId { K_SF, K_SI, K_MG, K_MD, K_MA, K_ZV, K_ZD, K_ZE, K_ZI, K_ZFR, K_ZIR, K_MT, K_GA, K_SC, K_FI, K_OT };
+
+template <typename T>
+struct SourceDev {
+    bool present = false;
+    int mode = 0;
+    int rows = 0, len = 0;
+    int64_t n = 0;
+    DBuf index, signal, base, rowptr;
+    double factor[3] = {1, 1, 1};
+};
+
+template <typename T>
+struct SimT final : kwh_sim {
+    using C = cx<T>;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    Geo g{};
+    int dim = 0;
+    int nphys[3] = {1, 1, 1};
+    double dphys[3] = {1, 1, 1};
+    double dt = 0;
+    int nt = 0;
+    int mid_axis = 1;
+    bool plain1 = false;   // axis-1 plain passes (3D)
+    int steps = 0;
+    double c_ref = 0;
+
+    // tables
+    AxisTabs<T> tab{};
+    AxisTabs<double> tab64{};
+    DBuf tw_buf[3], ddxp_buf[3], ddxn_buf[3], coll_buf[3], pml_buf[3], pmlsg_buf[3];
+    DBuf tw64_buf[3], kv64_buf[3];
+    DBuf kappa, skappa, nab1, nab2;
+
+    // media
+    DBuf c0m, rho0m, dtrm[3], taum, etam, bonam;
+    SMap<T> c0s{}, rho0s{}, dtrs[3]{}, taus{}, etas{}, bonas{};
+    bool nonlinear = false, absorbing = false, stokes = false, dispersion = false;
+
+    // state
+    DBuf p, u[3], rho[3], rhoT, ucol[3], p0s;
+    DBuf spec[6];          // spectra work buffers
+    DBuf scr_real, scr_spec, corr_u[3], corr_p;
+    bool has_p0 = false;
+    bool P_valid = false;  // spec[0] holds R2C_z(p) for the next step
+
+    // sources
+    SourceDev<T> psrc, usrc[3];
+
+    // sensor
+    int64_t nsens = 0;
+    int ncorner = 1;
+    bool cart = false;
+    DBuf sidx, sw;
+    uint32_t record = 0;
+    bool rec_p = false, rec_u = false, store_p = false, store_u = false;
+    DBuf ser_p, ser_u[3], pmax, pmin, psum2, umax[3], usum2[3];
+    int rec_steps = 0;
+    // directivity
+    int ndir = 0;
+    std::vector<std::unique_ptr<DBuf>> dir_w, dir_idx, dir_pos;
+    std::vector<int64_t> dir_count;
+    DBuf dir_field;
+
+    // timing
+    bool timing = false;
+    struct Pending { int id; hipEvent_t a, b; double bytes; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> ev_pool;
+    int64_t stat_launch[kNames] = {};
+    double stat_ms[kNames] = {};
+    double stat_bytes[kNames] = {};
+    // graph
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+
+    size_t lds_budget = 64 * 1024;
+
+    ~SimT() override {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : ev_pool) (void)hipEventDestroy(e);
+        for (auto& pd : pending) {
+            (void)hipEventDestroy(pd.a);
+            (void)hipEventDestroy(pd.b);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ------------------------------------------------------------------ timing helpers
+    hipEvent_t get_event() {
+        if (!ev_pool.empty()) {
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    template <typename F>
+    void timed(int id, double bytes, F&& f) {
+        if (!timing) {
+            f();
+            HIPCHK(hipGetLastError());
+            return;
+        }
+        hipEvent_t a = get_event(), b = get_event();
+        HIPCHK(hipEventRecord(a, stream));
+        f();
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(b, stream));
+        pending.push_back({id, a, b, bytes});
+    }
+    void collect_timing() {
+        for (auto& pd : pending) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, pd.a, pd.b));
+            stat_launch[pd.id] += 1;
+            stat_ms[pd.id] += ms;
+            stat_bytes[pd.id] = pd.bytes;
+            ev_pool.push_back(pd.a);
+            ev_pool.push_back(pd.b);
+        }
+        pending.clear();
+    }
+
+    // ------------------------------------------------------------------ launch geometry
+    struct ZCfg { ZTile zt; int blocks, threads; size_t lds; };
+    ZCfg zcfg(int nbuf) const {
+        const int L = g.m2;
+        const int npairs = (g.nrows + 1) / 2;
+        int ntr = 32;
+        while (ntr > 1 && ((size_t)nbuf * L * (ntr + 1) * sizeof(C) > lds_budget || ntr / 2 >= npairs)) ntr >>= 1;
+        ZCfg c{};
+        c.zt.ntr = ntr;
+        c.zt.lg_ntr = ilog2(ntr);
+        c.zt.pitch = ntr > 1 ? ntr + 1 : 1;
+        c.zt.q0 = 0;
+        c.blocks = (npairs + ntr - 1) / ntr;
+        int thr = ((ntr * L / 8 + 63) / 64) * 64;
+        c.threads = std::max(64, std::min(256, thr));
+        c.lds = (size_t)nbuf * L * c.zt.pitch * sizeof(C);
+        KWH_REQUIRE(c.lds <= 65536, KWH_EUNSUPPORTED, "contiguous axis too long for LDS staging");
+        return c;
+    }
+    struct SCfg { int64_t stride, inner, ncols; int L, lg_ntr, pitch, blocks, threads; size_t lds; };
+    SCfg scfg(int axis, int nbuf) const {
+        SCfg c{};
+        c.L = axis == 0 ? g.m0 : g.m1;
+        if (axis == 0) { c.stride = (int64_t)g.m1 * g.mzh; c.inner = c.stride; c.ncols = c.stride; }
+        else { c.stride = g.mzh; c.inner = g.mzh; c.ncols = (int64_t)g.m0 * g.mzh; }
+        const int L = std::max(c.L, 1);
+        int ntr = 64;
+        while (ntr > 1 && ((size_t)nbuf * L * ntr * sizeof(C) + 64 * sizeof(int64_t) > lds_budget || ntr / 2 >= c.ncols))
+            ntr >>= 1;
+        c.lg_ntr = ilog2(ntr);
+        c.pitch = ntr;
+        c.blocks = (int)((c.ncols + ntr - 1) / ntr);
+        int thr = ((ntr * L / 8 + 63) / 64) * 64;
+        c.threads = std::max(64, std::min(256, thr));
+        c.lds = (size_t)nbuf * L * ntr * sizeof(C) + 64 * sizeof(int64_t);
+        KWH_REQUIRE(c.lds <= 65536, KWH_EUNSUPPORTED, "axis too long for LDS staging");
+        return c;
+    }
+
+    double spec_bytes() const { return (double)g.NH * sizeof(C); }
+    double real_bytes() const { return (double)g.N * sizeof(T); }
+
+    // ------------------------------------------------------------------ pass wrappers
+    void strided(int axis, bool inv, std::initializer_list<void*> fields) {
+        SCfg c = scfg(axis, 2);
+        if (c.L <= 1) return;
+        StridedArgs a{};
+        int nf = 0;
+        for (void* f : fields) a.fields[nf++] = f;
+        a.nfields = nf;
+        a.axis = axis;
+        a.L = c.L;
+        a.stride = c.stride;
+        a.inner = c.inner;
+        a.ncols = c.ncols;
+        a.lg_ntr = c.lg_ntr;
+        a.pitch = c.pitch;
+        dim3 grid(c.blocks, nf);
+        const double bytes = 2.0 * spec_bytes() * nf;
+        timed(inv ? K_SI : K_SF, bytes, [&] {
+            if (inv)
+                hipLaunchKernelGGL((strided_pass<T, true>), grid, dim3(c.threads), c.lds, stream, a, tab.plan[axis], tab.tw[axis]);
+            else
+                hipLaunchKernelGGL((strided_pass<T, false>), grid, dim3(c.threads), c.lds, stream, a, tab.plan[axis], tab.tw[axis]);
+        });
+    }
+
+    void middle(int kid, std::vector<const void*> ins, std::vector<MidOp> ops) {
+        SCfg c = scfg(mid_axis, 3);
+        MidArgs a{};
+        a.nin = (int)ins.size();
+        for (int i = 0; i < a.nin; ++i) a.in[i] = ins[i];
+        a.nop = (int)ops.size();
+        for (int i = 0; i < a.nop; ++i) a.op[i] = ops[i];
+        a.axis = mid_axis;
+        a.L = std::max(c.L, 1);
+        a.stride = c.stride;
+        a.inner = c.inner;
+        a.ncols = c.ncols;
+        a.lg_ntr = c.lg_ntr;
+        a.pitch = c.pitch;
+        a.scale = 1.0 / (double)g.N;
+        const double bytes = spec_bytes() * (a.nin + a.nop);
+        timed(kid, bytes, [&] {
+            hipLaunchKernelGGL((middle_pass<T>), dim3(c.blocks), dim3(c.threads), c.lds, stream, a, tab, g);
+        });
+    }
+
+    // Full forward transform of a real field into `out` (z, then plain passes): leaves the
+    // spectrum ready for a middle pass.
+    void forward_to_middle(const T* src, C* out, T scale) {
+        ZCfg zc = zcfg(2);
+        timed(K_ZFR, real_bytes() + spec_bytes(), [&] {
+            hipLaunchKernelGGL((z_forward_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, src, out, scale, g,
+                               zc.zt, tab);
+        });
+        if (plain1) strided(1, false, {out});
+    }
+    void inverse_from_middle(C* in, T* dst, T scale, bool accumulate) {
+        if (plain1) strided(1, true, {in});
+        ZCfg zc = zcfg(2);
+        timed(K_ZIR, real_bytes() + spec_bytes(), [&] {
+            hipLaunchKernelGGL((z_inverse_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, in, dst, scale,
+                               accumulate ? 1 : 0, g, zc.zt, tab);
+        });
+    }
+
+    template <typename U> void fill(U* ptr, U v, int64_t n) {
+        timed(K_FI, (double)n * sizeof(U), [&] {
+            hipLaunchKernelGGL((fill_kernel<U>), dim3(grid_blocks(n)), dim3(256), 0, stream, ptr, v, n);
+        });
+    }
+
+    // ------------------------------------------------------------------ setup
+    void setup(const kwh_sim_desc& d, int device) {
+        dev = device;
+        HIPCHK(hipSetDevice(dev));
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (const char* e = std::getenv("KWH_LDS_BUDGET")) lds_budget = (size_t)std::atol(e);
+        dim = d.dim;
+        for (int a = 0; a < dim; ++a) { nphys[a] = d.n[a]; dphys[a] = d.d[a]; }
+        dt = d.dt;
+        nt = d.nt;
+        g.dim = dim;
+        g.off = 3 - dim;
+        int m[3] = {1, 1, 1};
+        for (int a = 0; a < dim; ++a) m[a + g.off] = nphys[a];
+        g.m0 = m[0]; g.m1 = m[1]; g.m2 = m[2];
+        g.mzh = g.m2 / 2 + 1;
+        g.N = (int64_t)g.m0 * g.m1 * g.m2;
+        g.NH = (int64_t)g.m0 * g.m1 * g.mzh;
+        g.nrows = g.m0 * g.m1;
+        mid_axis = g.m0 > 1 ? 0 : 1;
+        plain1 = mid_axis == 0 && g.m1 > 1;
+        const int64_t N = g.N;
+
+        // ---- media (KSpaceFirstOrder.swift:573-579)
+        auto upload_field = [&](const kwh_field& f, DBuf& buf, SMap<T>& sm, const char* name) {
+            if (f.count == 1) {
+                sm.p = nullptr;
+                sm.s = T(f.data[0]);
+                return;
+            }
+            KWH_REQUIRE(f.count == N, KWH_EINVAL, std::string(name) + " must be scalar or grid-shaped");
+            DBuf tmp;
+            tmp.alloc(N * sizeof(float));
+            HIPCHK(hipMemcpyAsync(tmp.p, f.data, N * sizeof(float), hipMemcpyHostToDevice, stream));
+            buf.alloc(N * sizeof(T));
+            hipLaunchKernelGGL((convert_kernel<float, T>), dim3(grid_blocks(N)), dim3(256), 0, stream, tmp.as<float>(),
+                               buf.as<T>(), T(1), N);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream));
+            sm.p = buf.as<T>();
+            sm.s = 0;
+        };
+        upload_field(d.sound_speed, c0m, c0s, "soundSpeed");
+        upload_field(d.density, rho0m, rho0s, "density");
+
+        build_axis_tables_fft();
+        if (d.smooth_c0 && c0s.p) smooth_inplace(c0m.as<T>());
+        if (d.smooth_rho0 && rho0s.p) smooth_inplace(rho0m.as<T>());
+
+        // c_ref = max(c0) (:786-788)
+        c_ref = c0s.p ? max_abs(c0m.as<T>(), N) : (double)c0s.s;
+        // Validate after smoothing as well.
+        KWH_REQUIRE(c_ref > 0 && std::isfinite(c_ref), KWH_EINVAL, "soundSpeed must be positive and finite");
+
+        build_axis_tables_ops(d);
+        build_kspace_tables(d);
+        build_media(d);
+        build_sources(d);
+        build_sensor(d);
+        allocate_state(d);
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    double max_abs(const T* x, int64_t n) {
+        DBuf r;
+        r.alloc(sizeof(unsigned long long));
+        HIPCHK(hipMemsetAsync(r.p, 0, sizeof(unsigned long long), stream));
+        hipLaunchKernelGGL((maxabs_kernel<T>), dim3(std::min(grid_blocks(n), 1024)), dim3(256), 0, stream, x, n,
+                           r.as<unsigned long long>());
+        HIPCHK(hipGetLastError());
+        unsigned long long bits = 0;
+        HIPCHK(hipMemcpyAsync(&bits, r.p, sizeof(bits), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        double v;
+        std::memcpy(&v, &bits, sizeof(v));
+        return v;
+    }
+    double max_abs64(const double* x, int64_t n) {
+        DBuf r;
+        r.alloc(sizeof(unsigned long long));
+        HIPCHK(hipMemsetAsync(r.p, 0, sizeof(unsigned long long), stream));
+        hipLaunchKernelGGL((maxabs_kernel<double>), dim3(std::min(grid_blocks(n), 1024)), dim3(256), 0, stream, x, n,
+                           r.as<unsigned long long>());
+        HIPCHK(hipGetLastError());
+        unsigned long long bits = 0;
+        HIPCHK(hipMemcpyAsync(&bits, r.p, sizeof(bits), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        double v;
+        std::memcpy(&v, &bits, sizeof(v));
+        return v;
+    }
+
+    template <typename U>
+    void upload(DBuf& b, const std::vector<U>& v) {
+        b.alloc(v.size() * sizeof(U));
+        if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // Twiddles and plans for T and for the float64 setup transforms.
+    void build_axis_tables_fft() {
+        const int m[3] = {g.m0, g.m1, g.m2};
+        for (int ax = 0; ax < 3; ++ax) {
+            const int L = m[ax];
+            FftPlan pl = L > 1 ? make_plan(L) : FftPlan{1, 0, {}};
+            tab.plan[ax] = pl;
+            tab64.plan[ax] = pl;
+            std::vector<C> tw(L);
+            std::vector<cx<double>> tw64(L);
+            for (int i = 0; i < L; ++i) {
+                const double ang = -2.0 * M_PI * (double)i / (double)L;
+                tw[i] = C{T(std::cos(ang)), T(std::sin(ang))};
+                tw64[i] = cx<double>{std::cos(ang), std::sin(ang)};
+            }
+            upload(tw_buf[ax], tw);
+            upload(tw64_buf[ax], tw64);
+            tab.tw[ax] = tw_buf[ax].as<C>();
+            tab64.tw[ax] = tw64_buf[ax].as<cx<double>>();
+            // wavenumbers (KWaveGrid.swift:72-78); dummy axes have k = 0
+            std::vector<double> kv(L, 0.0);
+            if (L > 1 || ax >= g.off) {
+                const int a = ax - g.off;
+                if (a >= 0) {
+                    auto f = fft_freq(L, dphys[a]);
+                    for (int i = 0; i < L; ++i) kv[i] = 2.0 * M_PI * f[i];
+                }
+            }
+            upload(kv64_buf[ax], kv);
+        }
+    }
+
+    // 1-D operator tables with the Hermitian part at an even-length Nyquist bin.
+    void build_axis_tables_ops(const kwh_sim_desc& d) {
+        const int m[3] = {g.m0, g.m1, g.m2};
+        for (int ax = 0; ax < 3; ++ax) {
+            const int L = m[ax];
+            const int a = ax - g.off;
+            std::vector<C> dp(L), dn(L), co(L);
+            std::vector<T> pm(L, T(1)), pms(L, T(1));
+            if (a >= 0) {
+                auto f = fft_freq(L, dphys[a]);
+                for (int i = 0; i < L; ++i) {
+                    const double k = 2.0 * M_PI * f[i];
+                    const double th = k * dphys[a] / 2.0;
+                    // i k e^{+i th} = (-k sin th, k cos th); i k e^{-i th} = (k sin th, k cos th)
+                    double dpr = -k * std::sin(th), dpi = k * std::cos(th);
+                    double dnr = k * std::sin(th), dni = k * std::cos(th);
+                    double cr = std::cos(th), ci = -std::sin(th);
+                    if (L % 2 == 0 && i == L / 2) { dpi = 0; dni = 0; ci = 0; }
+                    dp[i] = C{T(dpr), T(dpi)};
+                    dn[i] = C{T(dnr), T(dni)};
+                    co[i] = C{T(cr), T(ci)};
+                }
+                KWH_REQUIRE(d.pml_size >= 0 && d.pml_size <= L, KWH_EINVAL, "pmlSize must be between 0 and the grid size");
+                auto p1 = pml_profile(L, dphys[a], dt, c_ref, d.pml_size, d.pml_alpha, false);
+                auto p2 = pml_profile(L, dphys[a], dt, c_ref, d.pml_size, d.pml_alpha, true);
+                for (int i = 0; i < L; ++i) { pm[i] = T(p1[i]); pms[i] = T(p2[i]); }
+            } else {
+                for (int i = 0; i < L; ++i) { dp[i] = C{0, 0}; dn[i] = C{0, 0}; co[i] = C{1, 0}; }
+            }
+            upload(ddxp_buf[ax], dp);
+            upload(ddxn_buf[ax], dn);
+            upload(coll_buf[ax], co);
+            upload(pml_buf[ax], pm);
+            upload(pmlsg_buf[ax], pms);
+            tab.ddxp[ax] = ddxp_buf[ax].as<C>();
+            tab.ddxn[ax] = ddxn_buf[ax].as<C>();
+            tab.coll[ax] = coll_buf[ax].as<C>();
+            tab.pml[ax] = pml_buf[ax].as<T>();
+            tab.pmlsg[ax] = pmlsg_buf[ax].as<T>();
+        }
+    }
+
+    KGeo kgeo() const {
+        KGeo kg{};
+        kg.m0 = g.m0; kg.m1 = g.m1; kg.m2 = g.m2; kg.mzh = g.mzh; kg.NH = g.NH;
+        kg.kv0 = kv64_buf[0].as<double>();
+        kg.kv1 = kv64_buf[1].as<double>();
+        kg.kv2 = kv64_buf[2].as<double>();
+        return kg;
+    }
+
+    void build_kspace_tables(const kwh_sim_desc& d) {
+        const bool need_sk = (d.p_source.index && d.p_source.mode == KWH_SRC_ADDITIVE) ||
+                             [&] { for (int a = 0; a < 3; ++a) if (d.u_source[a].index && d.u_source[a].signal && d.u_source[a].mode == KWH_SRC_ADDITIVE) return true; return false; }();
+        kappa.alloc(g.NH * sizeof(T));
+        if (need_sk) skappa.alloc(g.NH * sizeof(T));
+        // absorption flags (makeAbsorption, :855-878)
+        absorbing = d.alpha_coeff.data != nullptr && d.alpha_mode != KWH_ALPHA_NONE;
+        double y = d.alpha_power;
+        if (d.alpha_mode == KWH_ALPHA_STOKES) y = 2.0;
+        stokes = absorbing && (d.alpha_mode == KWH_ALPHA_STOKES || std::fabs(y - 2.0) < 1e-10);
+        dispersion = absorbing && !stokes && d.alpha_mode != KWH_ALPHA_NO_DISPERSION;
+        if (absorbing && !stokes) nab1.alloc(g.NH * sizeof(T));
+        if (dispersion) nab2.alloc(g.NH * sizeof(T));
+        hipLaunchKernelGGL((kspace_tables_kernel<T>), dim3(grid_blocks(g.NH)), dim3(256), 0, stream, kgeo(),
+                           c_ref * dt / 2.0, y - 2.0, y - 1.0, kappa.as<T>(), skappa.as<T>(), nab1.as<T>(), nab2.as<T>());
+        HIPCHK(hipGetLastError());
+    }
+
+    void build_media(const kwh_sim_desc& d) {
+        const int64_t N = g.N;
+        // dt / staggered rho0 (:610-614)
+        for (int a = 0; a < dim; ++a) {
+            if (rho0s.p) {
+                dtrm[a].alloc(N * sizeof(T));
+                hipLaunchKernelGGL((stagger_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream, rho0m.as<T>(),
+                                   dtrm[a].as<T>(), g.m0, g.m1, g.m2, a + g.off, dt);
+                HIPCHK(hipGetLastError());
+                dtrs[a] = SMap<T>{dtrm[a].as<T>(), T(0)};
+            } else {
+                dtrs[a] = SMap<T>{nullptr, T(dt / (double)rho0s.s)};
+            }
+        }
+        if (absorbing) {
+            double y = d.alpha_mode == KWH_ALPHA_STOKES ? 2.0 : d.alpha_power;
+            const double db = db2neper_unit(y);
+            DBuf alpha_dev;
+            T as = 0;
+            const T* ap = nullptr;
+            if (d.alpha_coeff.count == 1) as = T(d.alpha_coeff.data[0]);
+            else {
+                KWH_REQUIRE(d.alpha_coeff.count == N, KWH_EINVAL, "alphaCoeff must be scalar or grid-shaped");
+                alpha_dev.alloc(N * sizeof(T));
+                DBuf tmp;
+                tmp.alloc(N * sizeof(float));
+                HIPCHK(hipMemcpyAsync(tmp.p, d.alpha_coeff.data, N * sizeof(float), hipMemcpyHostToDevice, stream));
+                hipLaunchKernelGGL((convert_kernel<float, T>), dim3(grid_blocks(N)), dim3(256), 0, stream,
+                                   tmp.as<float>(), alpha_dev.as<T>(), T(1), N);
+                HIPCHK(hipStreamSynchronize(stream));
+                ap = alpha_dev.as<T>();
+            }
+            const bool map = c0s.p || ap;
+            const double tanfac = std::tan(M_PI * y / 2.0);
+            if (map) {
+                taum.alloc(N * sizeof(T));
+                if (dispersion) etam.alloc(N * sizeof(T));
+                hipLaunchKernelGGL((absorption_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream, c0s.p, c0s.s,
+                                   ap, as, N, db, y, stokes ? 1 : 0, tanfac, taum.as<T>(),
+                                   etam.as<T>());
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipStreamSynchronize(stream));
+                taus = SMap<T>{taum.as<T>(), 0};
+                etas = SMap<T>{etam.as<T>(), 0};
+            } else {
+                const double c = c0s.s, a = (double)as * db;
+                taus = SMap<T>{nullptr, T(stokes ? -2.0 * a * c : -2.0 * a * std::pow(c, y - 1.0))};
+                etas = SMap<T>{nullptr, T(2.0 * a * std::pow(c, y) * tanfac)};
+            }
+        }
+        // B/A (:902-907): disabled when absent or all zero
+        if (d.bona.data) {
+            bool any = false;
+            for (int64_t i = 0; i < d.bona.count; ++i) any |= d.bona.data[i] != 0.0f;
+            if (any) {
+                nonlinear = true;
+                upload_scalar_or_map(d.bona, bonam, bonas, "bOnA");
+            }
+        }
+    }
+
+    void upload_scalar_or_map(const kwh_field& f, DBuf& buf, SMap<T>& sm, const char* name) {
+        if (f.count == 1) { sm = SMap<T>{nullptr, T(f.data[0])}; return; }
+        KWH_REQUIRE(f.count == g.N, KWH_EINVAL, std::string(name) + " must be scalar or grid-shaped");
+        std::vector<T> h(f.data, f.data + f.count);
+        upload(buf, h);
+        sm = SMap<T>{buf.as<T>(), 0};
+    }
+
+    // Smoothing in float64 (Filters.swift:273-347): window in k-space, then restore max|f|.
+    void smooth_inplace(T* f) {
+        const int64_t N = g.N;
+        DBuf x64, s64, win;
+        x64.alloc(N * sizeof(double));
+        s64.alloc(g.NH * sizeof(cx<double>));
+        win.alloc(g.NH * sizeof(double));
+        hipLaunchKernelGGL((convert_kernel<T, double>), dim3(grid_blocks(N)), dim3(256), 0, stream, f, x64.as<double>(), 1.0, N);
+        HIPCHK(hipGetLastError());
+        const double src_max = max_abs64(x64.as<double>(), N);
+        build_window(win.as<double>());
+        transform_apply64(x64.as<double>(), s64.as<cx<double>>(), win.as<double>());
+        const double out_max = max_abs64(x64.as<double>(), N);
+        const double scale = out_max > 0 ? src_max / out_max : 1.0;
+        hipLaunchKernelGGL((convert_kernel<double, T>), dim3(grid_blocks(N)), dim3(256), 0, stream, x64.as<double>(), f,
+                           T(scale), N);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void build_window(double* out) {
+        // rotationProfile (Filters.swift:252-262)
+        int Lw = 1;
+        for (int a = 0; a < dim; ++a) Lw = std::max(Lw, nphys[a] % 2 ? nphys[a] : nphys[a] + 1);
+        std::vector<double> w(Lw, 1.0);
+        if (Lw > 1)
+            for (int i = 0; i < Lw; ++i) {
+                const double x = 2.0 * M_PI * i / (Lw - 1);
+                w[i] = 0.42 - 0.5 * std::cos(x) + 0.08 * std::cos(2 * x);
+            }
+        DBuf wd;
+        upload(wd, w);
+        WinGeo wg{};
+        wg.dim = dim;
+        for (int a = 0; a < 3; ++a) wg.n[a] = a < dim ? nphys[a] : 1;
+        wg.off = g.off;
+        wg.m1 = g.m1;
+        wg.mzh = g.mzh;
+        wg.NH = g.NH;
+        wg.win = wd.as<double>();
+        wg.Lw = Lw;
+        wg.radius = (Lw - 1) / 2.0;
+        wg.place2d = dim == 2 ? 1 : 0;
+        hipLaunchKernelGGL((window_kernel<double>), dim3(grid_blocks(g.NH)), dim3(256), 0, stream, wg, out);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // x <- C2R(table * R2C(x)) / N in float64 with the same kernels instantiated for double.
+    void transform_apply64(double* x, cx<double>* s, const double* table) {
+        using D = double;
+        const size_t cs = sizeof(cx<double>);
+        // z forward
+        {
+            const int L = g.m2, npairs = (g.nrows + 1) / 2;
+            int ntr = 32;
+            while (ntr > 1 && ((size_t)2 * L * (ntr + 1) * cs > lds_budget || ntr / 2 >= npairs)) ntr >>= 1;
+            ZTile zt{0, ntr, ilog2(ntr), ntr > 1 ? ntr + 1 : 1};
+            int thr = std::max(64, std::min(256, ((ntr * L / 8 + 63) / 64) * 64));
+            const size_t lds = (size_t)2 * L * zt.pitch * cs;
+            const int blocks = (npairs + ntr - 1) / ntr;
+            hipLaunchKernelGGL((z_forward_real<D>), dim3(blocks), dim3(thr), lds, stream, x, s, 1.0, g, zt, tab64);
+            HIPCHK(hipGetLastError());
+            auto strided64 = [&](int axis, bool inv) {
+                const int La = axis == 0 ? g.m0 : g.m1;
+                if (La <= 1) return;
+                StridedArgs a{};
+                a.fields[0] = s;
+                a.nfields = 1;
+                a.axis = axis;
+                a.L = La;
+                if (axis == 0) { a.stride = (int64_t)g.m1 * g.mzh; a.inner = a.stride; a.ncols = a.stride; }
+                else { a.stride = g.mzh; a.inner = g.mzh; a.ncols = (int64_t)g.m0 * g.mzh; }
+                int nt_ = 64;
+                while (nt_ > 1 && ((size_t)2 * La * nt_ * cs + 64 * 8 > lds_budget || nt_ / 2 >= a.ncols)) nt_ >>= 1;
+                a.lg_ntr = ilog2(nt_);
+                a.pitch = nt_;
+                const int th = std::max(64, std::min(256, ((nt_ * La / 8 + 63) / 64) * 64));
+                const size_t l = (size_t)2 * La * nt_ * cs + 64 * 8;
+                const int bl = (int)((a.ncols + nt_ - 1) / nt_);
+                if (inv) hipLaunchKernelGGL((strided_pass<D, true>), dim3(bl, 1), dim3(th), l, stream, a, tab64.plan[axis], tab64.tw[axis]);
+                else hipLaunchKernelGGL((strided_pass<D, false>), dim3(bl, 1), dim3(th), l, stream, a, tab64.plan[axis], tab64.tw[axis]);
+                HIPCHK(hipGetLastError());
+            };
+            if (plain1) strided64(1, false);
+            // middle
+            {
+                const int ax = mid_axis;
+                const int La = ax == 0 ? g.m0 : g.m1;
+                MidArgs a{};
+                a.in[0] = s;
+                a.nin = 1;
+                a.op[0] = MidOp{OP_TAB, 0, 0, s, table};
+                a.nop = 1;
+                a.axis = ax;
+                a.L = std::max(La, 1);
+                if (ax == 0) { a.stride = (int64_t)g.m1 * g.mzh; a.inner = a.stride; a.ncols = a.stride; }
+                else { a.stride = g.mzh; a.inner = g.mzh; a.ncols = (int64_t)g.m0 * g.mzh; }
+                int nt_ = 64;
+                while (nt_ > 1 && ((size_t)3 * a.L * nt_ * cs + 64 * 8 > lds_budget || nt_ / 2 >= a.ncols)) nt_ >>= 1;
+                a.lg_ntr = ilog2(nt_);
+                a.pitch = nt_;
+                a.scale = 1.0 / (double)g.N;
+                const int th = std::max(64, std::min(256, ((nt_ * a.L / 8 + 63) / 64) * 64));
+                const size_t l = (size_t)3 * a.L * nt_ * cs + 64 * 8;
+                const int bl = (int)((a.ncols + nt_ - 1) / nt_);
+                hipLaunchKernelGGL((middle_pass<D>), dim3(bl), dim3(th), l, stream, a, tab64, g);
+                HIPCHK(hipGetLastError());
+            }
+            if (plain1) strided64(1, true);
+            hipLaunchKernelGGL((z_inverse_real<D>), dim3(blocks), dim3(thr), lds, stream, s, x, 1.0, 0, g, zt, tab64);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+    }
+
+    // Sources (KSpaceFirstOrder.swift:616-638 / :405-415).
+    void build_source(const kwh_source_field& sf, SourceDev<T>& sd, bool is_p) {
+        if (!sf.index || !sf.signal || sf.n_points <= 0) return;
+        const int64_t N = g.N;
+        KWH_REQUIRE(sf.rows == 1 || sf.rows == sf.n_points, KWH_EINVAL,
+                    "source signal must have 1 row or one row per source point");
+        KWH_REQUIRE(sf.len >= 0, KWH_EINVAL, "source signal length must be >= 0");
+        KWH_REQUIRE(sf.mode >= 0 && sf.mode <= 2, KWH_EINVAL, "invalid source mode");
+        std::vector<int64_t> idx(sf.index, sf.index + sf.n_points);
+        for (int64_t i = 0; i < sf.n_points; ++i) {
+            KWH_REQUIRE(idx[i] >= 0 && idx[i] < N, KWH_EINVAL, "source index outside the grid");
+            KWH_REQUIRE(i == 0 || idx[i] > idx[i - 1], KWH_EINVAL, "source indices must be ascending and unique");
+        }
+        sd.present = true;
+        sd.mode = sf.mode;
+        sd.rows = sf.rows;
+        sd.len = sf.len;
+        sd.n = sf.n_points;
+        upload(sd.index, idx);
+        // c0 at the source points (after smoothing)
+        std::vector<double> cpts(sf.n_points, (double)c0s.s);
+        if (c0s.p) {
+            DBuf out;
+            out.alloc(sf.n_points * sizeof(double));
+            hipLaunchKernelGGL((gather_values_kernel<T>), dim3(grid_blocks(sf.n_points)), dim3(256), 0, stream, c0s.p,
+                               sd.index.template as<int64_t>(), sf.n_points, out.as<double>());
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(cpts.data(), out.p, sf.n_points * sizeof(double), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        const double nd = dim;
+        std::vector<T> base(sf.n_points);
+        for (int64_t i = 0; i < sf.n_points; ++i) {
+            const double c = cpts[i];
+            double b;
+            if (is_p) b = sf.mode == KWH_SRC_DIRICHLET ? 1.0 / (nd * c * c) : 2.0 * dt / (nd * c);
+            else b = sf.mode == KWH_SRC_DIRICHLET ? 1.0 : 2.0 * c * dt;
+            base[i] = T(b);
+        }
+        for (int a = 0; a < 3; ++a)
+            sd.factor[a] = (sf.mode == KWH_SRC_DIRICHLET || a >= dim) ? 1.0 : 1.0 / dphys[a];
+        upload(sd.base, base);
+        std::vector<T> sig((size_t)sf.rows * sf.len);
+        for (size_t i = 0; i < sig.size(); ++i) sig[i] = T(sf.signal[i]);
+        upload(sd.signal, sig);
+        if (sf.mode != KWH_SRC_ADDITIVE) {
+            std::vector<int> rp(g.nrows + 1, 0);
+            for (int64_t i = 0; i < sf.n_points; ++i) rp[idx[i] / g.m2 + 1]++;
+            for (int r = 0; r < g.nrows; ++r) rp[r + 1] += rp[r];
+            upload(sd.rowptr, rp);
+        }
+    }
+
+    void build_sources(const kwh_sim_desc& d) {
+        build_source(d.p_source, psrc, true);
+        for (int a = 0; a < dim; ++a) build_source(d.u_source[a], usrc[a], false);
+        for (int a = dim; a < 3; ++a)
+            KWH_REQUIRE(d.u_source[a].signal == nullptr || d.u_source[a].index == nullptr, KWH_EINVAL,
+                        "velocity source component beyond the grid dimension");
+        const bool any_add = psrc.mode == KWH_SRC_ADDITIVE && psrc.present;
+        bool any_u_add = false;
+        for (int a = 0; a < dim; ++a) any_u_add |= usrc[a].present && usrc[a].mode == KWH_SRC_ADDITIVE;
+        if (any_add || any_u_add) {
+            scr_real.alloc(g.N * sizeof(T));
+            scr_spec.alloc(g.NH * sizeof(C));
+        }
+        if (any_add) corr_p.alloc(g.N * sizeof(T));
+        for (int a = 0; a < dim; ++a)
+            if (usrc[a].present && usrc[a].mode == KWH_SRC_ADDITIVE) corr_u[a].alloc(g.N * sizeof(T));
+        // p0 (+ smoothing, :640-645)
+        has_p0 = d.p0.data != nullptr;
+        if (has_p0) {
+            KWH_REQUIRE(d.p0.count == g.N, KWH_EINVAL, "p0 must be grid-shaped");
+            for (int64_t i = 0; i < g.N; ++i)
+                KWH_REQUIRE(std::isfinite(d.p0.data[i]), KWH_EINVAL, "p0 contains non-finite values");
+            std::vector<T> h(d.p0.data, d.p0.data + g.N);
+            upload(p0s, h);
+            if (d.smooth_p0) smooth_inplace(p0s.as<T>());
+        }
+        KWH_REQUIRE(has_p0 || psrc.present || usrc[0].present || usrc[1].present || usrc[2].present ||
+                        d.p_source.index != nullptr || d.u_source[0].index != nullptr,
+                    KWH_EINVAL, "solver requires source.p0 or a time-varying p/u source");
+    }
+
+    SparseSrc<T> sparse_of(const SourceDev<T>& s, int a) const {
+        SparseSrc<T> r{};
+        if (!s.present || s.mode == KWH_SRC_ADDITIVE) return r;
+        r.rowptr = s.rowptr.template as<int>();
+        r.index = s.index.template as<int64_t>();
+        r.signal = s.signal.template as<T>();
+        r.base = s.base.template as<T>();
+        r.rows = s.rows;
+        r.len = s.len;
+        r.mode = s.mode == KWH_SRC_DIRICHLET ? 0 : 2;
+        r.factor = T(s.factor[a]);
+        return r;
+    }
+
+    // Sensor (Recording.swift:8-97, Directivity.swift:40-79).
+    void build_sensor(const kwh_sim_desc& d) {
+        record = d.record;
+        rec_p = record & (KWH_REC_P | KWH_REC_P_MAX | KWH_REC_P_MIN | KWH_REC_P_RMS | KWH_REC_I_AVG);
+        rec_u = record & (KWH_REC_UX | KWH_REC_UY | KWH_REC_UZ | KWH_REC_U_MAX | KWH_REC_U_RMS | KWH_REC_I_AVG);
+        const int64_t N = g.N;
+        std::vector<int64_t> idx;
+        std::vector<T> wts;
+        if (d.sensor_cartesian && d.n_cartesian > 0) {
+            cart = true;
+            nsens = d.n_cartesian;
+            ncorner = 1 << dim;
+            idx.resize(nsens * ncorner);
+            wts.resize(nsens * ncorner);
+            for (int64_t p_ = 0; p_ < nsens; ++p_) {
+                int i0[3];
+                double fr[3];
+                for (int a = 0; a < dim; ++a) {
+                    const double gc = (double)d.sensor_cartesian[a * nsens + p_] / dphys[a] + (double)(nphys[a] / 2);
+                    KWH_REQUIRE(gc >= 0 && gc <= nphys[a] - 1, KWH_EINVAL, "Cartesian sensor point outside grid bounds");
+                    const double f = std::floor(gc);
+                    i0[a] = (int)f;
+                    fr[a] = gc - f;
+                }
+                for (int c = 0; c < ncorner; ++c) {
+                    int64_t flat = 0;
+                    double w = 1.0;
+                    for (int a = 0; a < dim; ++a) {
+                        const int bit = (c >> a) & 1;
+                        const int ai = std::min(i0[a] + bit, nphys[a] - 1);
+                        w *= bit ? fr[a] : (1 - fr[a]);
+                        flat = flat * nphys[a] + ai;
+                    }
+                    idx[p_ * ncorner + c] = flat;
+                    wts[p_ * ncorner + c] = T(w);
+                }
+            }
+        } else if (d.sensor_index && d.n_sensor > 0) {
+            nsens = d.n_sensor;
+            idx.assign(d.sensor_index, d.sensor_index + nsens);
+            for (int64_t i = 0; i < nsens; ++i) {
+                KWH_REQUIRE(idx[i] >= 0 && idx[i] < N, KWH_EINVAL, "sensor index outside the grid");
+                KWH_REQUIRE(i == 0 || idx[i] > idx[i - 1], KWH_EINVAL, "sensor indices must be ascending and unique");
+            }
+        }
+        if (nsens == 0) { rec_p = rec_u = false; return; }
+        upload(sidx, idx);
+        if (cart) upload(sw, wts);
+        store_p = rec_p && (record & (KWH_REC_P | KWH_REC_I_AVG));
+        store_u = rec_u && (record & (KWH_REC_UX | KWH_REC_UY | KWH_REC_UZ | KWH_REC_I_AVG));
+        if (store_p) ser_p.alloc((size_t)nt * nsens * sizeof(T));
+        if (rec_p && (record & KWH_REC_P_MAX)) pmax.alloc(nsens * sizeof(T));
+        if (rec_p && (record & KWH_REC_P_MIN)) pmin.alloc(nsens * sizeof(T));
+        if (rec_p && (record & KWH_REC_P_RMS)) psum2.alloc(nsens * sizeof(double));
+        for (int a = 0; a < dim && rec_u; ++a) {
+            if (store_u) ser_u[a].alloc((size_t)nt * nsens * sizeof(T));
+            if (record & KWH_REC_U_MAX) umax[a].alloc(nsens * sizeof(T));
+            if (record & KWH_REC_U_RMS) usum2[a].alloc(nsens * sizeof(double));
+            ucol[a].alloc(N * sizeof(T));
+        }
+        // 2D directivity on binary masks
+        if (d.directivity_angle && dim == 2 && !cart && rec_p) {
+            std::vector<double> ang(nsens);
+            for (int64_t i = 0; i < nsens; ++i) ang[i] = (double)d.directivity_angle[idx[i]];
+            std::vector<double> uniq(ang);
+            std::sort(uniq.begin(), uniq.end());
+            uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+            const double size = d.directivity_size > 0 ? d.directivity_size : 10.0 * std::max(dphys[0], dphys[1]);
+            ndir = (int)uniq.size();
+            DBuf kx, ky;
+            {
+                auto fx = fft_freq(nphys[0], dphys[0]);
+                auto fy = fft_freq(nphys[1], dphys[1]);
+                for (auto& v : fx) v *= 2 * M_PI;
+                for (auto& v : fy) v *= 2 * M_PI;
+                upload(kx, fx);
+                upload(ky, fy);
+            }
+            for (int k = 0; k < ndir; ++k) {
+                auto w = std::make_unique<DBuf>();
+                w->alloc(g.NH * sizeof(T));
+                DirGeo dg{};
+                dg.nx = nphys[0]; dg.ny = nphys[1]; dg.mzh = g.mzh; dg.NH = g.NH;
+                dg.kx = kx.as<double>(); dg.ky = ky.as<double>();
+                dg.ct = std::cos(uniq[k]); dg.st = std::sin(uniq[k]);
+                dg.half_size = size / 2; dg.gradient = d.directivity_pattern == KWH_DIR_GRADIENT;
+                hipLaunchKernelGGL((directivity_kernel<T>), dim3(grid_blocks(g.NH)), dim3(256), 0, stream, dg, w->as<T>());
+                HIPCHK(hipGetLastError());
+                std::vector<int64_t> gi, gp;
+                for (int64_t i = 0; i < nsens; ++i)
+                    if (ang[i] == uniq[k]) { gi.push_back(idx[i]); gp.push_back(i); }
+                auto bi = std::make_unique<DBuf>();
+                auto bp = std::make_unique<DBuf>();
+                upload(*bi, gi);
+                upload(*bp, gp);
+                dir_count.push_back((int64_t)gi.size());
+                dir_w.push_back(std::move(w));
+                dir_idx.push_back(std::move(bi));
+                dir_pos.push_back(std::move(bp));
+            }
+            HIPCHK(hipStreamSynchronize(stream));
+            dir_field.alloc(N * sizeof(T));
+            if (!scr_spec.p) scr_spec.alloc(g.NH * sizeof(C));
+        }
+    }
+
+    void allocate_state(const kwh_sim_desc&) {
+        const int64_t N = g.N;
+        p.alloc(N * sizeof(T));
+        for (int a = 0; a < dim; ++a) {
+            u[a].alloc(N * sizeof(T));
+            rho[a].alloc(N * sizeof(T));
+        }
+        if (absorbing && !stokes) rhoT.alloc(N * sizeof(T));
+        const int nspec = dim + (rec_u ? dim : 0);
+        for (int k = 0; k < std::max(nspec, 2); ++k) spec[k].alloc(g.NH * sizeof(C));
+        reset_state();
+    }
+
+    void reset_state() {
+        const int64_t N = g.N;
+        fill<T>(p.as<T>(), T(0), N);
+        for (int a = 0; a < dim; ++a) {
+            fill<T>(u[a].as<T>(), T(0), N);
+            fill<T>(rho[a].as<T>(), T(0), N);
+        }
+        fill<C>(spec[0].as<C>(), C{0, 0}, g.NH);
+        P_valid = true;
+        steps = 0;
+        rec_steps = 0;
+    }
+
+    // ------------------------------------------------------------------ the time step
+    void compute_correction(SourceDev<T>& s, int t, T* dst, double factor) {
+        // Re ifft(source_kappa * fft(scatter(v))) (SourceOp.apply :167-171)
+        fill<T>(scr_real.as<T>(), T(0), g.N);
+        timed(K_SC, 0, [&] {
+            hipLaunchKernelGGL((scatter_kernel<T>), dim3(grid_blocks(s.n)), dim3(256), 0, stream, scr_real.as<T>(),
+                               s.index.template as<int64_t>(), s.signal.template as<T>(), s.base.template as<T>(), s.rows, s.len, (int)s.n, t,
+                               T(factor));
+        });
+        forward_to_middle(scr_real.as<T>(), scr_spec.as<C>(), T(1));
+        middle(K_MT, {scr_spec.p}, {MidOp{OP_TAB, 0, 0, scr_spec.p, skappa.p}});
+        inverse_from_middle(scr_spec.as<C>(), dst, T(1), false);
+    }
+
+    void init_step() {
+        // t = 0 override with p0 (:724-733)
+        C* P = spec[0].as<C>();
+        forward_to_middle(p0s.as<T>(), P, T(1));
+        std::vector<MidOp> ops;
+        for (int a = 0; a < dim; ++a)
+            ops.push_back(MidOp{OP_DDXP_KAPPA, a + g.off, 0, spec[a].p, kappa.p});
+        middle(K_MG, {P}, ops);
+        if (plain1) {
+            if (dim == 3) strided(1, true, {spec[0].p, spec[1].p, spec[2].p});
+        }
+        InitArgs<T> ia{};
+        for (int a = 0; a < dim; ++a) {
+            ia.G[a] = spec[a].as<C>();
+            ia.u[a] = u[a].as<T>();
+            ia.rho[a] = rho[a].as<T>();
+            ia.dtr[a] = dtrs[a];
+        }
+        ia.c0 = c0s;
+        ia.p0 = p0s.as<T>();
+        ia.p = p.as<T>();
+        ia.outP = spec[0].as<C>();
+        ZCfg zc = zcfg(2);
+        timed(K_ZI, real_bytes() * (1 + 3 * dim) + spec_bytes() * (dim + 1), [&] {
+            hipLaunchKernelGGL((z_init<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ia, g, zc.zt, tab);
+        });
+        P_valid = true;
+    }
+
+    void main_step(int t) {
+        // k-space-corrected additive sources for this step
+        const T* cu[3] = {nullptr, nullptr, nullptr};
+        for (int a = 0; a < dim; ++a)
+            if (usrc[a].present && usrc[a].mode == KWH_SRC_ADDITIVE && t < usrc[a].len) {
+                compute_correction(usrc[a], t, corr_u[a].as<T>(), usrc[a].factor[a]);
+                cu[a] = corr_u[a].as<T>();
+            }
+        const T* cp = nullptr;
+        if (psrc.present && psrc.mode == KWH_SRC_ADDITIVE && t < psrc.len) {
+            // base scale 2 dt / (dim c); the 1/d_a factor is applied per axis in z_density
+            compute_correction(psrc, t, corr_p.as<T>(), 1.0);
+            cp = corr_p.as<T>();
+        }
+        C* P = spec[0].as<C>();
+        // velocity: grad p (velStep :669-678)
+        if (plain1) strided(1, false, {P});
+        {
+            std::vector<MidOp> ops;
+            for (int a = 0; a < dim; ++a) ops.push_back(MidOp{OP_DDXP_KAPPA, a + g.off, 0, spec[a].p, kappa.p});
+            middle(K_MG, {P}, ops);
+        }
+        if (plain1) strided(1, true, {spec[0].p, spec[1].p, spec[2].p});
+        {
+            VelArgs<T> va{};
+            for (int a = 0; a < dim; ++a) {
+                va.G[a] = spec[a].as<C>();
+                va.u[a] = u[a].as<T>();
+                va.dtr[a] = dtrs[a];
+                va.corr[a] = cu[a];
+                va.sp[a] = sparse_of(usrc[a], a);
+            }
+            va.t = t;
+            ZCfg zc = zcfg(2);
+            timed(K_ZV, (real_bytes() * (2 + (dtrs[0].p ? 1 : 0)) + 2 * spec_bytes()) * dim, [&] {
+                hipLaunchKernelGGL((z_velocity<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, va, g, zc.zt, tab);
+            });
+        }
+        // divergence (rhoStep :680-692)
+        if (plain1) strided(1, false, {spec[0].p, spec[1].p, spec[2].p});
+        {
+            std::vector<const void*> ins;
+            std::vector<MidOp> ops;
+            for (int a = 0; a < dim; ++a) {
+                ins.push_back(spec[a].p);
+                ops.push_back(MidOp{OP_DDXN_KAPPA, a + g.off, a, spec[a].p, kappa.p});
+                if (rec_u) ops.push_back(MidOp{OP_COLL, a + g.off, a, spec[dim + a].p, nullptr});
+            }
+            middle(K_MD, ins, ops);
+        }
+        if (plain1) {
+            strided(1, true, {spec[0].p, spec[1].p, spec[2].p});
+            if (rec_u) strided(1, true, {spec[3].p, spec[4].p, spec[5].p});
+        }
+        if (rec_u)
+            for (int a = 0; a < dim; ++a) {
+                ZCfg zc = zcfg(2);
+                C* s = spec[dim + a].as<C>();
+                T* dst = ucol[a].as<T>();
+                timed(K_ZIR, real_bytes() + spec_bytes(), [&] {
+                    hipLaunchKernelGGL((z_inverse_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, s, dst, T(1),
+                                       0, g, zc.zt, tab);
+                });
+            }
+        const bool fft_abs = absorbing && !stokes;
+        {
+            RhoArgs<T> ra{};
+            for (int a = 0; a < dim; ++a) {
+                ra.D[a] = spec[a].as<C>();
+                ra.rho[a] = rho[a].as<T>();
+                ra.corr_factor[a] = T(psrc.factor[a]);
+                ra.sp[a] = sparse_of(psrc, a);
+            }
+            ra.rho0 = rho0s;
+            ra.c0 = c0s;
+            ra.bona = bonas;
+            ra.tau = taus;
+            ra.corr = cp;
+            ra.p = p.as<T>();
+            ra.outP = spec[0].as<C>();
+            ra.rhoT = rhoT.as<T>();
+            ra.F1 = spec[0].as<C>();
+            ra.F2 = spec[1].as<C>();
+            ra.dt = T(dt);
+            ra.mode = fft_abs ? 1 : 0;
+            ra.nonlinear = nonlinear;
+            ra.stokes = stokes;
+            ra.dispersion = dispersion;
+            ra.t = t;
+            const int nbuf = 4 + (nonlinear ? 1 : 0);
+            ZCfg zc = zcfg(nbuf);
+            timed(K_ZD, real_bytes() * (3 * dim + 2) + spec_bytes() * (dim + 1), [&] {
+                hipLaunchKernelGGL((z_density<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ra, g, zc.zt, tab);
+            });
+        }
+        if (fft_abs) {
+            if (plain1) {
+                if (dispersion) strided(1, false, {spec[0].p, spec[1].p});
+                else strided(1, false, {spec[0].p});
+            }
+            std::vector<const void*> ins = {spec[0].p};
+            std::vector<MidOp> ops = {MidOp{OP_TAB, 0, 0, spec[0].p, nab1.p}};
+            if (dispersion) {
+                ins.push_back(spec[1].p);
+                ops.push_back(MidOp{OP_TAB, 0, 1, spec[1].p, nab2.p});
+            }
+            middle(K_MA, ins, ops);
+            if (plain1) {
+                if (dispersion) strided(1, true, {spec[0].p, spec[1].p});
+                else strided(1, true, {spec[0].p});
+            }
+            EosArgs<T> ea{};
+            ea.A1 = spec[0].as<C>();
+            ea.A2 = dispersion ? spec[1].as<C>() : nullptr;
+            ea.rhoT = rhoT.as<T>();
+            ea.c0 = c0s;
+            ea.rho0 = rho0s;
+            ea.tau = taus;
+            ea.eta = etas;
+            ea.bona = bonas;
+            ea.nonlinear = nonlinear;
+            ea.p = p.as<T>();
+            ea.outP = spec[0].as<C>();
+            ZCfg zc = zcfg(3);
+            timed(K_ZE, real_bytes() * 5 + spec_bytes() * 3, [&] {
+                hipLaunchKernelGGL((z_eos<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ea, g, zc.zt, tab);
+            });
+        }
+        P_valid = true;
+    }
+
+    void record_step(int t) {
+        if (!(rec_p || rec_u)) return;
+        const int first = rec_steps == 0;
+        const int slot = rec_steps;
+        if (rec_p) {
+            if (ndir > 0) {
+                // directivity: per unique angle Re ifft2(fft2(p) W) sampled at that angle's rows
+                forward_to_middle(p.as<T>(), scr_spec.as<C>(), T(1));
+                for (int k = 0; k < ndir; ++k) {
+                    C* tmp = spec[dim > 1 ? 1 : 0].as<C>();  // free at this point of the step
+                    middle(K_MT, {scr_spec.p}, {MidOp{OP_TAB, 0, 0, tmp, dir_w[k]->p}});
+                    inverse_from_middle(tmp, dir_field.as<T>(), T(1), false);
+                    gather(dir_field.as<T>(), dir_idx[k]->as<int64_t>(), nullptr, 1, dir_count[k], ser_p.as<T>(),
+                           pmax.as<T>(), pmin.as<T>(), psum2.as<double>(), slot, first, dir_pos[k]->as<int64_t>());
+                }
+            } else {
+                gather(p.as<T>(), sidx.as<int64_t>(), sw.as<T>(), ncorner, nsens, ser_p.as<T>(), pmax.as<T>(),
+                       pmin.as<T>(), psum2.as<double>(), slot, first, nullptr);
+            }
+        }
+        if (rec_u)
+            for (int a = 0; a < dim; ++a)
+                gather(ucol[a].as<T>(), sidx.as<int64_t>(), sw.as<T>(), ncorner, nsens, ser_u[a].as<T>(), umax[a].as<T>(),
+                       nullptr, usum2[a].as<double>(), slot, first, nullptr);
+        rec_steps++;
+    }
+
+    void gather(const T* field, const int64_t* idx, const T* w, int nc, int64_t n, T* series, T* vmax, T* vmin,
+                double* sum2, int slot, int first, const int64_t* outpos) {
+        if (outpos) {
+            // subset gather: write through positions (directivity rows)
+            timed(K_GA, 0, [&] {
+                hipLaunchKernelGGL((gather_pos_kernel<T>), dim3(grid_blocks(n)), dim3(256), 0, stream, field, idx, outpos, n,
+                                   nsens, series, vmax, vmin, sum2, slot, first);
+            });
+            return;
+        }
+        GatherArgs<T> a{field, idx, w, nc, n, series, vmax, vmin, sum2, slot, first};
+        timed(K_GA, (double)n * sizeof(T) * 2, [&] {
+            hipLaunchKernelGGL((gather_kernel<T>), dim3(grid_blocks(n)), dim3(256), 0, stream, a);
+        });
+    }
+
+    void step(int t) {
+        if (t == 0 && has_p0) init_step();
+        else main_step(t);
+        record_step(t);
+        steps = t + 1;
+    }
+
+    // ------------------------------------------------------------------ API
+    void check_range(int t0, int t1) {
+        KWH_REQUIRE(t0 == steps, KWH_ESTATE, "runs must be contiguous: t_begin must equal the steps already run");
+        KWH_REQUIRE(t1 >= t0 && t1 <= nt, KWH_EINVAL, "t_end must be in [t_begin, nt]");
+    }
+
+    void launch(int t0, int t1) override {
+        check_range(t0, t1);
+        HIPCHK(hipSetDevice(dev));
+        for (int t = t0; t < t1; ++t) step(t);
+    }
+    void sync() override {
+        HIPCHK(hipStreamSynchronize(stream));
+        if (timing) collect_timing();
+    }
+    void run(int t0, int t1, kwh_step_cb cb, int interval, void* user) override {
+        check_range(t0, t1);
+        if (!cb) {
+            launch(t0, t1);
+            sync();
+            return;
+        }
+        const int iv = std::max(interval, 1);
+        std::vector<T> host;
+        for (int t = t0; t < t1; ++t) {
+            launch(t, t + 1);
+            const bool due = (t % iv) == 0;
+            if (due) {
+                host.resize(g.N);
+                HIPCHK(hipMemcpyAsync(host.data(), p.p, g.N * sizeof(T), hipMemcpyDeviceToHost, stream));
+            }
+            sync();
+            cb(t, nt, due ? host.data() : nullptr, user);
+        }
+    }
+
+    int steps_done() const override { return steps; }
+
+    void output_shape(int field, int64_t shape[3], int* ndim) override {
+        shape[0] = shape[1] = shape[2] = 1;
+        auto grid = [&] { *ndim = dim; for (int a = 0; a < dim; ++a) shape[a] = nphys[a]; };
+        auto series = [&] { *ndim = 2; shape[0] = nsens; shape[1] = rec_steps; };
+        auto agg = [&] { *ndim = 1; shape[0] = nsens; };
+        const int uc = (field >= KWH_OUT_UX && field <= KWH_OUT_IZ_AVG) ? ((field - KWH_OUT_UX) % 3) : 0;
+        switch (field) {
+            case KWH_OUT_P_FINAL: grid(); return;
+            case KWH_OUT_P: KWH_REQUIRE(store_p && (record & KWH_REC_P), KWH_EINVAL, "p not recorded"); series(); return;
+            case KWH_OUT_P_MAX: KWH_REQUIRE(pmax.p, KWH_EINVAL, "pMax not recorded"); agg(); return;
+            case KWH_OUT_P_MIN: KWH_REQUIRE(pmin.p, KWH_EINVAL, "pMin not recorded"); agg(); return;
+            case KWH_OUT_P_RMS: KWH_REQUIRE(psum2.p, KWH_EINVAL, "pRms not recorded"); agg(); return;
+            case KWH_OUT_UX: case KWH_OUT_UY: case KWH_OUT_UZ:
+                KWH_REQUIRE(uc < dim && ser_u[uc].p && (record & (KWH_REC_UX << uc)), KWH_EINVAL, "velocity series not recorded");
+                series(); return;
+            case KWH_OUT_UX_MAX: case KWH_OUT_UY_MAX: case KWH_OUT_UZ_MAX: {
+                const int c = field - KWH_OUT_UX_MAX;
+                KWH_REQUIRE(c < dim && umax[c].p, KWH_EINVAL, "uMax not recorded"); agg(); return; }
+            case KWH_OUT_UX_RMS: case KWH_OUT_UY_RMS: case KWH_OUT_UZ_RMS: {
+                const int c = field - KWH_OUT_UX_RMS;
+                KWH_REQUIRE(c < dim && usum2[c].p, KWH_EINVAL, "uRms not recorded"); agg(); return; }
+            case KWH_OUT_UX_FINAL: case KWH_OUT_UY_FINAL: case KWH_OUT_UZ_FINAL: {
+                const int c = field - KWH_OUT_UX_FINAL;
+                KWH_REQUIRE(c < dim && (record & KWH_REC_U_FINAL), KWH_EINVAL, "uFinal not recorded"); grid(); return; }
+            case KWH_OUT_IX_AVG: case KWH_OUT_IY_AVG: case KWH_OUT_IZ_AVG: {
+                const int c = field - KWH_OUT_IX_AVG;
+                KWH_REQUIRE(c < dim && (record & KWH_REC_I_AVG) && ser_p.p && ser_u[c].p, KWH_EINVAL, "iAvg not recorded");
+                agg(); return; }
+            default: throw KwhError(KWH_EINVAL, "unknown output field");
+        }
+    }
+
+    void read_output(int field, void* dst, size_t bytes) override {
+        int64_t shape[3];
+        int nd = 0;
+        output_shape(field, shape, &nd);
+        int64_t count = 1;
+        for (int i = 0; i < nd; ++i) count *= shape[i];
+        KWH_REQUIRE(bytes >= (size_t)count * sizeof(T), KWH_EINVAL, "destination buffer too small");
+        HIPCHK(hipSetDevice(dev));
+        sync();
+        auto d2h = [&](const void* src, size_t b) {
+            HIPCHK(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        };
+        auto series_out = [&](const DBuf& s) {
+            std::vector<T> h((size_t)rec_steps * nsens);
+            HIPCHK(hipMemcpyAsync(h.data(), s.p, h.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            T* o = reinterpret_cast<T*>(dst);
+            for (int64_t s_ = 0; s_ < nsens; ++s_)
+                for (int t = 0; t < rec_steps; ++t) o[s_ * rec_steps + t] = h[(size_t)t * nsens + s_];
+        };
+        auto rms_out = [&](const DBuf& s2) {
+            DBuf tmp;
+            tmp.alloc(nsens * sizeof(T));
+            hipLaunchKernelGGL((rms_kernel<T>), dim3(grid_blocks(nsens)), dim3(256), 0, stream, s2.as<double>(), tmp.as<T>(),
+                               nsens, 1.0 / std::max(rec_steps, 1));
+            HIPCHK(hipGetLastError());
+            d2h(tmp.p, nsens * sizeof(T));
+        };
+        switch (field) {
+            case KWH_OUT_P_FINAL: d2h(p.p, g.N * sizeof(T)); return;
+            case KWH_OUT_P: series_out(ser_p); return;
+            case KWH_OUT_P_MAX: d2h(pmax.p, nsens * sizeof(T)); return;
+            case KWH_OUT_P_MIN: d2h(pmin.p, nsens * sizeof(T)); return;
+            case KWH_OUT_P_RMS: rms_out(psum2); return;
+            case KWH_OUT_UX: case KWH_OUT_UY: case KWH_OUT_UZ: series_out(ser_u[field - KWH_OUT_UX]); return;
+            case KWH_OUT_UX_MAX: case KWH_OUT_UY_MAX: case KWH_OUT_UZ_MAX: d2h(umax[field - KWH_OUT_UX_MAX].p, nsens * sizeof(T)); return;
+            case KWH_OUT_UX_RMS: case KWH_OUT_UY_RMS: case KWH_OUT_UZ_RMS: rms_out(usum2[field - KWH_OUT_UX_RMS]); return;
+            case KWH_OUT_UX_FINAL: case KWH_OUT_UY_FINAL: case KWH_OUT_UZ_FINAL: d2h(u[field - KWH_OUT_UX_FINAL].p, g.N * sizeof(T)); return;
+            case KWH_OUT_IX_AVG: case KWH_OUT_IY_AVG: case KWH_OUT_IZ_AVG: {
+                const int c = field - KWH_OUT_IX_AVG;
+                DBuf tmp;
+                tmp.alloc(nsens * sizeof(T));
+                hipLaunchKernelGGL((intensity_kernel<T>), dim3(grid_blocks(nsens, 64)), dim3(64), 0, stream, ser_p.as<T>(),
+                                   ser_u[c].as<T>(), tmp.as<T>(), nsens, rec_steps);
+                HIPCHK(hipGetLastError());
+                d2h(tmp.p, nsens * sizeof(T));
+                return;
+            }
+            default: throw KwhError(KWH_EINVAL, "unknown output field");
+        }
+    }
+
+    // State = p, u, rho, sensor accumulators and series, counters.
+    std::vector<std::pair<void*, size_t>> state_parts() {
+        std::vector<std::pair<void*, size_t>> v;
+        auto add = [&](DBuf& b) { if (b.p) v.push_back({b.p, b.bytes}); };
+        add(p);
+        for (int a = 0; a < dim; ++a) { add(u[a]); add(rho[a]); }
+        add(ser_p); add(pmax); add(pmin); add(psum2);
+        for (int a = 0; a < dim; ++a) { add(ser_u[a]); add(umax[a]); add(usum2[a]); }
+        return v;
+    }
+    size_t state_bytes() override {
+        size_t b = 2 * sizeof(int32_t);
+        for (auto& pr : state_parts()) b += pr.second;
+        return b;
+    }
+    void save_state(void* dst, size_t bytes) override {
+        KWH_REQUIRE(bytes >= state_bytes(), KWH_EINVAL, "state buffer too small");
+        sync();
+        char* o = reinterpret_cast<char*>(dst);
+        int32_t hdr[2] = {steps, rec_steps};
+        std::memcpy(o, hdr, sizeof(hdr));
+        o += sizeof(hdr);
+        for (auto& pr : state_parts()) {
+            HIPCHK(hipMemcpyAsync(o, pr.first, pr.second, hipMemcpyDeviceToHost, stream));
+            o += pr.second;
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    void load_state(const void* src, size_t bytes) override {
+        KWH_REQUIRE(bytes >= state_bytes(), KWH_EINVAL, "state buffer too small");
+        sync();
+        const char* o = reinterpret_cast<const char*>(src);
+        int32_t hdr[2];
+        std::memcpy(hdr, o, sizeof(hdr));
+        KWH_REQUIRE(hdr[0] >= 0 && hdr[0] <= nt, KWH_EINVAL, "state step counter out of range");
+        o += sizeof(hdr);
+        for (auto& pr : state_parts()) {
+            HIPCHK(hipMemcpyAsync(pr.first, o, pr.second, hipMemcpyHostToDevice, stream));
+            o += pr.second;
+        }
+        steps = hdr[0];
+        rec_steps = hdr[1];
+        // P = R2C_z(p) is derived state
+        ZCfg zc = zcfg(2);
+        hipLaunchKernelGGL((z_forward_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, p.as<T>(),
+                           spec[0].as<C>(), T(1), g, zc.zt, tab);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void set_timing(bool on) override {
+        sync();
+        timing = on;
+        for (int i = 0; i < kNames; ++i) { stat_launch[i] = 0; stat_ms[i] = 0; stat_bytes[i] = 0; }
+    }
+    int kernel_stats(kwh_kernel_stat* st, int cap) override {
+        sync();
+        int n = 0;
+        for (int i = 0; i < kNames; ++i) {
+            if (!stat_launch[i]) continue;
+            if (n < cap) {
+                std::snprintf(st[n].name, sizeof(st[n].name), "%s", kKernelNames[i]);
+                st[n].launches = stat_launch[i];
+                st[n].total_ms = stat_ms[i];
+                st[n].bytes_per_launch = stat_bytes[i];
+            }
+            ++n;
+        }
+        return n;
+    }
+    void set_graph(bool on) override { use_graph = on; }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------------
+
+template <typename F>
+static int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return KWH_OK;
+    } catch (const KwhError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return KWH_ENOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return KWH_EDEVICE;
+    }
+}
+
+static void validate_desc(const kwh_sim_desc* d) {
+    KWH_REQUIRE(d != nullptr, KWH_EINVAL, "descriptor is NULL");
+    KWH_REQUIRE(d->abi_version == KWH_ABI_VERSION, KWH_EINVAL, "descriptor ABI version mismatch");
+    KWH_REQUIRE(d->dim >= 1 && d->dim <= 3, KWH_EUNSUPPORTED, "kspaceFirstOrder: dim must be 1, 2 or 3");
+    for (int a = 0; a < d->dim; ++a) {
+        KWH_REQUIRE(d->n[a] >= 1, KWH_EINVAL, "grid sizes must be >= 1");
+        KWH_REQUIRE(d->d[a] > 0 && std::isfinite(d->d[a]), KWH_EINVAL, "grid spacing must be positive");
+    }
+    KWH_REQUIRE(d->nt > 0, KWH_EINVAL, "call grid.makeTime(...) before running the solver");
+    KWH_REQUIRE(d->dt > 0 && std::isfinite(d->dt), KWH_EINVAL, "dt must be positive");
+    KWH_REQUIRE(d->dtype == KWH_FLOAT32 || d->dtype == KWH_FLOAT64, KWH_EINVAL, "dtype must be float32 or float64");
+    KWH_REQUIRE(d->sound_speed.data && d->sound_speed.count >= 1, KWH_EINVAL, "soundSpeed is required");
+    KWH_REQUIRE(d->density.data && d->density.count >= 1, KWH_EINVAL, "density is required");
+    for (int64_t i = 0; i < d->sound_speed.count; ++i)
+        KWH_REQUIRE(d->sound_speed.data[i] > 0 && std::isfinite(d->sound_speed.data[i]), KWH_EINVAL,
+                    "soundSpeed must be positive and finite");
+    for (int64_t i = 0; i < d->density.count; ++i)
+        KWH_REQUIRE(d->density.data[i] > 0 && std::isfinite(d->density.data[i]), KWH_EINVAL,
+                    "density must be positive and finite");
+    KWH_REQUIRE(d->alpha_mode >= 0 && d->alpha_mode <= 3, KWH_EINVAL, "invalid alpha mode");
+    KWH_REQUIRE(d->pml_size >= 0, KWH_EINVAL, "pmlSize must be >= 0");
+    KWH_REQUIRE(std::isfinite(d->pml_alpha), KWH_EINVAL, "pmlAlpha must be finite");
+    int64_t N = 1;
+    for (int a = 0; a < d->dim; ++a) {
+        N *= d->n[a];
+        KWH_REQUIRE(d->pml_size <= d->n[a], KWH_EINVAL, "pmlSize must not exceed the grid size");
+    }
+    auto field_ok = [&](const kwh_field& f, const char* name) {
+        if (!f.data) return;
+        KWH_REQUIRE(f.count == 1 || f.count == N, KWH_EINVAL, std::string(name) + " must be scalar or grid-shaped");
+    };
+    field_ok(d->sound_speed, "soundSpeed");
+    field_ok(d->density, "density");
+    field_ok(d->alpha_coeff, "alphaCoeff");
+    field_ok(d->bona, "bOnA");
+    if (d->alpha_coeff.data && d->alpha_mode != KWH_ALPHA_NONE && d->alpha_mode != KWH_ALPHA_STOKES)
+        KWH_REQUIRE(std::isfinite(d->alpha_power), KWH_EINVAL, "alphaPower must be finite");
+    if (d->p0.data) {
+        KWH_REQUIRE(d->p0.count == N, KWH_EINVAL, "p0 must be grid-shaped");
+        for (int64_t i = 0; i < N; ++i)
+            KWH_REQUIRE(std::isfinite(d->p0.data[i]), KWH_EINVAL, "p0 contains non-finite values");
+    }
+    auto src_ok = [&](const kwh_source_field& s, const char* name) {
+        if (!s.index || !s.signal || s.n_points <= 0) return;
+        KWH_REQUIRE(s.rows == 1 || s.rows == s.n_points, KWH_EINVAL,
+                    std::string(name) + ": signal must have 1 row or one row per source point");
+        KWH_REQUIRE(s.len >= 0, KWH_EINVAL, std::string(name) + ": signal length must be >= 0");
+        KWH_REQUIRE(s.mode >= 0 && s.mode <= 2, KWH_EINVAL, std::string(name) + ": invalid source mode");
+        for (int64_t i = 0; i < s.n_points; ++i) {
+            KWH_REQUIRE(s.index[i] >= 0 && s.index[i] < N, KWH_EINVAL, std::string(name) + ": index outside the grid");
+            KWH_REQUIRE(i == 0 || s.index[i] > s.index[i - 1], KWH_EINVAL,
+                        std::string(name) + ": indices must be ascending and unique");
+        }
+        const int64_t n = (int64_t)s.rows * s.len;
+        for (int64_t i = 0; i < n; ++i)
+            KWH_REQUIRE(std::isfinite(s.signal[i]), KWH_EINVAL, std::string(name) + ": signal contains non-finite values");
+    };
+    src_ok(d->p_source, "p source");
+    src_ok(d->u_source[0], "ux source");
+    src_ok(d->u_source[1], "uy source");
+    src_ok(d->u_source[2], "uz source");
+    for (int a = d->dim; a < 3; ++a)
+        KWH_REQUIRE(!(d->u_source[a].index && d->u_source[a].signal), KWH_EINVAL,
+                    "velocity source component beyond the grid dimension");
+    const bool any_src = d->p0.data || (d->p_source.index && d->p_source.signal) ||
+                         (d->u_source[0].index && d->u_source[0].signal) ||
+                         (d->u_source[1].index && d->u_source[1].signal) ||
+                         (d->u_source[2].index && d->u_source[2].signal);
+    KWH_REQUIRE(any_src, KWH_EINVAL, "solver requires source.p0 or a time-varying p/u source");
+    if (d->sensor_index && d->n_sensor > 0) {
+        for (int64_t i = 0; i < d->n_sensor; ++i) {
+            KWH_REQUIRE(d->sensor_index[i] >= 0 && d->sensor_index[i] < N, KWH_EINVAL, "sensor index outside the grid");
+            KWH_REQUIRE(i == 0 || d->sensor_index[i] > d->sensor_index[i - 1], KWH_EINVAL,
+                        "sensor indices must be ascending and unique");
+        }
+    }
+    if (d->sensor_cartesian && d->n_cartesian > 0) {
+        for (int64_t p = 0; p < d->n_cartesian; ++p)
+            for (int a = 0; a < d->dim; ++a) {
+                const double gc = (double)d->sensor_cartesian[a * d->n_cartesian + p] / d->d[a] + (double)(d->n[a] / 2);
+                KWH_REQUIRE(gc >= 0 && gc <= d->n[a] - 1, KWH_EINVAL, "Cartesian sensor point outside grid bounds");
+            }
+    }
+}
+
+extern "C" {
+
+int kwh_abi_version(void) { return KWH_ABI_VERSION; }
+
+const char* kwh_last_error(void) { return g_last_error.c_str(); }
+
+int kwh_ctx_create(int device_ordinal, kwh_ctx** out) {
+    return guarded([&] {
+        KWH_REQUIRE(out != nullptr, KWH_EINVAL, "out is NULL");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        KWH_REQUIRE(e == hipSuccess && n > 0, KWH_EDEVICE, "no HIP device available");
+        KWH_REQUIRE(device_ordinal >= 0 && device_ordinal < n, KWH_EINVAL, "device ordinal out of range");
+        HIPCHK(hipSetDevice(device_ordinal));
+        *out = new kwh_ctx{device_ordinal};
+    });
+}
+
+void kwh_ctx_destroy(kwh_ctx* ctx) { delete ctx; }
+
+int kwh_sim_validate(const kwh_sim_desc* desc) {
+    return guarded([&] { validate_desc(desc); });
+}
+
+int kwh_sim_create(kwh_ctx* ctx, const kwh_sim_desc* desc, kwh_sim** out) {
+    return guarded([&] {
+        KWH_REQUIRE(ctx && out, KWH_EINVAL, "ctx/out is NULL");
+        *out = nullptr;
+        validate_desc(desc);
+        if (desc->dtype == KWH_FLOAT32) {
+            auto s = std::make_unique<SimT<float>>();
+            s->setup(*desc, ctx->device);
+            *out = s.release();
+        } else {
+            auto s = std::make_unique<SimT<double>>();
+            s->setup(*desc, ctx->device);
+            *out = s.release();
+        }
+    });
+}
+
+int kwh_sim_run(kwh_sim* sim, int32_t t_begin, int32_t t_end, kwh_step_cb cb, int32_t cb_interval, void* user) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        sim->run(t_begin, t_end, cb, cb_interval, user);
+    });
+}
+
+int kwh_sim_launch(kwh_sim* sim, int32_t t_begin, int32_t t_end) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        sim->launch(t_begin, t_end);
+    });
+}
+
+int kwh_sim_sync(kwh_sim* sim) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        sim->sync();
+    });
+}
+
+int kwh_sim_output_shape(kwh_sim* sim, int32_t field, int64_t shape[3], int32_t* ndim) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && shape && ndim, KWH_EINVAL, "NULL argument");
+        int nd = 0;
+        sim->output_shape(field, shape, &nd);
+        *ndim = nd;
+    });
+}
+
+int kwh_sim_read_output(kwh_sim* sim, int32_t field, void* dst, size_t dst_bytes) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && dst, KWH_EINVAL, "NULL argument");
+        sim->read_output(field, dst, dst_bytes);
+    });
+}
+
+int kwh_sim_steps_done(kwh_sim* sim, int32_t* steps) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && steps, KWH_EINVAL, "NULL argument");
+        *steps = sim->steps_done();
+    });
+}
+
+int kwh_sim_state_bytes(kwh_sim* sim, size_t* bytes) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && bytes, KWH_EINVAL, "NULL argument");
+        *bytes = sim->state_bytes();
+    });
+}
+
+int kwh_sim_save_state(kwh_sim* sim, void* dst, size_t bytes) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && dst, KWH_EINVAL, "NULL argument");
+        sim->save_state(dst, bytes);
+    });
+}
+
+int kwh_sim_load_state(kwh_sim* sim, const void* src, size_t bytes) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && src, KWH_EINVAL, "NULL argument");
+        sim->load_state(src, bytes);
+    });
+}
+
+int kwh_sim_set_kernel_timing(kwh_sim* sim, int enable) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        sim->set_timing(enable != 0);
+    });
+}
+
+int kwh_sim_kernel_stats(kwh_sim* sim, kwh_kernel_stat* stats, int32_t capacity, int32_t* count) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && count, KWH_EINVAL, "NULL argument");
+        *count = sim->kernel_stats(stats, stats ? capacity : 0);
+    });
+}
+
+int kwh_sim_set_graph(kwh_sim* sim, int enable) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        sim->set_graph(enable != 0);
+    });
+}
+
+void kwh_sim_destroy(kwh_sim* sim) { delete sim; }
+
+}  // extern "C"
