@@ -217,6 +217,8 @@ struct SimT final : kwh_sim {
     DBuf sidx, sw;
     uint32_t record = 0;
     bool rec_p = false, rec_u = false, store_p = false, store_u = false;
+    bool need_u[3] = {false, false, false};   // collocated component a is computed each step
+    int ucol_slot[3] = {-1, -1, -1};          // spectrum buffer index holding its colloc spectrum
     DBuf ser_p, ser_u[3], pmax, pmin, psum2, umax[3], usum2[3];
     int rec_steps = 0;
     // directivity
@@ -939,11 +941,18 @@ struct SimT final : kwh_sim {
         if (rec_p && (record & KWH_REC_P_MAX)) pmax.alloc(nsens * sizeof(T));
         if (rec_p && (record & KWH_REC_P_MIN)) pmin.alloc(nsens * sizeof(T));
         if (rec_p && (record & KWH_REC_P_RMS)) psum2.alloc(nsens * sizeof(double));
+        // The reference samples every component when any velocity output is requested
+        // (Recording.swift:15-17); only the components an output reads are computed here.
+        const bool all_u = record & (KWH_REC_U_MAX | KWH_REC_U_RMS | KWH_REC_I_AVG);
+        int slot = dim;
         for (int a = 0; a < dim && rec_u; ++a) {
-            if (store_u) ser_u[a].alloc((size_t)nt * nsens * sizeof(T));
+            need_u[a] = all_u || (record & (KWH_REC_UX << a));
+            if (!need_u[a]) continue;
+            if (record & ((KWH_REC_UX << a) | KWH_REC_I_AVG)) ser_u[a].alloc((size_t)nt * nsens * sizeof(T));
             if (record & KWH_REC_U_MAX) umax[a].alloc(nsens * sizeof(T));
             if (record & KWH_REC_U_RMS) usum2[a].alloc(nsens * sizeof(double));
             ucol[a].alloc(N * sizeof(T));
+            ucol_slot[a] = slot++;
         }
         // 2D directivity on binary masks
         if (d.directivity_angle && dim == 2 && !cart && rec_p) {
@@ -999,7 +1008,8 @@ struct SimT final : kwh_sim {
             rho[a].alloc(N * sizeof(T));
         }
         if (absorbing && !stokes) rhoT.alloc(N * sizeof(T));
-        const int nspec = dim + (rec_u ? dim : 0);
+        int nspec = dim;
+        for (int a = 0; a < dim; ++a) if (need_u[a]) ++nspec;
         for (int k = 0; k < std::max(nspec, 2); ++k) spec[k].alloc(g.NH * sizeof(C));
         reset_state();
     }
@@ -1057,6 +1067,14 @@ struct SimT final : kwh_sim {
         timed(K_ZI, real_bytes() * (1 + 3 * dim) + spec_bytes() * (dim + 1), [&] {
             hipLaunchKernelGGL((z_init<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ia, g, zc.zt, tab);
         });
+        // collocated velocity of the initial state for the t = 0 sample (:735-745)
+        for (int a = 0; a < dim; ++a) {
+            if (!need_u[a]) continue;
+            C* s = spec[ucol_slot[a]].as<C>();
+            forward_to_middle(u[a].as<T>(), s, T(1));
+            middle(K_MT, {s}, {MidOp{OP_COLL, a + g.off, 0, s, nullptr}});
+            inverse_from_middle(s, ucol[a].as<T>(), T(1), false);
+        }
         P_valid = true;
     }
 
@@ -1106,18 +1124,19 @@ struct SimT final : kwh_sim {
             for (int a = 0; a < dim; ++a) {
                 ins.push_back(spec[a].p);
                 ops.push_back(MidOp{OP_DDXN_KAPPA, a + g.off, a, spec[a].p, kappa.p});
-                if (rec_u) ops.push_back(MidOp{OP_COLL, a + g.off, a, spec[dim + a].p, nullptr});
+                if (need_u[a]) ops.push_back(MidOp{OP_COLL, a + g.off, a, spec[ucol_slot[a]].p, nullptr});
             }
             middle(K_MD, ins, ops);
         }
         if (plain1) {
             strided(1, true, {spec[0].p, spec[1].p, spec[2].p});
-            if (rec_u) strided(1, true, {spec[3].p, spec[4].p, spec[5].p});
+            for (int a = 0; a < dim; ++a)
+                if (need_u[a]) strided(1, true, {spec[ucol_slot[a]].p});
         }
-        if (rec_u)
-            for (int a = 0; a < dim; ++a) {
+        for (int a = 0; a < dim; ++a)
+            if (need_u[a]) {
                 ZCfg zc = zcfg(2);
-                C* s = spec[dim + a].as<C>();
+                C* s = spec[ucol_slot[a]].as<C>();
                 T* dst = ucol[a].as<T>();
                 timed(K_ZIR, real_bytes() + spec_bytes(), [&] {
                     hipLaunchKernelGGL((z_inverse_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, s, dst, T(1),
@@ -1211,8 +1230,8 @@ struct SimT final : kwh_sim {
                        pmin.as<T>(), psum2.as<double>(), slot, first, nullptr);
             }
         }
-        if (rec_u)
-            for (int a = 0; a < dim; ++a)
+        for (int a = 0; a < dim; ++a)
+            if (need_u[a])
                 gather(ucol[a].as<T>(), sidx.as<int64_t>(), sw.as<T>(), ncorner, nsens, ser_u[a].as<T>(), umax[a].as<T>(),
                        nullptr, usum2[a].as<double>(), slot, first, nullptr);
         rec_steps++;

@@ -1,0 +1,190 @@
+// KSpaceFirstOrderHIP.swift -- the `.hip` device path of kspaceFirstOrder for KWaveSwift.
+//
+// Drop into Sources/KWave/Solver/ next to KSpaceFirstOrder.swift, add `case hip` to DeviceKind
+// (Sources/KWave/KWave.swift:12-22) and, in kspaceFirstOrder (KSpaceFirstOrder.swift:94), before
+// `switch grid.dim`:
+//     if options.device == .hip {
+//         return kspaceFirstOrderHIP(grid: grid, medium: medium, source: source, sensor: sensor, options: options)
+//     }
+// Package.swift gains `.systemLibrary(name: "CKWaveHIP", path: "Sources/CKWaveHIP", pkgConfig: "kwavehip")`
+// (the CHDF5 pattern, Package.swift:18-22) and KWave depends on it.
+//
+// Not compiled in this repository (no Swift toolchain here); the C ABI it calls is the tested
+// contract (tests/test_abi.py, tests/test_gpu_parity.py drive the same entry points via ctypes).
+
+import Foundation
+import MLX
+import CKWaveHIP
+
+/// Keeps host copies alive while the descriptor points into them.
+private final class HostArena {
+    private var floats: [UnsafeMutableBufferPointer<Float>] = []
+    private var ints: [UnsafeMutableBufferPointer<Int64>] = []
+    deinit {
+        floats.forEach { $0.deallocate() }
+        ints.forEach { $0.deallocate() }
+    }
+    func copy(_ a: [Float]) -> UnsafeMutablePointer<Float> {
+        let b = UnsafeMutableBufferPointer<Float>.allocate(capacity: max(a.count, 1))
+        _ = b.initialize(from: a)
+        floats.append(b)
+        return b.baseAddress!
+    }
+    func copy(_ a: [Int64]) -> UnsafeMutablePointer<Int64> {
+        let b = UnsafeMutableBufferPointer<Int64>.allocate(capacity: max(a.count, 1))
+        _ = b.initialize(from: a)
+        ints.append(b)
+        return b.baseAddress!
+    }
+}
+
+private func hostFloats(_ a: MLXArray) -> [Float] {
+    a.asType(.float32).reshaped([a.size]).asArray(Float.self)
+}
+
+private func hostIndices(_ mask: MLXArray) -> [Int64] {
+    flatNonzeroIndices(mask).asArray(Int32.self).map { Int64($0) }
+}
+
+private func check(_ status: Int32, _ what: String) {
+    if status != 0 {
+        let msg = String(cString: kwh_last_error())
+        preconditionFailure("\(what): \(msg)")
+    }
+}
+
+private func field(_ a: MLXArray?, _ arena: HostArena) -> kwh_field {
+    guard let a else { return kwh_field(data: nil, count: 0) }
+    let h = hostFloats(a)
+    return kwh_field(data: arena.copy(h), count: Int64(h.count))
+}
+
+private func sourceField(mask: MLXArray?, signal: MLXArray?, mode: SourceMode, _ arena: HostArena) -> kwh_source_field {
+    guard let mask, let signal else {
+        return kwh_source_field(index: nil, n_points: 0, signal: nil, rows: 0, len: 0, mode: 0)
+    }
+    let idx = hostIndices(mask)
+    let rows = signal.ndim <= 1 ? 1 : signal.shape[0]
+    let sig = hostFloats(signal)
+    let m: Int32
+    switch mode {
+    case .dirichlet: m = 0
+    case .additive: m = 1
+    case .additiveNoCorrection: m = 2
+    }
+    return kwh_source_field(index: arena.copy(idx), n_points: Int64(idx.count), signal: arena.copy(sig),
+                            rows: Int32(rows), len: Int32(sig.count / max(rows, 1)), mode: m)
+}
+
+private var sharedContext: OpaquePointer? = nil
+
+func kspaceFirstOrderHIP(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource,
+                         sensor: KWaveSensor, options: SimulationOptions) -> SimulationOutput {
+    precondition(grid.nt > 0, "call grid.makeTime(...) before running the solver")
+    precondition(source.p0 != nil || source.pMask != nil || source.uMask != nil,
+                 "\(grid.dim)D solver requires source.p0 or a time-varying p/u source")
+    let arena = HostArena()
+    var d = kwh_sim_desc()
+    d.abi_version = Int32(KWH_ABI_VERSION)
+    d.dim = Int32(grid.dim)
+    d.n = (Int32(grid.nx), Int32(grid.ny), Int32(grid.nz))
+    d.d = (grid.dx, grid.dy, grid.dz)
+    d.dt = grid.dt
+    d.nt = Int32(grid.nt)
+    d.dtype = options.dtype == .float64 ? 1 : 0
+    switch options.pmlSize { case let .uniform(s): d.pml_size = Int32(s) }
+    d.pml_alpha = options.pmlAlpha
+    d.smooth_p0 = options.smoothP0 ? 1 : 0
+    d.smooth_c0 = options.smoothC0 ? 1 : 0
+    d.smooth_rho0 = options.smoothRho0 ? 1 : 0
+    d.sound_speed = field(medium.soundSpeed, arena)
+    d.density = field(medium.density, arena)
+    d.alpha_coeff = field(medium.alphaCoeff, arena)
+    d.alpha_power = medium.alphaPower ?? 1.5
+    switch medium.alphaMode {
+    case .noAbsorption: d.alpha_mode = 0
+    case .powerLaw: d.alpha_mode = 1
+    case .noDispersion: d.alpha_mode = 2
+    case .stokes: d.alpha_mode = 3
+    }
+    d.bona = field(medium.bOnA, arena)
+    d.p0 = field(source.p0, arena)
+    d.p_source = sourceField(mask: source.pMask, signal: source.p, mode: source.pMode, arena)
+    d.u_source = (sourceField(mask: source.uMask, signal: source.ux, mode: source.uMode, arena),
+                  sourceField(mask: source.uMask, signal: source.uy, mode: source.uMode, arena),
+                  sourceField(mask: source.uMask, signal: source.uz, mode: source.uMode, arena))
+    var nSensor = 0
+    if let mask = sensor.mask {
+        if mask.ndim == 2 && mask.dim(0) == grid.dim {              // Recording.swift:50-53
+            let pts = hostFloats(mask)
+            d.sensor_cartesian = UnsafePointer(arena.copy(pts))
+            d.n_cartesian = Int64(mask.dim(1))
+            nSensor = mask.dim(1)
+        } else {
+            let idx = hostIndices(mask)
+            d.sensor_index = UnsafePointer(arena.copy(idx))
+            d.n_sensor = Int64(idx.count)
+            nSensor = idx.count
+        }
+        if let ang = sensor.directivityAngle {
+            d.directivity_angle = UnsafePointer(arena.copy(hostFloats(ang)))
+            d.directivity_size = sensor.directivitySize ?? 0
+            d.directivity_pattern = sensor.directivityPattern == .gradient ? 1 : 0
+        }
+    }
+    d.record = UInt32(sensor.record.rawValue)
+
+    if sharedContext == nil { check(kwh_ctx_create(0, &sharedContext), "kwh_ctx_create") }
+    var sim: OpaquePointer? = nil
+    check(kwh_sim_create(sharedContext, &d, &sim), "kwh_sim_create")
+    defer { kwh_sim_destroy(sim) }
+
+    if options.progress == nil && options.fieldMonitor == nil {
+        check(kwh_sim_run(sim, 0, Int32(grid.nt), nil, 0, nil), "kwh_sim_run")
+    } else {
+        // Step by step so progress(t, nt) and fieldMonitor(t, p) fire as in the reference (:337-340).
+        let interval = max(options.fieldMonitorInterval, 1)
+        var host = [Float](repeating: 0, count: grid.totalGridPoints)
+        for t in 0..<grid.nt {
+            check(kwh_sim_run(sim, Int32(t), Int32(t + 1), nil, 0, nil), "kwh_sim_run")
+            if let monitor = options.fieldMonitor, t % interval == 0 {
+                host.withUnsafeMutableBytes { b in
+                    check(kwh_sim_read_output(sim, Int32(KWH_OUT_P_FINAL.rawValue), b.baseAddress, b.count),
+                          "kwh_sim_read_output")
+                }
+                monitor(t, MLXArray(host).reshaped(grid.size))
+            }
+            options.progress?(t, grid.nt)
+        }
+    }
+
+    func read(_ which: kwh_output) -> MLXArray? {
+        var shape: (Int64, Int64, Int64) = (1, 1, 1)
+        var nd: Int32 = 0
+        let ok = withUnsafeMutablePointer(to: &shape) { p in
+            p.withMemoryRebound(to: Int64.self, capacity: 3) { kwh_sim_output_shape(sim, Int32(which.rawValue), $0, &nd) }
+        }
+        guard ok == 0 else { return nil }
+        let dims = [Int(shape.0), Int(shape.1), Int(shape.2)].prefix(Int(nd)).map { $0 }
+        let count = dims.reduce(1, *)
+        if options.dtype == .float64 {
+            var h = [Double](repeating: 0, count: count)
+            h.withUnsafeMutableBytes { b in check(kwh_sim_read_output(sim, Int32(which.rawValue), b.baseAddress, b.count), "read") }
+            return MLXArray(converting: h).reshaped(dims)
+        }
+        var h = [Float](repeating: 0, count: count)
+        h.withUnsafeMutableBytes { b in check(kwh_sim_read_output(sim, Int32(which.rawValue), b.baseAddress, b.count), "read") }
+        return MLXArray(h).reshaped(dims)
+    }
+
+    _ = nSensor
+    var out = SimulationOutput()
+    out.pFinal = read(KWH_OUT_P_FINAL)
+    out.p = read(KWH_OUT_P); out.pMax = read(KWH_OUT_P_MAX); out.pMin = read(KWH_OUT_P_MIN); out.pRms = read(KWH_OUT_P_RMS)
+    out.ux = read(KWH_OUT_UX); out.uy = read(KWH_OUT_UY); out.uz = read(KWH_OUT_UZ)
+    out.uxMax = read(KWH_OUT_UX_MAX); out.uyMax = read(KWH_OUT_UY_MAX); out.uzMax = read(KWH_OUT_UZ_MAX)
+    out.uxRms = read(KWH_OUT_UX_RMS); out.uyRms = read(KWH_OUT_UY_RMS); out.uzRms = read(KWH_OUT_UZ_RMS)
+    out.uxFinal = read(KWH_OUT_UX_FINAL); out.uyFinal = read(KWH_OUT_UY_FINAL); out.uzFinal = read(KWH_OUT_UZ_FINAL)
+    out.ixAvg = read(KWH_OUT_IX_AVG); out.iyAvg = read(KWH_OUT_IY_AVG); out.izAvg = read(KWH_OUT_IZ_AVG)
+    return out
+}
