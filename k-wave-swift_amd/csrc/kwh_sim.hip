@@ -19,6 +19,7 @@
 #include "../../include/kwave_hip.h"
 #include "kwh_kernels.h"
 #include "kwh_setup.h"
+#include "kwh_sq.h"
 
 using namespace kwh;
 
@@ -241,6 +242,8 @@ struct SimT final : kwh_sim {
     hipGraph_t graph = nullptr;
 
     size_t lds_budget = 64 * 1024;
+    int sqR[3] = {0, 0, 0};   // R when the internal axis length is R*R with a register-resident kernel
+    bool use_sq = true;
 
     ~SimT() override {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -331,10 +334,17 @@ struct SimT final : kwh_sim {
     }
 
     double spec_bytes() const { return (double)g.NH * sizeof(C); }
+    double mapb(const void* p) const { return p ? (double)g.N * sizeof(T) : 0.0; }
     double real_bytes() const { return (double)g.N * sizeof(T); }
 
     // ------------------------------------------------------------------ pass wrappers
     void strided(int axis, bool inv, std::initializer_list<void*> fields) {
+        std::vector<void*> v(fields);
+        strided_v(axis, inv, v);
+    }
+    // columns per tile for the register-resident kernels: 256 lanes (128 in float64)
+    int sq_ntr(int R) const { return (sizeof(T) == 8 ? 128 : 256) / R; }
+    void strided_v(int axis, bool inv, const std::vector<void*>& fields) {
         SCfg c = scfg(axis, 2);
         if (c.L <= 1) return;
         StridedArgs a{};
@@ -346,10 +356,28 @@ struct SimT final : kwh_sim {
         a.stride = c.stride;
         a.inner = c.inner;
         a.ncols = c.ncols;
+        const double bytes = 2.0 * spec_bytes() * nf;
+        const int R = sqR[axis];
+        if (R) {
+            const int ntr = sq_ntr(R);
+            a.lg_ntr = ilog2(ntr);
+            a.pitch = ntr;
+            dim3 grid((unsigned)((c.ncols + ntr - 1) / ntr), nf);
+            const size_t lds = ((size_t)c.L * ntr + R) * sizeof(C);
+            timed(inv ? K_SI : K_SF, bytes, [&] {
+                if (R == 16) {
+                    if (inv) hipLaunchKernelGGL((sq_strided<T, 16, true>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
+                    else hipLaunchKernelGGL((sq_strided<T, 16, false>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
+                } else {
+                    if (inv) hipLaunchKernelGGL((sq_strided<T, 8, true>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
+                    else hipLaunchKernelGGL((sq_strided<T, 8, false>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
+                }
+            });
+            return;
+        }
         a.lg_ntr = c.lg_ntr;
         a.pitch = c.pitch;
         dim3 grid(c.blocks, nf);
-        const double bytes = 2.0 * spec_bytes() * nf;
         timed(inv ? K_SI : K_SF, bytes, [&] {
             if (inv)
                 hipLaunchKernelGGL((strided_pass<T, true>), grid, dim3(c.threads), c.lds, stream, a, tab.plan[axis], tab.tw[axis]);
@@ -370,18 +398,68 @@ struct SimT final : kwh_sim {
         a.stride = c.stride;
         a.inner = c.inner;
         a.ncols = c.ncols;
+        a.scale = 1.0 / (double)g.N;
+        // algorithmic bytes: inputs + outputs + one real k-space table read per table-using op
+        int ntab = 0;
+        for (auto& o : ops) if (o.kind == OP_DDXP_KAPPA || o.kind == OP_DDXN_KAPPA || o.kind == OP_TAB) ++ntab;
+        const double bytes = spec_bytes() * (a.nin + a.nop) + (double)g.NH * sizeof(T) * ntab;
+        const int R = sqR[mid_axis];
+        if (R) {
+            for (int f = 0; f < a.nin; ++f) {   // one k-space table per input (loaded once in-kernel)
+                const void* t0 = nullptr;
+                for (auto& o : ops)
+                    if (o.in == f && o.table) {
+                        KWH_REQUIRE(!t0 || t0 == o.table, KWH_EUNSUPPORTED, "middle pass: two tables for one input");
+                        t0 = o.table;
+                    }
+            }
+            const int ntr = sq_ntr(R);
+            a.lg_ntr = ilog2(ntr);
+            a.pitch = ntr;
+            const size_t lds = ((size_t)a.L * ntr + R) * sizeof(C);
+            const unsigned blocks = (unsigned)((c.ncols + ntr - 1) / ntr);
+            timed(kid, bytes, [&] {
+                if (R == 16) hipLaunchKernelGGL((sq_middle<T, 16>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
+                else hipLaunchKernelGGL((sq_middle<T, 8>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
+            });
+            return;
+        }
         a.lg_ntr = c.lg_ntr;
         a.pitch = c.pitch;
-        a.scale = 1.0 / (double)g.N;
-        const double bytes = spec_bytes() * (a.nin + a.nop);
         timed(kid, bytes, [&] {
             hipLaunchKernelGGL((middle_pass<T>), dim3(c.blocks), dim3(c.threads), c.lds, stream, a, tab, g);
+        });
+    }
+
+    // Launch configuration of the register-resident z kernels: P row pairs x R lanes.
+    struct SqZ { int P, blocks, threads; size_t lds; };
+    SqZ sqz() const {
+        const int R = sqR[2];
+        SqZ z{};
+        z.P = (sizeof(T) == 8 ? 128 : 256) / R;
+        const int npairs = (g.nrows + 1) / 2;
+        z.blocks = (npairs + z.P - 1) / z.P;
+        z.threads = z.P * R;
+        z.lds = (size_t)z.P * (R * R + R + 1) * sizeof(C);
+        return z;
+    }
+    template <typename K, typename... Args>
+    void zlaunch(int kid, double bytes, K kernel16, K kernel8, Args... args) {
+        SqZ z = sqz();
+        timed(kid, bytes, [&] {
+            if (sqR[2] == 16) hipLaunchKernelGGL(kernel16, dim3(z.blocks), dim3(z.threads), z.lds, stream, args..., z.P, tab);
+            else hipLaunchKernelGGL(kernel8, dim3(z.blocks), dim3(z.threads), z.lds, stream, args..., z.P, tab);
         });
     }
 
     // Full forward transform of a real field into `out` (z, then plain passes): leaves the
     // spectrum ready for a middle pass.
     void forward_to_middle(const T* src, C* out, T scale) {
+        if (sqR[2]) {
+            zlaunch(K_ZFR, real_bytes() + spec_bytes(), sq_z_forward_real<T, 16>, sq_z_forward_real<T, 8>, src, out, scale, g);
+            if (plain1) strided(1, false, {out});
+            return;
+        }
         ZCfg zc = zcfg(2);
         timed(K_ZFR, real_bytes() + spec_bytes(), [&] {
             hipLaunchKernelGGL((z_forward_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, src, out, scale, g,
@@ -391,6 +469,11 @@ struct SimT final : kwh_sim {
     }
     void inverse_from_middle(C* in, T* dst, T scale, bool accumulate) {
         if (plain1) strided(1, true, {in});
+        if (sqR[2]) {
+            zlaunch(K_ZIR, real_bytes() + spec_bytes(), sq_z_inverse_real<T, 16>, sq_z_inverse_real<T, 8>,
+                    (const C*)in, dst, scale, accumulate ? 1 : 0, g);
+            return;
+        }
         ZCfg zc = zcfg(2);
         timed(K_ZIR, real_bytes() + spec_bytes(), [&] {
             hipLaunchKernelGGL((z_inverse_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, in, dst, scale,
@@ -425,6 +508,12 @@ struct SimT final : kwh_sim {
         g.nrows = g.m0 * g.m1;
         mid_axis = g.m0 > 1 ? 0 : 1;
         plain1 = mid_axis == 0 && g.m1 > 1;
+        if (const char* e = std::getenv("KWH_GENERIC_FFT")) use_sq = std::atoi(e) == 0;
+        {
+            const int mm[3] = {g.m0, g.m1, g.m2};
+            for (int ax = 0; ax < 3; ++ax)
+                sqR[ax] = !use_sq ? 0 : (mm[ax] == 256 ? 16 : (mm[ax] == 64 ? 8 : 0));
+        }
         const int64_t N = g.N;
 
         // ---- media (KSpaceFirstOrder.swift:573-579)
@@ -1111,10 +1200,17 @@ struct SimT final : kwh_sim {
                 va.sp[a] = sparse_of(usrc[a], a);
             }
             va.t = t;
-            ZCfg zc = zcfg(2);
-            timed(K_ZV, (real_bytes() * (2 + (dtrs[0].p ? 1 : 0)) + 2 * spec_bytes()) * dim, [&] {
-                hipLaunchKernelGGL((z_velocity<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, va, g, zc.zt, tab);
-            });
+            double bytes = 0;
+            for (int a = 0; a < dim; ++a)
+                bytes += 2 * spec_bytes() + 2 * real_bytes() + mapb(dtrs[a].p) + (cu[a] ? real_bytes() : 0);
+            if (sqR[2]) {
+                zlaunch(K_ZV, bytes, sq_z_velocity<T, 16>, sq_z_velocity<T, 8>, va, g);
+            } else {
+                ZCfg zc = zcfg(2);
+                timed(K_ZV, bytes, [&] {
+                    hipLaunchKernelGGL((z_velocity<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, va, g, zc.zt, tab);
+                });
+            }
         }
         // divergence (rhoStep :680-692)
         if (plain1) strided(1, false, {spec[0].p, spec[1].p, spec[2].p});
@@ -1135,9 +1231,14 @@ struct SimT final : kwh_sim {
         }
         for (int a = 0; a < dim; ++a)
             if (need_u[a]) {
-                ZCfg zc = zcfg(2);
                 C* s = spec[ucol_slot[a]].as<C>();
                 T* dst = ucol[a].as<T>();
+                if (sqR[2]) {
+                    zlaunch(K_ZIR, real_bytes() + spec_bytes(), sq_z_inverse_real<T, 16>, sq_z_inverse_real<T, 8>,
+                            (const C*)s, dst, T(1), 0, g);
+                    continue;
+                }
+                ZCfg zc = zcfg(2);
                 timed(K_ZIR, real_bytes() + spec_bytes(), [&] {
                     hipLaunchKernelGGL((z_inverse_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, s, dst, T(1),
                                        0, g, zc.zt, tab);
@@ -1169,10 +1270,24 @@ struct SimT final : kwh_sim {
             ra.dispersion = dispersion;
             ra.t = t;
             const int nbuf = 4 + (nonlinear ? 1 : 0);
-            ZCfg zc = zcfg(nbuf);
-            timed(K_ZD, real_bytes() * (3 * dim + 2) + spec_bytes() * (dim + 1), [&] {
-                hipLaunchKernelGGL((z_density<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ra, g, zc.zt, tab);
-            });
+            double bytes = dim * (spec_bytes() + 2 * real_bytes()) + mapb(rho0s.p) + (cp ? real_bytes() : 0);
+            if (nonlinear) bytes += mapb(bonas.p);
+            if (fft_abs) bytes += real_bytes() + spec_bytes() * (dispersion ? 2 : 1);
+            else bytes += real_bytes() + mapb(c0s.p) + spec_bytes() + (stokes ? mapb(taus.p) : 0);
+            if (sqR[2]) {
+                if (nonlinear) {
+                    if (fft_abs) zlaunch(K_ZD, bytes, sq_z_density<T, 16, true, 1>, sq_z_density<T, 8, true, 1>, ra, g);
+                    else zlaunch(K_ZD, bytes, sq_z_density<T, 16, true, 0>, sq_z_density<T, 8, true, 0>, ra, g);
+                } else {
+                    if (fft_abs) zlaunch(K_ZD, bytes, sq_z_density<T, 16, false, 1>, sq_z_density<T, 8, false, 1>, ra, g);
+                    else zlaunch(K_ZD, bytes, sq_z_density<T, 16, false, 0>, sq_z_density<T, 8, false, 0>, ra, g);
+                }
+            } else {
+                ZCfg zc = zcfg(nbuf);
+                timed(K_ZD, bytes, [&] {
+                    hipLaunchKernelGGL((z_density<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ra, g, zc.zt, tab);
+                });
+            }
         }
         if (fft_abs) {
             if (plain1) {
@@ -1202,10 +1317,16 @@ struct SimT final : kwh_sim {
             ea.nonlinear = nonlinear;
             ea.p = p.as<T>();
             ea.outP = spec[0].as<C>();
-            ZCfg zc = zcfg(3);
-            timed(K_ZE, real_bytes() * 5 + spec_bytes() * 3, [&] {
-                hipLaunchKernelGGL((z_eos<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ea, g, zc.zt, tab);
-            });
+            double bytes = spec_bytes() * (dispersion ? 3 : 2) + 2 * real_bytes() + mapb(c0s.p) + mapb(taus.p) +
+                           (dispersion ? mapb(etas.p) : 0) + (nonlinear ? mapb(bonas.p) + mapb(rho0s.p) : 0);
+            if (sqR[2]) {
+                zlaunch(K_ZE, bytes, sq_z_eos<T, 16>, sq_z_eos<T, 8>, ea, g);
+            } else {
+                ZCfg zc = zcfg(3);
+                timed(K_ZE, bytes, [&] {
+                    hipLaunchKernelGGL((z_eos<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ea, g, zc.zt, tab);
+                });
+            }
         }
         P_valid = true;
     }

@@ -1,0 +1,556 @@
+// kwh_sq.h -- register-resident kernels for axis lengths L = R*R (R = 16: 256, R = 8: 64, R = 4: 16).
+//
+// A length-L transform is two radix-R stages. Lane j (0..R-1) of a transform holds elements
+// n = j + R*r (r = 0..R-1) in registers:
+//   stage A (Stockham Ns = 1): DFT_R over r, result to LDS at n' = j*R + r;
+//   stage B (Ns = R): read n = j + R*r, twiddle W_L^(j*r), DFT_R -> element j + R*r in registers.
+// Input and output of a transform are therefore both in the lane-natural pattern n = j + R*r, so
+// an inverse, an elementwise update and a forward chain without extra LDS traffic, and the
+// outputs of a forward feed the next inverse directly (middle pass). One LDS transpose per
+// transform; every lane busy in both stages.
+#pragma once
+#include "kwh_kernels.h"
+
+namespace kwh {
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dftR(cx<T>* v) {
+    dft<T, R, INV>(v);
+}
+
+// ------------------------------------------------------------------------------------------
+// Contiguous-axis transforms of a row pair (a + i b), lanes j of transform w in a workgroup.
+// LDS per transform: L + R complex (one pad slot per R elements).
+// ------------------------------------------------------------------------------------------
+template <typename T, int R>
+struct SqRow {
+    static constexpr int L = R * R;
+    static constexpr int LP = L + R + 1;
+    __device__ static __forceinline__ int pidx(int n) { return n + n / R; }
+
+    // Two-stage transform of v (lane-natural in, lane-natural out). lds: this transform's slab.
+    template <bool INV>
+    __device__ static __forceinline__ void transform(cx<T>* v, cx<T>* lds, int j, const cx<T>* __restrict__ tw) {
+        dftR<T, R, INV>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[pidx(j * R + r)] = v[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            cx<T> x = lds[pidx(j + R * r)];
+            if (r > 0) {
+                cx<T> t = tw[j * r];
+                if (INV) t.y = -t.y;
+                x = cmul(x, t);
+            }
+            v[r] = x;
+        }
+        dftR<T, R, INV>(v);
+        __syncthreads();
+    }
+
+    // Load the Hermitian-extended Z = A + iB of rows (ra, rb) in the lane-natural pattern.
+    // Elements n = j + R r < L/2 read k = n from base (j); n >= L/2 read k = L - n from base
+    // (L - j) with negative compile-time offsets, so every load is base + immediate.
+    __device__ static __forceinline__ void load_spectrum(const cx<T>* __restrict__ spec, int ra, int rb, int nrows,
+                                                         int mzh, int j, cx<T>* v) {
+        const bool va = ra < nrows, vb = rb < nrows;
+        const cx<T>* sa = spec + (int64_t)(va ? ra : 0) * mzh;
+        const cx<T>* sb = spec + (int64_t)(vb ? rb : 0) * mzh;
+        const cx<T>* la = sa + j;
+        const cx<T>* lb = sb + j;
+        const cx<T>* ha = sa + (L - j);
+        const cx<T>* hb = sb + (L - j);
+        const cx<T> zero{T(0), T(0)};
+#pragma unroll
+        for (int r = 0; r < R / 2; ++r) {
+            const cx<T> A = va ? la[R * r] : zero;
+            const cx<T> B = vb ? lb[R * r] : zero;
+            v[r] = cx<T>{A.x - B.y, A.y + B.x};
+        }
+#pragma unroll
+        for (int r = R / 2; r < R; ++r) {
+            const cx<T> A = va ? ha[-R * r] : zero;
+            const cx<T> B = vb ? hb[-R * r] : zero;
+            v[r] = cx<T>{A.x + B.y, B.x - A.y};
+        }
+        if (j == 0) {   // DC (n = 0) and Nyquist (n = L/2) bins are real parts only
+            const cx<T> A0 = va ? sa[0] : zero, B0 = vb ? sb[0] : zero;
+            const cx<T> AN = va ? sa[L / 2] : zero, BN = vb ? sb[L / 2] : zero;
+            v[0] = cx<T>{A0.x, B0.x};
+            v[R / 2] = cx<T>{AN.x, BN.x};
+        }
+    }
+
+    // Sum of the Hermitian-extended spectra of nf fields (rows ra, rb): the spectrum of the sum.
+    __device__ static __forceinline__ void load_spectrum_sum(const cx<T>* const* specs, int nf, int ra, int rb,
+                                                             int nrows, int mzh, int j, cx<T>* v) {
+        load_spectrum(specs[0], ra, rb, nrows, mzh, j, v);
+        for (int f = 1; f < nf; ++f) {
+            cx<T> t[R];
+            load_spectrum(specs[f], ra, rb, nrows, mzh, j, t);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = v[r] + t[r];
+        }
+    }
+
+    // Forward of v (a in .x, b in .y), split into half spectra, stored to rows (ra, rb).
+    __device__ static __forceinline__ void forward_store(cx<T>* spec, int ra, int rb, int nrows, int mzh, cx<T>* lds,
+                                                         int j, const cx<T>* __restrict__ tw, cx<T>* v) {
+        transform<false>(v, lds, j, tw);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[pidx(j + R * r)] = v[r];
+        __syncthreads();
+        const bool va = ra < nrows, vb = rb < nrows;
+#pragma unroll
+        for (int m = 0; m <= R / 2; ++m) {
+            const int k = j + R * m;
+            if (k > L / 2) break;
+            const cx<T> zk = lds[pidx(k)];
+            const cx<T> zm = lds[pidx(k == 0 ? 0 : L - k)];
+            if (va) spec[(int64_t)ra * mzh + k] = cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+            if (vb) spec[(int64_t)rb * mzh + k] = cx<T>{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+        }
+        __syncthreads();
+    }
+};
+
+// Row pair handled by a lane group: P transforms per workgroup, R lanes each.
+template <int R>
+struct SqZPos {
+    int w, j, q, ra, rb;
+    __device__ SqZPos(int P) {
+        const int tid = threadIdx.x;
+        w = tid / R;
+        j = tid - w * R;
+        const int tile = xcd_tile(blockIdx.x, gridDim.x);
+        q = tile * P + w;
+        ra = 2 * q;
+        rb = ra + 1;
+    }
+};
+
+// Sparse sources on register-resident rows: go through LDS (natural order) when the tile has any.
+template <typename T, int R>
+__device__ void sq_apply_sparse(const SparseSrc<T>& s, const Geo& g, int row0, int nrow_tile, cx<T>* lds_all,
+                                int w, int j, int t, cx<T>* v) {
+    using S = SqRow<T, R>;
+    cx<T>* lds = lds_all + w * S::LP;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[S::pidx(j + R * r)] = v[r];
+    __syncthreads();
+    if (t < s.len) {
+        const int r1 = min(g.nrows, row0 + nrow_tile);
+        const int e0 = s.rowptr[row0], e1 = s.rowptr[r1];
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+            const int64_t cell = s.index[e];
+            const int row = (int)(cell / g.m2);
+            const int n = (int)(cell - (int64_t)row * g.m2);
+            const int lr = row - row0;
+            const T val = s.signal[(int64_t)(s.rows == 1 ? 0 : e) * s.len + t] * s.base[e] * s.factor;
+            T& slot = comp(lds_all[(lr >> 1) * S::LP + S::pidx(n)], lr & 1);
+            slot = s.mode == 0 ? val : slot + val;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = lds[S::pidx(j + R * r)];
+    __syncthreads();
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(256) sq_z_forward_real(const T* __restrict__ src, cx<T>* spec, T scale, Geo g,
+                                                         int P, AxisTabs<T> tab) {
+    using S = SqRow<T, R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    SqZPos<R> ps(P);
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    cx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int n = ps.j + R * r;
+        v[r].x = va ? src[(int64_t)ps.ra * S::L + n] * scale : T(0);
+        v[r].y = vb ? src[(int64_t)ps.rb * S::L + n] * scale : T(0);
+    }
+    S::forward_store(spec, ps.ra, ps.rb, g.nrows, g.mzh, lds + ps.w * S::LP, ps.j, tab.tw[2], v);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(256) sq_z_inverse_real(const cx<T>* spec, T* __restrict__ dst, T scale,
+                                                         int accumulate, Geo g, int P, AxisTabs<T> tab) {
+    using S = SqRow<T, R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    SqZPos<R> ps(P);
+    cx<T> v[R];
+    S::load_spectrum(spec, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+    S::template transform<true>(v, lds + ps.w * S::LP, ps.j, tab.tw[2]);
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int n = ps.j + R * r;
+        if (va) {
+            const int64_t c = (int64_t)ps.ra * S::L + n;
+            dst[c] = accumulate ? dst[c] + v[r].x * scale : v[r].x * scale;
+        }
+        if (vb) {
+            const int64_t c = (int64_t)ps.rb * S::L + n;
+            dst[c] = accumulate ? dst[c] + v[r].y * scale : v[r].y * scale;
+        }
+    }
+}
+
+// Velocity update, per axis: C2R_z(G_a) -> u_a = pml_sg (pml_sg u_a - dt/rho_a g) [+ sources] -> R2C_z.
+template <typename T, int R>
+__global__ void __launch_bounds__(256) sq_z_velocity(VelArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    using S = SqRow<T, R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* lds_all = reinterpret_cast<cx<T>*>(smem);
+    SqZPos<R> ps(P);
+    cx<T>* lds = lds_all + ps.w * S::LP;
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    for (int ax = 0; ax < g.dim; ++ax) {
+        const int ia = ax + g.off;
+        cx<T> v[R];
+        S::load_spectrum(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        S::template transform<true>(v, lds, ps.j, tab.tw[2]);
+        T* u = a.u[ax];
+        const T* pm = tab.pmlsg[ia];
+        const SMap<T> dtr = a.dtr[ax];
+        const T* corr = a.corr[ax];
+        const T pa = ia == 2 ? T(0) : pm[axis_pos(g, ra, 0, ia)];
+        const T pb = ia == 2 ? T(0) : pm[axis_pos(g, rb, 0, ia)];
+        const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+        const T* ua = u + oa;
+        const T* ub = u + ob;
+        const T* pz = pm + ps.j;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const T qa = ia == 2 ? pz[R * r] : pa, qb = ia == 2 ? pz[R * r] : pb;
+            const T da = dtr.p ? dtr.p[oa + R * r] : dtr.s;
+            const T db = dtr.p ? dtr.p[ob + R * r] : dtr.s;
+            T xa = qa * (qa * ua[R * r] - da * v[r].x);
+            T xb = qb * (qb * ub[R * r] - db * v[r].y);
+            if (corr) { xa += corr[oa + R * r]; xb += corr[ob + R * r]; }
+            v[r].x = va ? xa : T(0);
+            v[r].y = vb ? xb : T(0);
+        }
+        if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{(int)(xcd_tile(blockIdx.x, gridDim.x) * P), P, 0, 0}))
+            sq_apply_sparse<T, R>(a.sp[ax], g, 2 * xcd_tile(blockIdx.x, gridDim.x) * P, 2 * P, lds_all, ps.w, ps.j,
+                                  a.t, v);
+        {
+            T* wa = u + oa;
+            T* wb = u + ob;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (va) wa[R * r] = v[r].x;
+                if (vb) wb[R * r] = v[r].y;
+            }
+        }
+        S::forward_store(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], v);
+    }
+}
+
+// Density update + EOS (MODE 0) or absorption inputs (MODE 1); see z_density. NL: B/A active.
+// Only the total density is accumulated per element; sum_a du_a (needed for Stokes absorption
+// and for rho0 div u) is the inverse transform of sum_a D_a, re-read from the divergence spectra.
+template <typename T, int R, bool NL, int MODE>
+__global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    using S = SqRow<T, R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* lds_all = reinterpret_cast<cx<T>*>(smem);
+    SqZPos<R> ps(P);
+    cx<T>* lds = lds_all + ps.w * S::LP;
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    const int row0 = 2 * xcd_tile(blockIdx.x, gridDim.x) * P;
+    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+    cx<T> rt[R];
+    cx<T> rp[NL ? R : 1];
+    if constexpr (NL) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            T sa = 0, sb = 0;
+            for (int ax = 0; ax < g.dim; ++ax) {
+                sa += a.rho[ax][oa + R * r];
+                sb += a.rho[ax][ob + R * r];
+            }
+            rp[r] = cx<T>{sa, sb};
+        }
+    }
+    for (int ax = 0; ax < g.dim; ++ax) {
+        const int ia = ax + g.off;
+        cx<T> v[R];
+        S::load_spectrum(a.D[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        S::template transform<true>(v, lds, ps.j, tab.tw[2]);
+        T* rho = a.rho[ax];
+        const T* pm = tab.pml[ia];
+        const T pa = ia == 2 ? T(0) : pm[axis_pos(g, ra, 0, ia)];
+        const T pb = ia == 2 ? T(0) : pm[axis_pos(g, rb, 0, ia)];
+        const T* pz = pm + ps.j;
+        const T cf = a.corr_factor[ax];
+        const T* rha = rho + oa;
+        const T* rhb = rho + ob;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const T qa = ia == 2 ? pz[R * r] : pa, qb = ia == 2 ? pz[R * r] : pb;
+            const T r0a = a.rho0.p ? a.rho0.p[oa + R * r] : a.rho0.s;
+            const T r0b = a.rho0.p ? a.rho0.p[ob + R * r] : a.rho0.s;
+            T ma = a.dt * r0a * v[r].x, mb = a.dt * r0b * v[r].y;
+            if constexpr (NL) {
+                ma *= (T(2) * rp[r].x + r0a) / r0a;
+                mb *= (T(2) * rp[r].y + r0b) / r0b;
+            }
+            T xa = qa * (qa * rha[R * r] - ma);
+            T xb = qb * (qb * rhb[R * r] - mb);
+            if (a.corr) { xa += a.corr[oa + R * r] * cf; xb += a.corr[ob + R * r] * cf; }
+            v[r] = cx<T>{xa, xb};
+        }
+        if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{row0 / 2, P, 0, 0}))
+            sq_apply_sparse<T, R>(a.sp[ax], g, row0, 2 * P, lds_all, ps.w, ps.j, a.t, v);
+        T* wa = rho + oa;
+        T* wb = rho + ob;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (va) wa[R * r] = v[r].x;
+            if (vb) wb[R * r] = v[r].y;
+            if (ax == 0) rt[r] = v[r];
+            else { rt[r].x += v[r].x; rt[r].y += v[r].y; }
+        }
+    }
+    const bool need_div = MODE == 1 || a.stokes;
+    cx<T> v[R];
+    if (need_div) {
+        S::load_spectrum_sum(a.D, g.dim, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        S::template transform<true>(v, lds, ps.j, tab.tw[2]);
+    }
+    if constexpr (MODE == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            T ea = rt[r].x, eb = rt[r].y;
+            const T r0a = a.rho0.p ? a.rho0.p[oa + R * r] : a.rho0.s;
+            const T r0b = a.rho0.p ? a.rho0.p[ob + R * r] : a.rho0.s;
+            if (a.stokes) {
+                ea += a.tau.at(oa + R * r) * r0a * v[r].x;
+                eb += a.tau.at(ob + R * r) * r0b * v[r].y;
+            }
+            if constexpr (NL) {
+                ea += a.bona.at(oa + R * r) * (rt[r].x * rt[r].x) / (T(2) * r0a);
+                eb += a.bona.at(ob + R * r) * (rt[r].y * rt[r].y) / (T(2) * r0b);
+            }
+            const T cA = a.c0.p ? a.c0.p[oa + R * r] : a.c0.s;
+            const T cB = a.c0.p ? a.c0.p[ob + R * r] : a.c0.s;
+            const T pA = cA * cA * ea, pB = cB * cB * eb;
+            if (va) a.p[oa + R * r] = pA;
+            if (vb) a.p[ob + R * r] = pB;
+            v[r] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
+        }
+        S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], v);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const T r0a = a.rho0.p ? a.rho0.p[oa + R * r] : a.rho0.s;
+            const T r0b = a.rho0.p ? a.rho0.p[ob + R * r] : a.rho0.s;
+            v[r] = cx<T>{va ? r0a * v[r].x : T(0), vb ? r0b * v[r].y : T(0)};
+        }
+        S::forward_store(a.F1, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], v);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (va) a.rhoT[oa + R * r] = rt[r].x;
+            if (vb) a.rhoT[ob + R * r] = rt[r].y;
+            rt[r] = cx<T>{va ? rt[r].x : T(0), vb ? rt[r].y : T(0)};
+        }
+        if (a.dispersion) S::forward_store(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], rt);
+    }
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(256) sq_z_eos(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    using S = SqRow<T, R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    SqZPos<R> ps(P);
+    lds += ps.w * S::LP;
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    cx<T> e[R], v[R];
+    S::load_spectrum(a.A1, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, e);
+    S::template transform<true>(e, lds, ps.j, tab.tw[2]);
+    if (a.A2) {
+        S::load_spectrum(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        S::template transform<true>(v, lds, ps.j, tab.tw[2]);
+    }
+    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t ca = oa + R * r, cb = ob + R * r;
+        const T rta = a.rhoT[ca], rtb = a.rhoT[cb];
+        T ea = rta + a.tau.at(ca) * e[r].x;
+        T eb = rtb + a.tau.at(cb) * e[r].y;
+        if (a.A2) {
+            ea -= a.eta.at(ca) * v[r].x;
+            eb -= a.eta.at(cb) * v[r].y;
+        }
+        if (a.nonlinear) {
+            ea += a.bona.at(ca) * (rta * rta) / (T(2) * a.rho0.at(ca));
+            eb += a.bona.at(cb) * (rtb * rtb) / (T(2) * a.rho0.at(cb));
+        }
+        const T cA = a.c0.at(ca), cB = a.c0.at(cb);
+        const T pA = cA * cA * ea, pB = cB * cB * eb;
+        if (va) a.p[ca] = pA;
+        if (vb) a.p[cb] = pB;
+        e[r] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
+    }
+    S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], e);
+}
+
+// ------------------------------------------------------------------------------------------
+// Strided / middle passes over a tile of ntr contiguous columns: lane (w, j), w fastest.
+// LDS: element n of column w at n * ntr + w + n / R (pad one slot per R rows).
+// ------------------------------------------------------------------------------------------
+template <typename T, int R>
+struct SqCol {
+    static constexpr int L = R * R;
+    __device__ static __forceinline__ int lidx(int n, int w, int ntr) { return n * ntr + w + n / R; }
+
+    template <bool INV>
+    __device__ static __forceinline__ void stageB(cx<T>* v, const cx<T>* lds, int j, int w, int ntr,
+                                                  const cx<T>* __restrict__ tw) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            cx<T> x = lds[lidx(j + R * r, w, ntr)];
+            if (r > 0) {
+                cx<T> t = tw[j * r];
+                if (INV) t.y = -t.y;
+                x = cmul(x, t);
+            }
+            v[r] = x;
+        }
+        dftR<T, R, INV>(v);
+    }
+    template <bool INV>
+    __device__ static __forceinline__ void transform(cx<T>* v, cx<T>* lds, int j, int w, int ntr,
+                                                     const cx<T>* __restrict__ tw) {
+        dftR<T, R, INV>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lidx(j * R + r, w, ntr)] = v[r];
+        __syncthreads();
+        stageB<INV>(v, lds, j, w, ntr, tw);
+        __syncthreads();
+    }
+};
+
+template <typename T, int R, bool INV>
+__global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __restrict__ tw) {
+    using S = SqCol<T, R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    const int ntr = 1 << a.lg_ntr;
+    const int w = threadIdx.x & (ntr - 1), j = threadIdx.x >> a.lg_ntr;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t c = (int64_t)tile * ntr + w;
+    const bool valid = c < a.ncols;
+    const int64_t cc = valid ? c : 0;
+    const int64_t base = (cc / a.inner) * (int64_t)S::L * a.inner + cc % a.inner;
+    cx<T>* data = reinterpret_cast<cx<T>*>(a.fields[blockIdx.y]) + base + (int64_t)j * a.stride;
+    const int64_t step = (int64_t)R * a.stride;
+    cx<T> v[R];
+    {
+        const cx<T>* q = data;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v[r] = valid ? *q : cx<T>{T(0), T(0)};
+            q += step;
+        }
+    }
+    S::template transform<INV>(v, lds, j, w, ntr, tw);
+    if (valid) {
+        cx<T>* q = data;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            *q = v[r];
+            q += step;
+        }
+    }
+}
+
+// Middle pass: forward along the middle axis, operators, inverse, store. Every table-using
+// operator of one input reads the same real k-space table (checked on the host), loaded once.
+template <typename T, int R>
+__global__ void __launch_bounds__(256) sq_middle(MidArgs a, AxisTabs<T> tab, Geo g) {
+    using S = SqCol<T, R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    const int ntr = 1 << a.lg_ntr;
+    const int w = threadIdx.x & (ntr - 1), j = threadIdx.x >> a.lg_ntr;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t c = (int64_t)tile * ntr + w;
+    const bool valid = c < a.ncols;
+    const int64_t cc = valid ? c : 0;
+    const int64_t base = (cc / a.inner) * (int64_t)S::L * a.inner + cc % a.inner + (int64_t)j * a.stride;
+    const int64_t step = (int64_t)R * a.stride;
+    const cx<T>* tw = tab.tw[a.axis];
+    const T scale = T(a.scale);
+    int i0c, i1c, i2c;
+    if (a.axis == 0) { i0c = 0; i1c = (int)(cc / g.mzh); i2c = (int)(cc % g.mzh); }
+    else { i0c = (int)(cc / g.mzh); i1c = 0; i2c = (int)(cc % g.mzh); }
+    const int64_t hstride = a.axis == 0 ? (int64_t)g.m1 * g.mzh : (int64_t)g.mzh;
+    const int64_t hbase = (a.axis == 0 ? cc : (int64_t)i0c * g.m1 * g.mzh + i2c) + (int64_t)j * hstride;
+    const int64_t hstep = (int64_t)R * hstride;
+    for (int f = 0; f < a.nin; ++f) {
+        cx<T> X[R];
+        {
+            const cx<T>* q = reinterpret_cast<const cx<T>*>(a.in[f]) + base;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                X[r] = valid ? *q : cx<T>{T(0), T(0)};
+                q += step;
+            }
+        }
+        S::template transform<false>(X, lds, j, w, ntr, tw);
+        const T* ktab = nullptr;
+        for (int o = 0; o < a.nop; ++o)
+            if (a.op[o].in == f && a.op[o].table) { ktab = reinterpret_cast<const T*>(a.op[o].table); break; }
+        T kap[R];
+        if (ktab) {
+            const T* q = ktab + hbase;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                kap[r] = *q * scale;
+                q += hstep;
+            }
+        }
+        for (int o = 0; o < a.nop; ++o) {
+            const MidOp op = a.op[o];
+            if (op.in != f) continue;
+            const bool use_kappa = op.table != nullptr;
+            const bool use_vec = op.kind != OP_TAB;
+            const cx<T>* tab1 = op.kind == OP_DDXP_KAPPA || op.kind == OP_DDXP ? tab.ddxp[op.axis]
+                              : op.kind == OP_DDXN_KAPPA ? tab.ddxn[op.axis] : tab.coll[op.axis];
+            const bool vec_on_mid = op.axis == a.axis;
+            const int fixed_idx = op.axis == 0 ? i0c : (op.axis == 1 ? i1c : i2c);
+            const cx<T> fixed_vec = use_vec && !vec_on_mid ? tab1[fixed_idx] : cx<T>{T(1), T(0)};
+            const cx<T>* tj = tab1 + j;
+            cx<T> v[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cx<T> m = use_vec ? (vec_on_mid ? tj[R * r] : fixed_vec) : cx<T>{T(1), T(0)};
+                const T s = use_kappa ? kap[r] : scale;
+                v[r] = cmul(X[r], cx<T>{m.x * s, m.y * s});
+            }
+            S::template transform<true>(v, lds, j, w, ntr, tw);
+            if (valid) {
+                cx<T>* q = reinterpret_cast<cx<T>*>(op.out) + base;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    *q = v[r];
+                    q += step;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace kwh

@@ -44,10 +44,11 @@ def metrics(a, b):
     return rl2, rmax
 
 
-def check(name, grid, medium, source, sensor, fields, dtype=F32, l2=None, mx=None, final_scale=None, **opt):
+def check(name, grid, medium, source, sensor, fields, dtype=F32, l2=None, mx=None, final_scale=None, workers=1,
+          **opt):
     print(f"\n[case] {name} dtype={dtype.value} grid={grid.size} nt={grid.nt}", flush=True)
     gpu = kspaceFirstOrder(grid, medium, source, sensor, SimulationOptions(dtype=dtype, **opt))
-    ref = O.kspace_first_order(grid, medium, source, sensor, SimulationOptions(**opt))
+    ref = O.kspace_first_order(grid, medium, source, sensor, SimulationOptions(**opt), workers=workers)
     if l2 is None:
         l2 = 1e-4 if dtype == F32 else 1e-9
     if mx is None:
@@ -414,3 +415,97 @@ def test_deterministic_replay():
     a = kspaceFirstOrder(*args)
     b = kspaceFirstOrder(*args)
     assert np.array_equal(a.pFinal, b.pFinal)
+
+
+# ------------------------------------------------------------------------------------------ square-length fast path
+def test_2d_256_register_resident_path():
+    """256^2 runs the register-resident radix-16 x 16 kernels on both axes."""
+    n = 256
+    grid = KWaveGrid(n, 1e-4, n, 1e-4)
+    grid.setTime(120, 2e-8)
+    med = _hetero_2d(n, seed=31)
+    sens = np.zeros((n, n), np.float32)
+    sens[100, :] = 1     # 28 cells from the disc: reached within the run
+    check("2d_256_sq", grid, med, KWaveSource(p0=makeDisc(n, n, radius=6)),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.uy), ["p", "uy"], pmlSize=20)
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_3d_64_register_resident_path(dtype):
+    """64^3 runs the radix-8 x 8 register-resident kernels on every axis."""
+    n = 64
+    c = random_field((n, n, n), 41, 6.0, 1400.0, 1600.0)
+    rho = random_field((n, n, n), 42, 6.0, 900.0, 1100.0)
+    med = KWaveMedium(c, rho, alphaCoeff=0.5, alphaPower=1.1, alphaMode=AbsorptionMode.powerLaw, bOnA=5.0)
+    grid = KWaveGrid(n, 1e-4, n, 1e-4, n, 1e-4)
+    grid.setTime(60, 0.3 * 1e-4 / float(c.max()))
+    sens = np.zeros((n, n, n), np.float32)
+    sens[16, :, :] = 1   # 6 cells downstream of the velocity source: the burst dominates the series
+    mask = np.zeros((n, n, n), np.float32)
+    mask[10, 28:36, 28:36] = 1
+    sig = (0.2 * toneBurst(1 / grid.dt, 2e6, 2)).astype(np.float32)
+    check("3d_64_sq", grid, med, KWaveSource(p0=makeBall(n, n, n, radius=5), uMask=mask, ux=sig),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.ux | RecordField.uz), ["p", "ux", "uz"], dtype,
+          pmlSize=8)
+
+
+def test_generic_and_register_paths_agree():
+    """The generic Stockham engine and the register-resident kernels on the same 64^3 problem."""
+    import os
+    n = 64
+    grid = KWaveGrid(n, 1e-4, n, 1e-4, n, 1e-4)
+    grid.setTime(30, 2e-8)
+    args = (grid, KWaveMedium(1500.0, 1000.0), KWaveSource(p0=makeBall(n, n, n, radius=6)), KWaveSensor())
+    print("\n[case] generic vs register-resident 64^3", flush=True)
+    a = kspaceFirstOrder(*args, SimulationOptions(dtype=F64))
+    os.environ["KWH_GENERIC_FFT"] = "1"
+    try:
+        b = kspaceFirstOrder(*args, SimulationOptions(dtype=F64))
+    finally:
+        del os.environ["KWH_GENERIC_FFT"]
+    rl2, rmax = metrics(a.pFinal, b.pFinal)
+    print(f"  relL2={rl2:.3e} relMax={rmax:.3e}", flush=True)
+    assert rl2 < 1e-12
+
+
+def test_3d_256_headline_workload_short():
+    """The 256^3 workload itself (GRF media, power law y = 1.1, p0 ball, plane p + uz), 6 steps
+    against the float64 oracle."""
+    n = 256
+    c = random_field((n, n, n), 3, 20.0, 1400.0, 1600.0)
+    rho = random_field((n, n, n), 4, 20.0, 900.0, 1100.0)
+    med = KWaveMedium(c, rho, alphaCoeff=0.5, alphaPower=1.1, alphaMode=AbsorptionMode.powerLaw)
+    grid = KWaveGrid(n, 1e-4, n, 1e-4, n, 1e-4)
+    grid.setTime(6, 0.3 * 1e-4 / float(c.max()))
+    sens = np.zeros((n, n, n), np.float32)
+    sens[136, :, :] = 1    # close enough to the ball for signal within 6 steps
+    check("3d_256_short", grid, med, KWaveSource(p0=makeBall(n, n, n, radius=8)),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.uz), ["p", "uz", "pFinal"], pmlSize=10,
+          workers=16)
+
+
+def test_3d_256_fp32_vs_fp64_100_steps():
+    """FP32 against FP64 on the device at 256^3 over 100 steps: relative L2 of the plane p / uz
+    series and of pFinal normalised by max_t |p| (target <= 1e-4)."""
+    n = 256
+    c = random_field((n, n, n), 3, 20.0, 1400.0, 1600.0)
+    rho = random_field((n, n, n), 4, 20.0, 900.0, 1100.0)
+    med = KWaveMedium(c, rho, alphaCoeff=0.5, alphaPower=1.1, alphaMode=AbsorptionMode.powerLaw)
+    grid = KWaveGrid(n, 1e-4, n, 1e-4, n, 1e-4)
+    grid.setTime(100, 0.3 * 1e-4 / float(c.max()))
+    sens = np.zeros((n, n, n), np.float32)
+    sens[150, :, :] = 1
+    args = (grid, med, KWaveSource(p0=makeBall(n, n, n, radius=8)),
+            KWaveSensor(mask=sens, record=RecordField.p | RecordField.uz))
+    print("\n[case] 3d_256 fp32 vs fp64, 100 steps", flush=True)
+    a = kspaceFirstOrder(*args, SimulationOptions(pmlSize=10, dtype=F32))
+    b = kspaceFirstOrder(*args, SimulationOptions(pmlSize=10, dtype=F64))
+    for f in ("p", "uz"):
+        rl2, rmax = metrics(getattr(a, f), getattr(b, f))
+        print(f"  {f} relL2={rl2:.3e} relMax={rmax:.3e}", flush=True)
+        assert rl2 <= 1e-4
+    scale = np.abs(b.p).max()
+    d = a.pFinal.astype(np.float64) - b.pFinal
+    rl2 = math.sqrt(np.mean(d * d)) / scale
+    print(f"  pFinal/max|p| relL2={rl2:.3e}", flush=True)
+    assert rl2 <= 1e-4
