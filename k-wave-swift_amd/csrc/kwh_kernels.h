@@ -323,11 +323,15 @@ __device__ __forceinline__ int axis_pos(const Geo& g, int row, int n, int ax) {
     return ax == 2 ? n : (ax == 1 ? row % g.m1 : row / g.m1);
 }
 
-// Scalar-or-map accessor.
+// Scalar-or-map accessor. `dp` always points at device memory: the map itself (mask = ~0) or a
+// one-element buffer holding the scalar (mask = 0), so at() is a branch-free load. `p` / `s`
+// keep the host-side view (p == nullptr for a scalar).
 template <typename T> struct SMap {
     const T* p;
     T s;
-    __device__ __forceinline__ T at(int64_t i) const { return p ? p[i] : s; }
+    const T* dp;
+    int64_t mask;
+    __device__ __forceinline__ T at(int64_t i) const { return dp[i & mask]; }
 };
 
 // Sparse sources of one field: rowptr over internal rows, flat indices, signal table.
@@ -414,7 +418,7 @@ template <typename T> struct VelArgs {
     cx<T>* G[3];           // in: gradient spectra after the outer inverse passes; out: R2C_z(u_a)
     T* u[3];
     SMap<T> dtr[3];        // dt / rho0 staggered along a
-    const T* corr[3];      // additive k-space-corrected source contribution (dense), or null
+    SMap<T> corr[3];       // additive k-space-corrected source contribution (zero scalar when absent)
     SparseSrc<T> sp[3];
     int t;
 };
@@ -435,13 +439,13 @@ __global__ void __launch_bounds__(256) z_velocity(VelArgs<T> a, Geo g, ZTile zt0
         const bool sparse = sparse_in_tile(a.sp[ax].rowptr, g, zt);
         T* u = a.u[ax];
         const T* pmls = tab.pmlsg[ia];
-        const T* corr = a.corr[ax];
+        const SMap<T> corr = a.corr[ax];
         const SMap<T> dtr = a.dtr[ax];
         KWH_CELL_LOOP(zt, g, tid, nthr, {
             T& s = comp(R[n * zt.pitch + w], part);
             const T pm = pmls[axis_pos(g, row, n, ia)];
             T v = pm * (pm * u[cell] - dtr.at(cell) * s);
-            if (corr) v += corr[cell];
+            v += corr.at(cell);
             s = v;
             if (!sparse) u[cell] = v;
         })
@@ -466,7 +470,7 @@ template <typename T> struct RhoArgs {
     SMap<T> c0;
     SMap<T> bona;          // nonlinearity, active if nonlinear != 0
     SMap<T> tau;           // Stokes absorption coefficient (mode 0 with stokes != 0)
-    const T* corr;         // additive corrected p-source field (base scale), or null
+    SMap<T> corr;          // additive corrected p-source field (base scale); zero scalar when absent
     T corr_factor[3];      // 1/d_a
     SparseSrc<T> sp[3];
     T* p;                  // mode 0 output
@@ -514,7 +518,7 @@ __global__ void __launch_bounds__(256) z_density(RhoArgs<T> a, Geo g, ZTile zt0,
             if (a.nonlinear) mass *= (T(2) * comp(RP[n * zt.pitch + w], part) + r0) / r0;
             const T pm = pm_[axis_pos(g, row, n, ia)];
             T v = pm * (pm * rho[cell] - mass);
-            if (a.corr) v += a.corr[cell] * cf;
+            v += a.corr.at(cell) * cf;
             s = v;
             if (!sparse) rho[cell] = v;
             T& dv = comp(DIV[n * zt.pitch + w], part);

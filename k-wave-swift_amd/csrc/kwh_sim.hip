@@ -242,6 +242,17 @@ struct SimT final : kwh_sim {
     hipGraph_t graph = nullptr;
 
     size_t lds_budget = 64 * 1024;
+    // one-element device buffers behind scalar SMaps (branch-free at() in the kernels)
+    DBuf scal_pool;
+    int scal_n = 0;
+    SMap<T> zero_map{};
+    SMap<T> mk_map(const T* p) const { return SMap<T>{p, T(0), p, ~(int64_t)0}; }
+    SMap<T> mk_scalar(T v) {
+        KWH_REQUIRE(scal_n < 64, KWH_EUNSUPPORTED, "scalar pool exhausted");
+        T* d = scal_pool.as<T>() + scal_n++;
+        HIPCHK(hipMemcpy(d, &v, sizeof(T), hipMemcpyHostToDevice));
+        return SMap<T>{nullptr, v, d, 0};
+    }
     int sqR[3] = {0, 0, 0};   // R when the internal axis length is R*R with a register-resident kernel
     bool use_sq = true;
 
@@ -365,13 +376,12 @@ struct SimT final : kwh_sim {
             dim3 grid((unsigned)((c.ncols + ntr - 1) / ntr), nf);
             const size_t lds = ((size_t)c.L * ntr + R) * sizeof(C);
             timed(inv ? K_SI : K_SF, bytes, [&] {
-                if (R == 16) {
-                    if (inv) hipLaunchKernelGGL((sq_strided<T, 16, true>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
-                    else hipLaunchKernelGGL((sq_strided<T, 16, false>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
-                } else {
-                    if (inv) hipLaunchKernelGGL((sq_strided<T, 8, true>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
-                    else hipLaunchKernelGGL((sq_strided<T, 8, false>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
-                }
+#define KWH_SQS(RR, NN)                                                                                     \
+    if (inv) hipLaunchKernelGGL((sq_strided<T, RR, NN, true>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]); \
+    else hipLaunchKernelGGL((sq_strided<T, RR, NN, false>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
+                if (R == 16) { if (ntr == 16) { KWH_SQS(16, 16) } else { KWH_SQS(16, 8) } }
+                else { if (ntr == 32) { KWH_SQS(8, 32) } else { KWH_SQS(8, 16) } }
+#undef KWH_SQS
             });
             return;
         }
@@ -419,8 +429,13 @@ struct SimT final : kwh_sim {
             const size_t lds = ((size_t)a.L * ntr + R) * sizeof(C);
             const unsigned blocks = (unsigned)((c.ncols + ntr - 1) / ntr);
             timed(kid, bytes, [&] {
-                if (R == 16) hipLaunchKernelGGL((sq_middle<T, 16>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
-                else hipLaunchKernelGGL((sq_middle<T, 8>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
+                if (R == 16) {
+                    if (ntr == 16) hipLaunchKernelGGL((sq_middle<T, 16, 16>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
+                    else hipLaunchKernelGGL((sq_middle<T, 16, 8>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
+                } else {
+                    if (ntr == 32) hipLaunchKernelGGL((sq_middle<T, 8, 32>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
+                    else hipLaunchKernelGGL((sq_middle<T, 8, 16>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
+                }
             });
             return;
         }
@@ -493,6 +508,9 @@ struct SimT final : kwh_sim {
         HIPCHK(hipSetDevice(dev));
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         if (const char* e = std::getenv("KWH_LDS_BUDGET")) lds_budget = (size_t)std::atol(e);
+        scal_pool.alloc(64 * sizeof(T));
+        zero_map = mk_scalar(T(0));
+        taus = etas = bonas = zero_map;
         dim = d.dim;
         for (int a = 0; a < dim; ++a) { nphys[a] = d.n[a]; dphys[a] = d.d[a]; }
         dt = d.dt;
@@ -519,8 +537,7 @@ struct SimT final : kwh_sim {
         // ---- media (KSpaceFirstOrder.swift:573-579)
         auto upload_field = [&](const kwh_field& f, DBuf& buf, SMap<T>& sm, const char* name) {
             if (f.count == 1) {
-                sm.p = nullptr;
-                sm.s = T(f.data[0]);
+                sm = mk_scalar(T(f.data[0]));
                 return;
             }
             KWH_REQUIRE(f.count == N, KWH_EINVAL, std::string(name) + " must be scalar or grid-shaped");
@@ -532,8 +549,7 @@ struct SimT final : kwh_sim {
                                buf.as<T>(), T(1), N);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(stream));
-            sm.p = buf.as<T>();
-            sm.s = 0;
+            sm = mk_map(buf.as<T>());
         };
         upload_field(d.sound_speed, c0m, c0s, "soundSpeed");
         upload_field(d.density, rho0m, rho0s, "density");
@@ -702,9 +718,9 @@ struct SimT final : kwh_sim {
                 hipLaunchKernelGGL((stagger_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream, rho0m.as<T>(),
                                    dtrm[a].as<T>(), g.m0, g.m1, g.m2, a + g.off, dt);
                 HIPCHK(hipGetLastError());
-                dtrs[a] = SMap<T>{dtrm[a].as<T>(), T(0)};
+                dtrs[a] = mk_map(dtrm[a].as<T>());
             } else {
-                dtrs[a] = SMap<T>{nullptr, T(dt / (double)rho0s.s)};
+                dtrs[a] = mk_scalar(T(dt / (double)rho0s.s));
             }
         }
         if (absorbing) {
@@ -735,12 +751,12 @@ struct SimT final : kwh_sim {
                                    etam.as<T>());
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipStreamSynchronize(stream));
-                taus = SMap<T>{taum.as<T>(), 0};
-                etas = SMap<T>{etam.as<T>(), 0};
+                taus = mk_map(taum.as<T>());
+                etas = dispersion ? mk_map(etam.as<T>()) : zero_map;
             } else {
                 const double c = c0s.s, a = (double)as * db;
-                taus = SMap<T>{nullptr, T(stokes ? -2.0 * a * c : -2.0 * a * std::pow(c, y - 1.0))};
-                etas = SMap<T>{nullptr, T(2.0 * a * std::pow(c, y) * tanfac)};
+                taus = mk_scalar(T(stokes ? -2.0 * a * c : -2.0 * a * std::pow(c, y - 1.0)));
+                etas = mk_scalar(T(2.0 * a * std::pow(c, y) * tanfac));
             }
         }
         // B/A (:902-907): disabled when absent or all zero
@@ -755,11 +771,11 @@ struct SimT final : kwh_sim {
     }
 
     void upload_scalar_or_map(const kwh_field& f, DBuf& buf, SMap<T>& sm, const char* name) {
-        if (f.count == 1) { sm = SMap<T>{nullptr, T(f.data[0])}; return; }
+        if (f.count == 1) { sm = mk_scalar(T(f.data[0])); return; }
         KWH_REQUIRE(f.count == g.N, KWH_EINVAL, std::string(name) + " must be scalar or grid-shaped");
         std::vector<T> h(f.data, f.data + f.count);
         upload(buf, h);
-        sm = SMap<T>{buf.as<T>(), 0};
+        sm = mk_map(buf.as<T>());
     }
 
     // Smoothing in float64 (Filters.swift:273-347): window in k-space, then restore max|f|.
@@ -1196,15 +1212,18 @@ struct SimT final : kwh_sim {
                 va.G[a] = spec[a].as<C>();
                 va.u[a] = u[a].as<T>();
                 va.dtr[a] = dtrs[a];
-                va.corr[a] = cu[a];
+                va.corr[a] = cu[a] ? mk_map(cu[a]) : zero_map;
                 va.sp[a] = sparse_of(usrc[a], a);
             }
             va.t = t;
             double bytes = 0;
             for (int a = 0; a < dim; ++a)
                 bytes += 2 * spec_bytes() + 2 * real_bytes() + mapb(dtrs[a].p) + (cu[a] ? real_bytes() : 0);
+            bool anycorr = false;
+            for (int a = 0; a < dim; ++a) anycorr |= cu[a] != nullptr;
             if (sqR[2]) {
-                zlaunch(K_ZV, bytes, sq_z_velocity<T, 16>, sq_z_velocity<T, 8>, va, g);
+                if (anycorr) zlaunch(K_ZV, bytes, sq_z_velocity<T, 16, true>, sq_z_velocity<T, 8, true>, va, g);
+                else zlaunch(K_ZV, bytes, sq_z_velocity<T, 16, false>, sq_z_velocity<T, 8, false>, va, g);
             } else {
                 ZCfg zc = zcfg(2);
                 timed(K_ZV, bytes, [&] {
@@ -1257,7 +1276,7 @@ struct SimT final : kwh_sim {
             ra.c0 = c0s;
             ra.bona = bonas;
             ra.tau = taus;
-            ra.corr = cp;
+            ra.corr = cp ? mk_map(cp) : zero_map;
             ra.p = p.as<T>();
             ra.outP = spec[0].as<C>();
             ra.rhoT = rhoT.as<T>();
@@ -1275,13 +1294,18 @@ struct SimT final : kwh_sim {
             if (fft_abs) bytes += real_bytes() + spec_bytes() * (dispersion ? 2 : 1);
             else bytes += real_bytes() + mapb(c0s.p) + spec_bytes() + (stokes ? mapb(taus.p) : 0);
             if (sqR[2]) {
+                const bool cr = cp != nullptr;
+#define KWH_ZD(NL, MODE)                                                                                       \
+    (cr ? zlaunch(K_ZD, bytes, sq_z_density<T, 16, NL, MODE, true>, sq_z_density<T, 8, NL, MODE, true>, ra, g)    \
+        : zlaunch(K_ZD, bytes, sq_z_density<T, 16, NL, MODE, false>, sq_z_density<T, 8, NL, MODE, false>, ra, g))
                 if (nonlinear) {
-                    if (fft_abs) zlaunch(K_ZD, bytes, sq_z_density<T, 16, true, 1>, sq_z_density<T, 8, true, 1>, ra, g);
-                    else zlaunch(K_ZD, bytes, sq_z_density<T, 16, true, 0>, sq_z_density<T, 8, true, 0>, ra, g);
+                    if (fft_abs) KWH_ZD(true, 1);
+                    else KWH_ZD(true, 0);
                 } else {
-                    if (fft_abs) zlaunch(K_ZD, bytes, sq_z_density<T, 16, false, 1>, sq_z_density<T, 8, false, 1>, ra, g);
-                    else zlaunch(K_ZD, bytes, sq_z_density<T, 16, false, 0>, sq_z_density<T, 8, false, 0>, ra, g);
+                    if (fft_abs) KWH_ZD(false, 1);
+                    else KWH_ZD(false, 0);
                 }
+#undef KWH_ZD
             } else {
                 ZCfg zc = zcfg(nbuf);
                 timed(K_ZD, bytes, [&] {

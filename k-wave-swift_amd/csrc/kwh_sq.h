@@ -22,27 +22,39 @@ __device__ __forceinline__ void dftR(cx<T>* v) {
 // Contiguous-axis transforms of a row pair (a + i b), lanes j of transform w in a workgroup.
 // LDS per transform: L + R complex (one pad slot per R elements).
 // ------------------------------------------------------------------------------------------
+// Per-lane twiddles W_L^(j*r), r = 1..R-1, loaded once per kernel (same for every transform of
+// this lane; the inverse uses the conjugates).
+template <typename T, int R>
+struct LaneTw {
+    cx<T> w[R];
+    __device__ __forceinline__ void load(const cx<T>* __restrict__ tw, int j) {
+        w[0] = cx<T>{T(1), T(0)};
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[r] = tw[j * r];
+    }
+};
+
 template <typename T, int R>
 struct SqRow {
     static constexpr int L = R * R;
     static constexpr int LP = L + R + 1;
+    // padded LDS slot of element n is n + n / R; for the two access patterns used:
+    //   n = j*R + r -> j*(R+1) + r        n = j + R*r -> j + (R+1)*r
     __device__ static __forceinline__ int pidx(int n) { return n + n / R; }
 
     // Two-stage transform of v (lane-natural in, lane-natural out). lds: this transform's slab.
     template <bool INV>
-    __device__ static __forceinline__ void transform(cx<T>* v, cx<T>* lds, int j, const cx<T>* __restrict__ tw) {
+    __device__ static __forceinline__ void transform(cx<T>* v, cx<T>* lds, int j, const LaneTw<T, R>& tw) {
         dftR<T, R, INV>(v);
+        cx<T>* wr = lds + j * (R + 1);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[pidx(j * R + r)] = v[r];
+        for (int r = 0; r < R; ++r) wr[r] = v[r];
         __syncthreads();
+        const cx<T>* rd = lds + j;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            cx<T> x = lds[pidx(j + R * r)];
-            if (r > 0) {
-                cx<T> t = tw[j * r];
-                if (INV) t.y = -t.y;
-                x = cmul(x, t);
-            }
+            cx<T> x = rd[(R + 1) * r];
+            if (r > 0) x = cmul(x, INV ? conjv(tw.w[r]) : tw.w[r]);
             v[r] = x;
         }
         dftR<T, R, INV>(v);
@@ -54,31 +66,28 @@ struct SqRow {
     // (L - j) with negative compile-time offsets, so every load is base + immediate.
     __device__ static __forceinline__ void load_spectrum(const cx<T>* __restrict__ spec, int ra, int rb, int nrows,
                                                          int mzh, int j, cx<T>* v) {
-        const bool va = ra < nrows, vb = rb < nrows;
-        const cx<T>* sa = spec + (int64_t)(va ? ra : 0) * mzh;
-        const cx<T>* sb = spec + (int64_t)(vb ? rb : 0) * mzh;
+        const T ma = ra < nrows ? T(1) : T(0), mb = rb < nrows ? T(1) : T(0);
+        const cx<T>* sa = spec + (int64_t)(ra < nrows ? ra : 0) * mzh;
+        const cx<T>* sb = spec + (int64_t)(rb < nrows ? rb : 0) * mzh;
         const cx<T>* la = sa + j;
         const cx<T>* lb = sb + j;
         const cx<T>* ha = sa + (L - j);
         const cx<T>* hb = sb + (L - j);
-        const cx<T> zero{T(0), T(0)};
 #pragma unroll
         for (int r = 0; r < R / 2; ++r) {
-            const cx<T> A = va ? la[R * r] : zero;
-            const cx<T> B = vb ? lb[R * r] : zero;
+            const cx<T> A = la[R * r] * ma;
+            const cx<T> B = lb[R * r] * mb;
             v[r] = cx<T>{A.x - B.y, A.y + B.x};
         }
 #pragma unroll
         for (int r = R / 2; r < R; ++r) {
-            const cx<T> A = va ? ha[-R * r] : zero;
-            const cx<T> B = vb ? hb[-R * r] : zero;
+            const cx<T> A = ha[-R * r] * ma;
+            const cx<T> B = hb[-R * r] * mb;
             v[r] = cx<T>{A.x + B.y, B.x - A.y};
         }
         if (j == 0) {   // DC (n = 0) and Nyquist (n = L/2) bins are real parts only
-            const cx<T> A0 = va ? sa[0] : zero, B0 = vb ? sb[0] : zero;
-            const cx<T> AN = va ? sa[L / 2] : zero, BN = vb ? sb[L / 2] : zero;
-            v[0] = cx<T>{A0.x, B0.x};
-            v[R / 2] = cx<T>{AN.x, BN.x};
+            v[0] = cx<T>{sa[0].x * ma, sb[0].x * mb};
+            v[R / 2] = cx<T>{sa[L / 2].x * ma, sb[L / 2].x * mb};
         }
     }
 
@@ -96,20 +105,26 @@ struct SqRow {
 
     // Forward of v (a in .x, b in .y), split into half spectra, stored to rows (ra, rb).
     __device__ static __forceinline__ void forward_store(cx<T>* spec, int ra, int rb, int nrows, int mzh, cx<T>* lds,
-                                                         int j, const cx<T>* __restrict__ tw, cx<T>* v) {
+                                                         int j, const LaneTw<T, R>& tw, cx<T>* v) {
         transform<false>(v, lds, j, tw);
+        cx<T>* wr = lds + j;
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[pidx(j + R * r)] = v[r];
+        for (int r = 0; r < R; ++r) wr[(R + 1) * r] = v[r];
         __syncthreads();
         const bool va = ra < nrows, vb = rb < nrows;
+        // k = j + R m: slot j + (R+1) m; mirror L - k: slot (L - j + R - [j>0]) - (R+1) m (k > 0)
+        const cx<T>* pk = lds + j;
+        const cx<T>* pm = lds + (L - j + R - (j > 0 ? 1 : 0));
+        cx<T>* oa = spec + (int64_t)(va ? ra : 0) * mzh + j;
+        cx<T>* ob = spec + (int64_t)(vb ? rb : 0) * mzh + j;
 #pragma unroll
         for (int m = 0; m <= R / 2; ++m) {
             const int k = j + R * m;
             if (k > L / 2) break;
-            const cx<T> zk = lds[pidx(k)];
-            const cx<T> zm = lds[pidx(k == 0 ? 0 : L - k)];
-            if (va) spec[(int64_t)ra * mzh + k] = cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
-            if (vb) spec[(int64_t)rb * mzh + k] = cx<T>{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+            const cx<T> zk = pk[(R + 1) * m];
+            const cx<T> zm = k == 0 ? zk : pm[-(R + 1) * m];
+            if (va) oa[R * m] = cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+            if (vb) ob[R * m] = cx<T>{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
         }
         __syncthreads();
     }
@@ -173,7 +188,9 @@ __global__ void __launch_bounds__(256) sq_z_forward_real(const T* __restrict__ s
         v[r].x = va ? src[(int64_t)ps.ra * S::L + n] * scale : T(0);
         v[r].y = vb ? src[(int64_t)ps.rb * S::L + n] * scale : T(0);
     }
-    S::forward_store(spec, ps.ra, ps.rb, g.nrows, g.mzh, lds + ps.w * S::LP, ps.j, tab.tw[2], v);
+    LaneTw<T, R> lt;
+    lt.load(tab.tw[2], ps.j);
+    S::forward_store(spec, ps.ra, ps.rb, g.nrows, g.mzh, lds + ps.w * S::LP, ps.j, lt, v);
 }
 
 template <typename T, int R>
@@ -183,9 +200,11 @@ __global__ void __launch_bounds__(256) sq_z_inverse_real(const cx<T>* spec, T* _
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
     SqZPos<R> ps(P);
+    LaneTw<T, R> lt;
+    lt.load(tab.tw[2], ps.j);
     cx<T> v[R];
     S::load_spectrum(spec, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-    S::template transform<true>(v, lds + ps.w * S::LP, ps.j, tab.tw[2]);
+    S::template transform<true>(v, lds + ps.w * S::LP, ps.j, lt);
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -201,41 +220,65 @@ __global__ void __launch_bounds__(256) sq_z_inverse_real(const cx<T>* spec, T* _
     }
 }
 
+// Scheduling fence: keeps the compiler from hoisting the loads of later element groups above
+// earlier ones (bounds live registers; other waves hide the latency).
+#define KWH_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+constexpr int kGroup = 8;
+
 // Velocity update, per axis: C2R_z(G_a) -> u_a = pml_sg (pml_sg u_a - dt/rho_a g) [+ sources] -> R2C_z.
-template <typename T, int R>
+template <typename T, int R, bool CORR>
 __global__ void __launch_bounds__(256) sq_z_velocity(VelArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* lds_all = reinterpret_cast<cx<T>*>(smem);
     SqZPos<R> ps(P);
+    LaneTw<T, R> lt;
+    lt.load(tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
     for (int ax = 0; ax < g.dim; ++ax) {
         const int ia = ax + g.off;
         cx<T> v[R];
         S::load_spectrum(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-        S::template transform<true>(v, lds, ps.j, tab.tw[2]);
+        S::template transform<true>(v, lds, ps.j, lt);
+        KWH_SCHED_FENCE();
         T* u = a.u[ax];
-        const T* pm = tab.pmlsg[ia];
         const SMap<T> dtr = a.dtr[ax];
-        const T* corr = a.corr[ax];
-        const T pa = ia == 2 ? T(0) : pm[axis_pos(g, ra, 0, ia)];
-        const T pb = ia == 2 ? T(0) : pm[axis_pos(g, rb, 0, ia)];
-        const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+        const SMap<T> corr = a.corr[ax];
+        const T* pm = tab.pmlsg[ia];
         const T* ua = u + oa;
         const T* ub = u + ob;
-        const T* pz = pm + ps.j;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const T qa = ia == 2 ? pz[R * r] : pa, qb = ia == 2 ? pz[R * r] : pb;
-            const T da = dtr.p ? dtr.p[oa + R * r] : dtr.s;
-            const T db = dtr.p ? dtr.p[ob + R * r] : dtr.s;
-            T xa = qa * (qa * ua[R * r] - da * v[r].x);
-            T xb = qb * (qb * ub[R * r] - db * v[r].y);
-            if (corr) { xa += corr[oa + R * r]; xb += corr[ob + R * r]; }
+        auto upd = [&](int r, T qa, T qb) {
+            T xa = qa * (qa * ua[R * r] - dtr.at(oa + R * r) * v[r].x);
+            T xb = qb * (qb * ub[R * r] - dtr.at(ob + R * r) * v[r].y);
+            if constexpr (CORR) {
+                xa += corr.at(oa + R * r);
+                xb += corr.at(ob + R * r);
+            }
             v[r].x = va ? xa : T(0);
             v[r].y = vb ? xb : T(0);
+        };
+        if (ia == 2) {   // staggered PML along z varies with n and is shared by both rows
+            const T* pz = pm + ps.j;
+#pragma unroll
+            for (int g0 = 0; g0 < R; g0 += kGroup) {
+#pragma unroll
+                for (int r = g0; r < g0 + kGroup && r < R; ++r) {
+                    const T q = pz[R * r];
+                    upd(r, q, q);
+                }
+                KWH_SCHED_FENCE();
+            }
+        } else {
+            const T qa = pm[axis_pos(g, ra, 0, ia)], qb = pm[axis_pos(g, rb, 0, ia)];
+#pragma unroll
+            for (int g0 = 0; g0 < R; g0 += kGroup) {
+#pragma unroll
+                for (int r = g0; r < g0 + kGroup && r < R; ++r) upd(r, qa, qb);
+                KWH_SCHED_FENCE();
+            }
         }
         if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{(int)(xcd_tile(blockIdx.x, gridDim.x) * P), P, 0, 0}))
             sq_apply_sparse<T, R>(a.sp[ax], g, 2 * xcd_tile(blockIdx.x, gridDim.x) * P, 2 * P, lds_all, ps.w, ps.j,
@@ -249,19 +292,21 @@ __global__ void __launch_bounds__(256) sq_z_velocity(VelArgs<T> a, Geo g, int P,
                 if (vb) wb[R * r] = v[r].y;
             }
         }
-        S::forward_store(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], v);
+        S::forward_store(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
     }
 }
 
 // Density update + EOS (MODE 0) or absorption inputs (MODE 1); see z_density. NL: B/A active.
 // Only the total density is accumulated per element; sum_a du_a (needed for Stokes absorption
 // and for rho0 div u) is the inverse transform of sum_a D_a, re-read from the divergence spectra.
-template <typename T, int R, bool NL, int MODE>
+template <typename T, int R, bool NL, int MODE, bool CORR>
 __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* lds_all = reinterpret_cast<cx<T>*>(smem);
     SqZPos<R> ps(P);
+    LaneTw<T, R> lt;
+    lt.load(tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
@@ -284,20 +329,16 @@ __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, 
         const int ia = ax + g.off;
         cx<T> v[R];
         S::load_spectrum(a.D[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-        S::template transform<true>(v, lds, ps.j, tab.tw[2]);
+        S::template transform<true>(v, lds, ps.j, lt);
+        KWH_SCHED_FENCE();
         T* rho = a.rho[ax];
         const T* pm = tab.pml[ia];
-        const T pa = ia == 2 ? T(0) : pm[axis_pos(g, ra, 0, ia)];
-        const T pb = ia == 2 ? T(0) : pm[axis_pos(g, rb, 0, ia)];
-        const T* pz = pm + ps.j;
         const T cf = a.corr_factor[ax];
         const T* rha = rho + oa;
         const T* rhb = rho + ob;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const T qa = ia == 2 ? pz[R * r] : pa, qb = ia == 2 ? pz[R * r] : pb;
-            const T r0a = a.rho0.p ? a.rho0.p[oa + R * r] : a.rho0.s;
-            const T r0b = a.rho0.p ? a.rho0.p[ob + R * r] : a.rho0.s;
+        auto upd = [&](int r, T qa, T qb) {
+            const T r0a = a.rho0.at(oa + R * r);
+            const T r0b = a.rho0.at(ob + R * r);
             T ma = a.dt * r0a * v[r].x, mb = a.dt * r0b * v[r].y;
             if constexpr (NL) {
                 ma *= (T(2) * rp[r].x + r0a) / r0a;
@@ -305,8 +346,31 @@ __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, 
             }
             T xa = qa * (qa * rha[R * r] - ma);
             T xb = qb * (qb * rhb[R * r] - mb);
-            if (a.corr) { xa += a.corr[oa + R * r] * cf; xb += a.corr[ob + R * r] * cf; }
+            if constexpr (CORR) {
+                xa += a.corr.at(oa + R * r) * cf;
+                xb += a.corr.at(ob + R * r) * cf;
+            }
             v[r] = cx<T>{xa, xb};
+        };
+        if (ia == 2) {
+            const T* pz = pm + ps.j;
+#pragma unroll
+            for (int g0 = 0; g0 < R; g0 += kGroup) {
+#pragma unroll
+                for (int r = g0; r < g0 + kGroup && r < R; ++r) {
+                    const T q = pz[R * r];
+                    upd(r, q, q);
+                }
+                KWH_SCHED_FENCE();
+            }
+        } else {
+            const T qa = pm[axis_pos(g, ra, 0, ia)], qb = pm[axis_pos(g, rb, 0, ia)];
+#pragma unroll
+            for (int g0 = 0; g0 < R; g0 += kGroup) {
+#pragma unroll
+                for (int r = g0; r < g0 + kGroup && r < R; ++r) upd(r, qa, qb);
+                KWH_SCHED_FENCE();
+            }
         }
         if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{row0 / 2, P, 0, 0}))
             sq_apply_sparse<T, R>(a.sp[ax], g, row0, 2 * P, lds_all, ps.w, ps.j, a.t, v);
@@ -324,14 +388,14 @@ __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, 
     cx<T> v[R];
     if (need_div) {
         S::load_spectrum_sum(a.D, g.dim, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-        S::template transform<true>(v, lds, ps.j, tab.tw[2]);
+        S::template transform<true>(v, lds, ps.j, lt);
     }
     if constexpr (MODE == 0) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             T ea = rt[r].x, eb = rt[r].y;
-            const T r0a = a.rho0.p ? a.rho0.p[oa + R * r] : a.rho0.s;
-            const T r0b = a.rho0.p ? a.rho0.p[ob + R * r] : a.rho0.s;
+            const T r0a = a.rho0.at(oa + R * r);
+            const T r0b = a.rho0.at(ob + R * r);
             if (a.stokes) {
                 ea += a.tau.at(oa + R * r) * r0a * v[r].x;
                 eb += a.tau.at(ob + R * r) * r0b * v[r].y;
@@ -340,29 +404,29 @@ __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, 
                 ea += a.bona.at(oa + R * r) * (rt[r].x * rt[r].x) / (T(2) * r0a);
                 eb += a.bona.at(ob + R * r) * (rt[r].y * rt[r].y) / (T(2) * r0b);
             }
-            const T cA = a.c0.p ? a.c0.p[oa + R * r] : a.c0.s;
-            const T cB = a.c0.p ? a.c0.p[ob + R * r] : a.c0.s;
+            const T cA = a.c0.at(oa + R * r);
+            const T cB = a.c0.at(ob + R * r);
             const T pA = cA * cA * ea, pB = cB * cB * eb;
             if (va) a.p[oa + R * r] = pA;
             if (vb) a.p[ob + R * r] = pB;
             v[r] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
         }
-        S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], v);
+        S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
     } else {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const T r0a = a.rho0.p ? a.rho0.p[oa + R * r] : a.rho0.s;
-            const T r0b = a.rho0.p ? a.rho0.p[ob + R * r] : a.rho0.s;
+            const T r0a = a.rho0.at(oa + R * r);
+            const T r0b = a.rho0.at(ob + R * r);
             v[r] = cx<T>{va ? r0a * v[r].x : T(0), vb ? r0b * v[r].y : T(0)};
         }
-        S::forward_store(a.F1, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], v);
+        S::forward_store(a.F1, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (va) a.rhoT[oa + R * r] = rt[r].x;
             if (vb) a.rhoT[ob + R * r] = rt[r].y;
             rt[r] = cx<T>{va ? rt[r].x : T(0), vb ? rt[r].y : T(0)};
         }
-        if (a.dispersion) S::forward_store(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], rt);
+        if (a.dispersion) S::forward_store(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, rt);
     }
 }
 
@@ -372,15 +436,17 @@ __global__ void __launch_bounds__(256) sq_z_eos(EosArgs<T> a, Geo g, int P, Axis
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
     SqZPos<R> ps(P);
+    LaneTw<T, R> lt;
+    lt.load(tab.tw[2], ps.j);
     lds += ps.w * S::LP;
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
     cx<T> e[R], v[R];
     S::load_spectrum(a.A1, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, e);
-    S::template transform<true>(e, lds, ps.j, tab.tw[2]);
+    S::template transform<true>(e, lds, ps.j, lt);
     if (a.A2) {
         S::load_spectrum(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-        S::template transform<true>(v, lds, ps.j, tab.tw[2]);
+        S::template transform<true>(v, lds, ps.j, lt);
     }
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
 #pragma unroll
@@ -403,54 +469,48 @@ __global__ void __launch_bounds__(256) sq_z_eos(EosArgs<T> a, Geo g, int P, Axis
         if (vb) a.p[cb] = pB;
         e[r] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
     }
-    S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, tab.tw[2], e);
+    S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, e);
 }
 
 // ------------------------------------------------------------------------------------------
 // Strided / middle passes over a tile of ntr contiguous columns: lane (w, j), w fastest.
 // LDS: element n of column w at n * ntr + w + n / R (pad one slot per R rows).
 // ------------------------------------------------------------------------------------------
-template <typename T, int R>
+template <typename T, int R, int NTR>
 struct SqCol {
     static constexpr int L = R * R;
-    __device__ static __forceinline__ int lidx(int n, int w, int ntr) { return n * ntr + w + n / R; }
-
+    // element n of column w at n*NTR + w + n/R:
+    //   stage A write n = j*R + r -> (j*R*NTR + w + j) + r*NTR
+    //   stage B read  n = j + R*r -> (j*NTR + w) + r*(R*NTR + 1)
     template <bool INV>
-    __device__ static __forceinline__ void stageB(cx<T>* v, const cx<T>* lds, int j, int w, int ntr,
-                                                  const cx<T>* __restrict__ tw) {
+    __device__ static __forceinline__ void transform(cx<T>* v, cx<T>* lds, int j, int w, const LaneTw<T, R>& tw) {
+        dftR<T, R, INV>(v);
+        cx<T>* wr = lds + (j * R * NTR + w + j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) wr[r * NTR] = v[r];
+        __syncthreads();
+        const cx<T>* rd = lds + (j * NTR + w);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            cx<T> x = lds[lidx(j + R * r, w, ntr)];
-            if (r > 0) {
-                cx<T> t = tw[j * r];
-                if (INV) t.y = -t.y;
-                x = cmul(x, t);
-            }
+            cx<T> x = rd[r * (R * NTR + 1)];
+            if (r > 0) x = cmul(x, INV ? conjv(tw.w[r]) : tw.w[r]);
             v[r] = x;
         }
         dftR<T, R, INV>(v);
-    }
-    template <bool INV>
-    __device__ static __forceinline__ void transform(cx<T>* v, cx<T>* lds, int j, int w, int ntr,
-                                                     const cx<T>* __restrict__ tw) {
-        dftR<T, R, INV>(v);
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds[lidx(j * R + r, w, ntr)] = v[r];
-        __syncthreads();
-        stageB<INV>(v, lds, j, w, ntr, tw);
         __syncthreads();
     }
 };
 
-template <typename T, int R, bool INV>
+template <typename T, int R, int NTR, bool INV>
 __global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __restrict__ tw) {
-    using S = SqCol<T, R>;
+    using S = SqCol<T, R, NTR>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
-    const int ntr = 1 << a.lg_ntr;
-    const int w = threadIdx.x & (ntr - 1), j = threadIdx.x >> a.lg_ntr;
+    const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
+    LaneTw<T, R> lt;
+    lt.load(tw, j);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t c = (int64_t)tile * ntr + w;
+    const int64_t c = (int64_t)tile * NTR + w;
     const bool valid = c < a.ncols;
     const int64_t cc = valid ? c : 0;
     const int64_t base = (cc / a.inner) * (int64_t)S::L * a.inner + cc % a.inner;
@@ -461,11 +521,11 @@ __global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __
         const cx<T>* q = data;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            v[r] = valid ? *q : cx<T>{T(0), T(0)};
+            v[r] = *q;
             q += step;
         }
     }
-    S::template transform<INV>(v, lds, j, w, ntr, tw);
+    S::template transform<INV>(v, lds, j, w, lt);
     if (valid) {
         cx<T>* q = data;
 #pragma unroll
@@ -478,20 +538,20 @@ __global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __
 
 // Middle pass: forward along the middle axis, operators, inverse, store. Every table-using
 // operator of one input reads the same real k-space table (checked on the host), loaded once.
-template <typename T, int R>
+template <typename T, int R, int NTR>
 __global__ void __launch_bounds__(256) sq_middle(MidArgs a, AxisTabs<T> tab, Geo g) {
-    using S = SqCol<T, R>;
+    using S = SqCol<T, R, NTR>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
-    const int ntr = 1 << a.lg_ntr;
-    const int w = threadIdx.x & (ntr - 1), j = threadIdx.x >> a.lg_ntr;
+    const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
+    LaneTw<T, R> lt;
+    lt.load(tab.tw[a.axis], j);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t c = (int64_t)tile * ntr + w;
+    const int64_t c = (int64_t)tile * NTR + w;
     const bool valid = c < a.ncols;
     const int64_t cc = valid ? c : 0;
     const int64_t base = (cc / a.inner) * (int64_t)S::L * a.inner + cc % a.inner + (int64_t)j * a.stride;
     const int64_t step = (int64_t)R * a.stride;
-    const cx<T>* tw = tab.tw[a.axis];
     const T scale = T(a.scale);
     int i0c, i1c, i2c;
     if (a.axis == 0) { i0c = 0; i1c = (int)(cc / g.mzh); i2c = (int)(cc % g.mzh); }
@@ -505,11 +565,11 @@ __global__ void __launch_bounds__(256) sq_middle(MidArgs a, AxisTabs<T> tab, Geo
             const cx<T>* q = reinterpret_cast<const cx<T>*>(a.in[f]) + base;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                X[r] = valid ? *q : cx<T>{T(0), T(0)};
+                X[r] = *q;
                 q += step;
             }
         }
-        S::template transform<false>(X, lds, j, w, ntr, tw);
+        S::template transform<false>(X, lds, j, w, lt);
         const T* ktab = nullptr;
         for (int o = 0; o < a.nop; ++o)
             if (a.op[o].in == f && a.op[o].table) { ktab = reinterpret_cast<const T*>(a.op[o].table); break; }
@@ -540,7 +600,7 @@ __global__ void __launch_bounds__(256) sq_middle(MidArgs a, AxisTabs<T> tab, Geo
                 const T s = use_kappa ? kap[r] : scale;
                 v[r] = cmul(X[r], cx<T>{m.x * s, m.y * s});
             }
-            S::template transform<true>(v, lds, j, w, ntr, tw);
+            S::template transform<true>(v, lds, j, w, lt);
             if (valid) {
                 cx<T>* q = reinterpret_cast<cx<T>*>(op.out) + base;
 #pragma unroll
