@@ -61,6 +61,20 @@ def workload(name, dtype):
     return grid, med, src, sens, opts, info, model
 
 
+def measured_traffic(workload, dtype, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), or None when that kernel was not counted."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    rec = table.get(f"{workload}/{dtype}/{kernel}")
+    if not rec:
+        return None
+    return {"bytes": rec["fetch_bytes"] + rec["write_bytes"], "source": rec["summary"]}
+
+
 def cpu_baseline(grid, med, src, sens, opts, steps, workers):
     """Time the float64 oracle (numpy/scipy pocketfft, `workers` threads) on `steps` steps."""
     from oracle import kwave_oracle as O
@@ -90,22 +104,38 @@ def main():
     total = args.warmup + args.steps
     if total > grid.nt:
         grid.setTime(total, grid.dt)
-    sim = Simulation(grid, med, src, sens, opts)
-    setup_s = time.perf_counter() - t_setup
+    media_s = time.perf_counter() - t_setup
+    t_create = time.perf_counter()
+    sim = Simulation(grid, med, src, sens, opts)   # host -> device upload + operator setup
+    create_s = time.perf_counter() - t_create
+    setup_s = media_s + create_s
     print(f"[bench] setup done in {setup_s:.1f}s ({info['workload']})", file=sys.stderr, flush=True)
 
+    # Warmup: every kernel launch timed with HIP events, to name the dominant kernel.
+    timing = not args.no_kernel_timing
+    if timing:
+        sim.set_kernel_timing(True)
     sim.launch(0, args.warmup)
     sim.sync()
+    warm_stats = sim.kernel_stats() if timing else []
     print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
-    if not args.no_kernel_timing:
-        sim.set_kernel_timing(True)
+    dom_name = max(warm_stats, key=lambda x: x["total_ms"])["name"] if warm_stats else None
+    # Timed region: events only around the dominant kernel's launches (on the simulation stream).
+    if dom_name:
+        sim.set_kernel_timing(True, only=dom_name)
+    else:
+        sim.set_kernel_timing(False)
     sim.sync()
     t0 = time.perf_counter()
     sim.launch(args.warmup, total)
     sim.sync()
     elapsed = time.perf_counter() - t0
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
-    stats = sim.kernel_stats() if not args.no_kernel_timing else []
+    stats = sim.kernel_stats() if dom_name else []
+    t_read = time.perf_counter()
+    outputs = sim.outputs()                         # device -> host copy of the recorded series
+    read_s = time.perf_counter() - t_read
+    del outputs
     N = grid.totalGridPoints
     value = N * args.steps / elapsed / 1e6
     ms_step = elapsed / args.steps * 1e3
@@ -114,18 +144,22 @@ def main():
 
     roof = None
     kernels = []
-    for st in stats:
+    for st in warm_stats:
         avg_ms = st["total_ms"] / max(st["launches"], 1)
         kernels.append({"name": st["name"], "launches": st["launches"], "avg_us": round(avg_ms * 1e3, 2),
-                        "ms_per_step": round(st["total_ms"] / args.steps, 4),
+                        "ms_per_step": round(st["total_ms"] / max(args.warmup, 1), 4),
                         "GBps": round(st["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None})
-    if stats:
-        dom = max(stats, key=lambda x: x["total_ms"])
+    dom = next((st for st in stats if st["name"] == dom_name and st["launches"]), None)
+    if dom:
         avg_s = dom["total_ms"] / dom["launches"] * 1e-3
         ach = dom["bytes_per_launch"] / avg_s / 1e9
+        traffic = measured_traffic(args.workload, args.dtype, dom["name"])
         roof = {"bound": "hbm", "kernel": dom["name"], "achieved": round(ach, 1), "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": round(ach * 1e9 / HBM_PEAK, 4), "traffic": None,
-                "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_us": round(avg_s * 1e6, 2)}
+                "unit": "GB/s", "frac": round(ach * 1e9 / HBM_PEAK, 4),
+                "traffic": traffic["bytes"] if traffic else None,
+                "traffic_source": traffic["source"] if traffic else None,
+                "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_us": round(avg_s * 1e6, 2),
+                "launches_timed": dom["launches"]}
     step_model = {"bytes_per_step": model_bytes, "GBps": round(model_bytes / (ms_step * 1e-3) / 1e9, 1),
                   "frac_of_8TBps": round(model_bytes / (ms_step * 1e-3) / HBM_PEAK, 4)}
     sim.close()
@@ -145,9 +179,12 @@ def main():
         "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": args.dtype, "data": "synthetic (seeded random media)",
-        "config": dict(info, dtype=args.dtype, kernel_events=not args.no_kernel_timing),
+        "config": dict(info, dtype=args.dtype),
         "roofline": roof, "step_model": step_model, "kernels": kernels, "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
+        # whole job seen from the host boundary: upload + setup, the timed steps, read-back
+        "host_boundary": {"create_s": round(create_s, 3), "read_s": round(read_s, 3),
+                          "Mcell_steps_per_s": round(N * args.steps / (create_s + elapsed + read_s) / 1e6, 2)},
     }
     print(json.dumps(line), flush=True)
 

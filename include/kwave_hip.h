@@ -142,7 +142,7 @@ typedef struct {
     char name[48];
     int64_t launches;
     double total_ms;          /* summed HIP-event durations over the timed launches */
-    double bytes_per_launch;  /* algorithmic bytes moved by one launch */
+    double bytes_per_launch;  /* algorithmic bytes moved per launch, averaged over the timed launches */
 } kwh_kernel_stat;
 
 /* ABI version compiled into the library. */
@@ -192,6 +192,9 @@ int kwh_sim_load_state(kwh_sim* sim, const void* src, size_t bytes);
 
 /* Kernel timing with HIP events around every launch (off by default; adds event overhead). */
 int kwh_sim_set_kernel_timing(kwh_sim* sim, int enable);
+/* Event timing of one kernel only (a name kwh_sim_kernel_stats reports, e.g. "z_density"):
+ * two events per launch of that kernel, none elsewhere. Resets the statistics. */
+int kwh_sim_set_kernel_timing_only(kwh_sim* sim, const char* kernel_name);
 int kwh_sim_kernel_stats(kwh_sim* sim, kwh_kernel_stat* stats, int32_t capacity, int32_t* count);
 
 /* Capture one time step into a hipGraph and replay it for subsequent steps (1 = on). */

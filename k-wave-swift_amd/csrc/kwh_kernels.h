@@ -324,14 +324,18 @@ __device__ __forceinline__ int axis_pos(const Geo& g, int row, int n, int ax) {
 }
 
 // Scalar-or-map accessor. `dp` always points at device memory: the map itself (mask = ~0) or a
-// one-element buffer holding the scalar (mask = 0), so at() is a branch-free load. `p` / `s`
-// keep the host-side view (p == nullptr for a scalar).
+// buffer holding the scalar (mask = 0), so at() is a branch-free load. A scalar's buffer holds
+// kScalarRep copies: lane(off, j) returns a per-lane pointer whose [R*r] elements (j + R*r <
+// kScalarRep) are the map's row elements or the scalar, so the register-resident kernels address
+// every element as base + immediate. `p` / `s` keep the host-side view (p == nullptr: scalar).
+constexpr int kScalarRep = 256;
 template <typename T> struct SMap {
     const T* p;
     T s;
     const T* dp;
     int64_t mask;
     __device__ __forceinline__ T at(int64_t i) const { return dp[i & mask]; }
+    __device__ __forceinline__ const T* lane(int64_t off, int j) const { return p ? p + off : dp + j; }
 };
 
 // Sparse sources of one field: rowptr over internal rows, flat indices, signal table.
@@ -575,9 +579,26 @@ template <typename T> struct EosArgs {
     const T* rhoT;
     SMap<T> c0, rho0, tau, eta, bona;
     int nonlinear;
+    // 1: tau = tk c^te and eta = ek c^ee from the cell's sound speed (scalar alphaCoeff over a
+    // sound-speed map), instead of reading precomputed tau / eta maps
+    int tau_eta_from_c;
+    T tk, te, ek, ee;
     T* p;
     cx<T>* outP;
 };
+
+// Absorption coefficients of one cell (makeAbsorption, KSpaceFirstOrder.swift:855-878).
+template <typename T>
+__device__ __forceinline__ void eos_tau_eta(const EosArgs<T>& a, int64_t cell, T c, T& tau, T& eta) {
+    if (a.tau_eta_from_c) {
+        const T l = log2(c);
+        tau = a.tk * exp2(a.te * l);
+        eta = a.ek * exp2(a.ee * l);
+    } else {
+        tau = a.tau.at(cell);
+        eta = a.eta.at(cell);
+    }
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256) z_eos(EosArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {
@@ -592,14 +613,20 @@ __global__ void __launch_bounds__(256) z_eos(EosArgs<T> a, Geo g, ZTile zt0, Axi
     cx<T>* R = z_inverse<T>(a.A1, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);
     __syncthreads();
     KWH_CELL_LOOP(zt, g, tid, nthr, {
-        comp(E[n * zt.pitch + w], part) = a.rhoT[cell] + a.tau.at(cell) * comp(R[n * zt.pitch + w], part);
+        T tau = 0;
+        T eta = 0;
+        eos_tau_eta(a, cell, a.c0.at(cell), tau, eta);
+        comp(E[n * zt.pitch + w], part) = a.rhoT[cell] + tau * comp(R[n * zt.pitch + w], part);
     })
     __syncthreads();
     if (a.A2) {
         R = z_inverse<T>(a.A2, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);
         __syncthreads();
         KWH_CELL_LOOP(zt, g, tid, nthr, {
-            comp(E[n * zt.pitch + w], part) -= a.eta.at(cell) * comp(R[n * zt.pitch + w], part);
+            T tau = 0;
+            T eta = 0;
+            eos_tau_eta(a, cell, a.c0.at(cell), tau, eta);
+            comp(E[n * zt.pitch + w], part) -= eta * comp(R[n * zt.pitch + w], part);
         })
         __syncthreads();
     }
@@ -720,51 +747,55 @@ template <typename T> struct GatherArgs {
     int first;             // first recorded step: initialise max/min
 };
 
+// Up to four fields (p, ux, uy, uz) sampled by one launch: blockIdx.y selects the field.
+template <typename T> struct GatherSet {
+    GatherArgs<T> f[4];
+};
+
 template <typename T>
-__global__ void gather_kernel(GatherArgs<T> a) {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= a.n) return;
-    T v;
-    if (a.weight) {
-        v = T(0);
-        for (int c = 0; c < a.ncorner; ++c) v += a.field[a.idx[s * a.ncorner + c]] * a.weight[s * a.ncorner + c];
-    } else {
-        v = a.field[a.idx[s]];
+__global__ void gather_kernel(GatherSet<T> set) {
+    const GatherArgs<T>& a = set.f[blockIdx.y];
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)gridDim.x * blockDim.x) {
+        T v;
+        if (a.weight) {
+            v = T(0);
+            for (int c = 0; c < a.ncorner; ++c) v += a.field[a.idx[s * a.ncorner + c]] * a.weight[s * a.ncorner + c];
+        } else {
+            v = a.field[a.idx[s]];
+        }
+        if (a.series) a.series[(int64_t)a.t * a.n + s] = v;
+        if (a.vmax) a.vmax[s] = a.first ? v : fmax(a.vmax[s], v);
+        if (a.vmin) a.vmin[s] = a.first ? v : fmin(a.vmin[s], v);
+        if (a.sum2) a.sum2[s] = (a.first ? 0.0 : a.sum2[s]) + (double)v * (double)v;
     }
-    if (a.series) a.series[(int64_t)a.t * a.n + s] = v;
-    if (a.vmax) a.vmax[s] = a.first ? v : fmax(a.vmax[s], v);
-    if (a.vmin) a.vmin[s] = a.first ? v : fmin(a.vmin[s], v);
-    if (a.sum2) a.sum2[s] = (a.first ? 0.0 : a.sum2[s]) + (double)v * (double)v;
 }
 
 // Gather through output positions (a subset of sensor rows, e.g. one directivity angle).
 template <typename T>
 __global__ void gather_pos_kernel(const T* field, const int64_t* idx, const int64_t* pos, int64_t n, int64_t nsens,
                                   T* series, T* vmax, T* vmin, double* sum2, int t, int first) {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const T v = field[idx[s]];
-    const int64_t q = pos[s];
-    if (series) series[(int64_t)t * nsens + q] = v;
-    if (vmax) vmax[q] = first ? v : fmax(vmax[q], v);
-    if (vmin) vmin[q] = first ? v : fmin(vmin[q], v);
-    if (sum2) sum2[q] = (first ? 0.0 : sum2[q]) + (double)v * (double)v;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
+        const T v = field[idx[s]];
+        const int64_t q = pos[s];
+        if (series) series[(int64_t)t * nsens + q] = v;
+        if (vmax) vmax[q] = first ? v : fmax(vmax[q], v);
+        if (vmin) vmin[q] = first ? v : fmin(vmin[q], v);
+        if (sum2) sum2[q] = (first ? 0.0 : sum2[q]) + (double)v * (double)v;
+    }
 }
 
 // pRms / uRms = sqrt(sum2 / nt).
 template <typename T>
 __global__ void rms_kernel(const double* sum2, T* out, int64_t n, double inv_nt) {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s < n) out[s] = T(sqrt(sum2[s] * inv_nt));
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x)
+        out[s] = T(sqrt(sum2[s] * inv_nt));
 }
 
 // Intensity average (Recording.swift:113-124) from stored series [nt][n]:
 // I = (1/nt) sum_t p[t] * Re(ifft(e^{i w/2} fft(u)))[t] = (1/nt^2) Re sum_k conj(P_k) S_k U_k.
 // One thread per sensor point, direct DFT in float64 (cost n * nt^2).
 template <typename T>
-__global__ void intensity_kernel(const T* p, const T* u, T* out, int64_t n, int nt) {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= n) return;
+__device__ void intensity_point(const T* p, const T* u, T* out, int64_t n, int nt, int64_t s) {
     const double two_pi = 6.283185307179586476925286766559;
     double acc = 0.0;
     const int half = nt / 2;
@@ -788,6 +819,12 @@ __global__ void intensity_kernel(const T* p, const T* u, T* out, int64_t n, int 
         acc += pr * sur + pi * sui;
     }
     out[s] = T(acc / ((double)nt * (double)nt));
+}
+
+template <typename T>
+__global__ void intensity_kernel(const T* p, const T* u, T* out, int64_t n, int nt) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x)
+        intensity_point(p, u, out, n, nt, s);
 }
 
 }  // namespace kwh

@@ -160,8 +160,8 @@ __global__ void absorption_kernel(const T* c0, T cs, const T* alpha, T as, int64
 
 template <typename T>
 __global__ void gather_values_kernel(const T* f, const int64_t* idx, int64_t n, double* out) {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s < n) out[s] = (double)f[idx[s]];
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x)
+        out[s] = (double)f[idx[s]];
 }
 
 }  // namespace kwh

@@ -151,17 +151,19 @@ struct kwh_sim {
     virtual void save_state(void* dst, size_t bytes) = 0;
     virtual void load_state(const void* src, size_t bytes) = 0;
     virtual void set_timing(bool on) = 0;
+    virtual void set_timing_only(const char* name) = 0;
     virtual int kernel_stats(kwh_kernel_stat* st, int cap) = 0;
     virtual void set_graph(bool on) = 0;
 };
 
 namespace {
 
-constexpr int kNames = 16;
+constexpr int kNames = 17;
 const char* kKernelNames[kNames] = {"strided_fwd", "strided_inv", "middle_grad", "middle_div", "middle_abs",
                                     "z_velocity",  "z_density",   "z_eos",       "z_init",     "z_forward_real",
-                                    "z_inverse_real", "middle_table", "gather", "source_scatter", "fill", "other"};
-enum KId { K_SF, K_SI, K_MG, K_MD, K_MA, K_ZV, K_ZD, K_ZE, K_ZI, K_ZFR, K_ZIR, K_MT, K_GA, K_SC, K_FI, K_OT };
+                                    "z_inverse_real", "middle_table", "gather", "source_scatter", "fill", "other",
+                                    "z_colloc_rows"};
+enum KId { K_SF, K_SI, K_MG, K_MD, K_MA, K_ZV, K_ZD, K_ZE, K_ZI, K_ZFR, K_ZIR, K_MT, K_GA, K_SC, K_FI, K_OT, K_ZCR };
 
 template <typename T>
 struct SourceDev {
@@ -200,6 +202,8 @@ struct SimT final : kwh_sim {
     DBuf c0m, rho0m, dtrm[3], taum, etam, bonam;
     SMap<T> c0s{}, rho0s{}, dtrs[3]{}, taus{}, etas{}, bonas{};
     bool nonlinear = false, absorbing = false, stokes = false, dispersion = false;
+    bool tau_eta_from_c = false;        // tau = tk c^te, eta = ek c^ee evaluated in the EOS kernel
+    double tk = 0, te = 0, ek = 0, ee = 0;
 
     // state
     DBuf p, u[3], rho[3], rhoT, ucol[3], p0s;
@@ -220,6 +224,9 @@ struct SimT final : kwh_sim {
     bool rec_p = false, rec_u = false, store_p = false, store_u = false;
     bool need_u[3] = {false, false, false};   // collocated component a is computed each step
     int ucol_slot[3] = {-1, -1, -1};          // spectrum buffer index holding its colloc spectrum
+    // rows (flat / m2) read by the sensor; the contiguous-axis component is collocated on them only
+    DBuf zrows;
+    int nzrows = 0;
     DBuf ser_p, ser_u[3], pmax, pmin, psum2, umax[3], usum2[3];
     int rec_steps = 0;
     // directivity
@@ -230,6 +237,7 @@ struct SimT final : kwh_sim {
 
     // timing
     bool timing = false;
+    int timing_only = -1;   // >= 0: only launches of this kernel id are timed
     struct Pending { int id; hipEvent_t a, b; double bytes; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
@@ -249,8 +257,9 @@ struct SimT final : kwh_sim {
     SMap<T> mk_map(const T* p) const { return SMap<T>{p, T(0), p, ~(int64_t)0}; }
     SMap<T> mk_scalar(T v) {
         KWH_REQUIRE(scal_n < 64, KWH_EUNSUPPORTED, "scalar pool exhausted");
-        T* d = scal_pool.as<T>() + scal_n++;
-        HIPCHK(hipMemcpy(d, &v, sizeof(T), hipMemcpyHostToDevice));
+        T* d = scal_pool.as<T>() + (size_t)kScalarRep * scal_n++;
+        const std::vector<T> rep(kScalarRep, v);
+        HIPCHK(hipMemcpy(d, rep.data(), kScalarRep * sizeof(T), hipMemcpyHostToDevice));
         return SMap<T>{nullptr, v, d, 0};
     }
     int sqR[3] = {0, 0, 0};   // R when the internal axis length is R*R with a register-resident kernel
@@ -280,7 +289,7 @@ struct SimT final : kwh_sim {
     }
     template <typename F>
     void timed(int id, double bytes, F&& f) {
-        if (!timing) {
+        if (!timing || (timing_only >= 0 && id != timing_only)) {
             f();
             HIPCHK(hipGetLastError());
             return;
@@ -298,7 +307,7 @@ struct SimT final : kwh_sim {
             HIPCHK(hipEventElapsedTime(&ms, pd.a, pd.b));
             stat_launch[pd.id] += 1;
             stat_ms[pd.id] += ms;
-            stat_bytes[pd.id] = pd.bytes;
+            stat_bytes[pd.id] += pd.bytes;   // launches of one kernel may differ in size (fields per pass)
             ev_pool.push_back(pd.a);
             ev_pool.push_back(pd.b);
         }
@@ -374,7 +383,7 @@ struct SimT final : kwh_sim {
             a.lg_ntr = ilog2(ntr);
             a.pitch = ntr;
             dim3 grid((unsigned)((c.ncols + ntr - 1) / ntr), nf);
-            const size_t lds = ((size_t)c.L * ntr + R) * sizeof(C);
+            const size_t lds = ((size_t)c.L * ntr + R + R * R) * sizeof(C);   // slab + twiddle table
             timed(inv ? K_SI : K_SF, bytes, [&] {
 #define KWH_SQS(RR, NN)                                                                                     \
     if (inv) hipLaunchKernelGGL((sq_strided<T, RR, NN, true>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]); \
@@ -409,11 +418,16 @@ struct SimT final : kwh_sim {
         a.inner = c.inner;
         a.ncols = c.ncols;
         a.scale = 1.0 / (double)g.N;
-        // algorithmic bytes: inputs + outputs + one real k-space table read per table-using op
-        int ntab = 0;
-        for (auto& o : ops) if (o.kind == OP_DDXP_KAPPA || o.kind == OP_DDXN_KAPPA || o.kind == OP_TAB) ++ntab;
-        const double bytes = spec_bytes() * (a.nin + a.nop) + (double)g.NH * sizeof(T) * ntab;
         const int R = sqR[mid_axis];
+        // algorithmic bytes: inputs + outputs + one read of each distinct k-space table
+        std::vector<const void*> tabs_used;
+        for (int i = 0; i < a.nop; ++i) {
+            const MidOp& o = a.op[i];
+            if (o.table && std::find(tabs_used.begin(), tabs_used.end(), o.table) == tabs_used.end())
+                tabs_used.push_back(o.table);
+        }
+        const int ntab = (int)tabs_used.size();
+        const double bytes = spec_bytes() * (a.nin + a.nop) + (double)g.NH * sizeof(T) * ntab;
         if (R) {
             for (int f = 0; f < a.nin; ++f) {   // one k-space table per input (loaded once in-kernel)
                 const void* t0 = nullptr;
@@ -426,7 +440,7 @@ struct SimT final : kwh_sim {
             const int ntr = sq_ntr(R);
             a.lg_ntr = ilog2(ntr);
             a.pitch = ntr;
-            const size_t lds = ((size_t)a.L * ntr + R) * sizeof(C);
+            const size_t lds = ((size_t)a.L * ntr + R + R * R) * sizeof(C);   // slab + twiddle table
             const unsigned blocks = (unsigned)((c.ncols + ntr - 1) / ntr);
             timed(kid, bytes, [&] {
                 if (R == 16) {
@@ -448,14 +462,18 @@ struct SimT final : kwh_sim {
 
     // Launch configuration of the register-resident z kernels: P row pairs x R lanes.
     struct SqZ { int P, blocks, threads; size_t lds; };
+    // Row pairs per workgroup: 256 (FP32) or 128 (FP64) threads, halved until the LDS fits 64 KiB.
     SqZ sqz() const {
         const int R = sqR[2];
         SqZ z{};
         z.P = (sizeof(T) == 8 ? 128 : 256) / R;
+        const size_t per_pair = (size_t)(R * R + R + 1) * sizeof(C);
+        const size_t twb = (size_t)R * R * sizeof(C);   // LDS twiddle table
+        while (z.P > 1 && z.P * per_pair + twb > 65536) z.P /= 2;
         const int npairs = (g.nrows + 1) / 2;
         z.blocks = (npairs + z.P - 1) / z.P;
         z.threads = z.P * R;
-        z.lds = (size_t)z.P * (R * R + R + 1) * sizeof(C);
+        z.lds = (size_t)z.P * per_pair + twb;
         return z;
     }
     template <typename K, typename... Args>
@@ -508,7 +526,7 @@ struct SimT final : kwh_sim {
         HIPCHK(hipSetDevice(dev));
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         if (const char* e = std::getenv("KWH_LDS_BUDGET")) lds_budget = (size_t)std::atol(e);
-        scal_pool.alloc(64 * sizeof(T));
+        scal_pool.alloc((size_t)64 * kScalarRep * sizeof(T));
         zero_map = mk_scalar(T(0));
         taus = etas = bonas = zero_map;
         dim = d.dim;
@@ -743,7 +761,18 @@ struct SimT final : kwh_sim {
             }
             const bool map = c0s.p || ap;
             const double tanfac = std::tan(M_PI * y / 2.0);
-            if (map) {
+            if (c0s.p && !ap && !stokes) {
+                // scalar alphaCoeff over a sound-speed map: the EOS evaluates tau and eta per
+                // cell from c0 instead of reading two precomputed maps
+                const double a = (double)as * db;
+                tau_eta_from_c = true;
+                tk = -2.0 * a;
+                te = y - 1.0;
+                ek = 2.0 * a * tanfac;
+                ee = y;
+                taus = zero_map;
+                etas = zero_map;
+            } else if (map) {
                 taum.alloc(N * sizeof(T));
                 if (dispersion) etam.alloc(N * sizeof(T));
                 hipLaunchKernelGGL((absorption_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream, c0s.p, c0s.s,
@@ -1059,6 +1088,15 @@ struct SimT final : kwh_sim {
             ucol[a].alloc(N * sizeof(T));
             ucol_slot[a] = slot++;
         }
+        if (need_u[dim - 1]) {
+            std::vector<int> rl;
+            rl.reserve(idx.size());
+            for (int64_t f : idx) rl.push_back((int)(f / g.m2));
+            std::sort(rl.begin(), rl.end());
+            rl.erase(std::unique(rl.begin(), rl.end()), rl.end());
+            nzrows = (int)rl.size();
+            upload(zrows, rl);
+        }
         // 2D directivity on binary masks
         if (d.directivity_angle && dim == 2 && !cart && rec_p) {
             std::vector<double> ang(nsens);
@@ -1231,6 +1269,24 @@ struct SimT final : kwh_sim {
                 });
             }
         }
+        // collocated contiguous-axis velocity on the sensor rows only, from R2C_z(u) (:735-745)
+        const bool zcol = sqR[2] && need_u[dim - 1];
+        if (zcol && nzrows > 0) {
+            SqZ z = sqz();
+            const int npairs = (nzrows + 1) / 2;
+            const unsigned blocks = (unsigned)((npairs + z.P - 1) / z.P);
+            const C* s = spec[dim - 1].as<C>();
+            T* dst = ucol[dim - 1].as<T>();
+            timed(K_ZCR, (double)nzrows * g.m2 * (sizeof(T) + sizeof(C) / 2), [&] {
+                if (sqR[2] == 16)
+                    hipLaunchKernelGGL((sq_z_colloc_rows<T, 16>), dim3(blocks), dim3(z.threads), z.lds, stream, s, dst,
+                                       (const int*)zrows.p, nzrows, g, z.P, tab);
+                else
+                    hipLaunchKernelGGL((sq_z_colloc_rows<T, 8>), dim3(blocks), dim3(z.threads), z.lds, stream, s, dst,
+                                       (const int*)zrows.p, nzrows, g, z.P, tab);
+            });
+        }
+        auto full_colloc = [&](int a) { return need_u[a] && !(zcol && a == dim - 1); };
         // divergence (rhoStep :680-692)
         if (plain1) strided(1, false, {spec[0].p, spec[1].p, spec[2].p});
         {
@@ -1239,17 +1295,17 @@ struct SimT final : kwh_sim {
             for (int a = 0; a < dim; ++a) {
                 ins.push_back(spec[a].p);
                 ops.push_back(MidOp{OP_DDXN_KAPPA, a + g.off, a, spec[a].p, kappa.p});
-                if (need_u[a]) ops.push_back(MidOp{OP_COLL, a + g.off, a, spec[ucol_slot[a]].p, nullptr});
+                if (full_colloc(a)) ops.push_back(MidOp{OP_COLL, a + g.off, a, spec[ucol_slot[a]].p, nullptr});
             }
             middle(K_MD, ins, ops);
         }
         if (plain1) {
             strided(1, true, {spec[0].p, spec[1].p, spec[2].p});
             for (int a = 0; a < dim; ++a)
-                if (need_u[a]) strided(1, true, {spec[ucol_slot[a]].p});
+                if (full_colloc(a)) strided(1, true, {spec[ucol_slot[a]].p});
         }
         for (int a = 0; a < dim; ++a)
-            if (need_u[a]) {
+            if (full_colloc(a)) {
                 C* s = spec[ucol_slot[a]].as<C>();
                 T* dst = ucol[a].as<T>();
                 if (sqR[2]) {
@@ -1295,14 +1351,17 @@ struct SimT final : kwh_sim {
             else bytes += real_bytes() + mapb(c0s.p) + spec_bytes() + (stokes ? mapb(taus.p) : 0);
             if (sqR[2]) {
                 const bool cr = cp != nullptr;
-#define KWH_ZD(NL, MODE)                                                                                       \
-    (cr ? zlaunch(K_ZD, bytes, sq_z_density<T, 16, NL, MODE, true>, sq_z_density<T, 8, NL, MODE, true>, ra, g)    \
+                // MODE 0: EOS; 1: FFT absorption inputs; 2: EOS with Stokes absorption
+#define KWH_ZD(NL, MODE)                                                                                            \
+    (cr ? zlaunch(K_ZD, bytes, sq_z_density<T, 16, NL, MODE, true>, sq_z_density<T, 8, NL, MODE, true>, ra, g) \
         : zlaunch(K_ZD, bytes, sq_z_density<T, 16, NL, MODE, false>, sq_z_density<T, 8, NL, MODE, false>, ra, g))
                 if (nonlinear) {
                     if (fft_abs) KWH_ZD(true, 1);
+                    else if (stokes) KWH_ZD(true, 2);
                     else KWH_ZD(true, 0);
                 } else {
                     if (fft_abs) KWH_ZD(false, 1);
+                    else if (stokes) KWH_ZD(false, 2);
                     else KWH_ZD(false, 0);
                 }
 #undef KWH_ZD
@@ -1339,6 +1398,11 @@ struct SimT final : kwh_sim {
             ea.eta = etas;
             ea.bona = bonas;
             ea.nonlinear = nonlinear;
+            ea.tau_eta_from_c = tau_eta_from_c ? 1 : 0;
+            ea.tk = T(tk);
+            ea.te = T(te);
+            ea.ek = T(ek);
+            ea.ee = T(ee);
             ea.p = p.as<T>();
             ea.outP = spec[0].as<C>();
             double bytes = spec_bytes() * (dispersion ? 3 : 2) + 2 * real_bytes() + mapb(c0s.p) + mapb(taus.p) +
@@ -1370,15 +1434,23 @@ struct SimT final : kwh_sim {
                     gather(dir_field.as<T>(), dir_idx[k]->as<int64_t>(), nullptr, 1, dir_count[k], ser_p.as<T>(),
                            pmax.as<T>(), pmin.as<T>(), psum2.as<double>(), slot, first, dir_pos[k]->as<int64_t>());
                 }
-            } else {
-                gather(p.as<T>(), sidx.as<int64_t>(), sw.as<T>(), ncorner, nsens, ser_p.as<T>(), pmax.as<T>(),
-                       pmin.as<T>(), psum2.as<double>(), slot, first, nullptr);
             }
         }
+        // p (without directivity) and the collocated velocity components in one launch
+        GatherSet<T> set{};
+        int nf = 0;
+        if (rec_p && ndir == 0)
+            set.f[nf++] = GatherArgs<T>{p.as<T>(), sidx.as<int64_t>(), sw.as<T>(), ncorner, nsens, ser_p.as<T>(),
+                                        pmax.as<T>(), pmin.as<T>(), psum2.as<double>(), slot, first};
         for (int a = 0; a < dim; ++a)
             if (need_u[a])
-                gather(ucol[a].as<T>(), sidx.as<int64_t>(), sw.as<T>(), ncorner, nsens, ser_u[a].as<T>(), umax[a].as<T>(),
-                       nullptr, usum2[a].as<double>(), slot, first, nullptr);
+                set.f[nf++] = GatherArgs<T>{ucol[a].as<T>(), sidx.as<int64_t>(), sw.as<T>(), ncorner, nsens,
+                                            ser_u[a].as<T>(), umax[a].as<T>(), nullptr, usum2[a].as<double>(), slot,
+                                            first};
+        if (nf > 0)
+            timed(K_GA, (double)nf * nsens * sizeof(T) * 2, [&] {
+                hipLaunchKernelGGL((gather_kernel<T>), dim3(grid_blocks(nsens), nf), dim3(256), 0, stream, set);
+            });
         rec_steps++;
     }
 
@@ -1392,9 +1464,10 @@ struct SimT final : kwh_sim {
             });
             return;
         }
-        GatherArgs<T> a{field, idx, w, nc, n, series, vmax, vmin, sum2, slot, first};
+        GatherSet<T> set{};
+        set.f[0] = GatherArgs<T>{field, idx, w, nc, n, series, vmax, vmin, sum2, slot, first};
         timed(K_GA, (double)n * sizeof(T) * 2, [&] {
-            hipLaunchKernelGGL((gather_kernel<T>), dim3(grid_blocks(n)), dim3(256), 0, stream, a);
+            hipLaunchKernelGGL((gather_kernel<T>), dim3(grid_blocks(n)), dim3(256), 0, stream, set);
         });
     }
 
@@ -1581,7 +1654,16 @@ struct SimT final : kwh_sim {
     void set_timing(bool on) override {
         sync();
         timing = on;
+        timing_only = -1;
         for (int i = 0; i < kNames; ++i) { stat_launch[i] = 0; stat_ms[i] = 0; stat_bytes[i] = 0; }
+    }
+    void set_timing_only(const char* name) override {
+        int id = -1;
+        for (int i = 0; i < kNames; ++i)
+            if (std::strcmp(kKernelNames[i], name) == 0) id = i;
+        KWH_REQUIRE(id >= 0, KWH_EINVAL, "unknown kernel name");
+        set_timing(true);
+        timing_only = id;
     }
     int kernel_stats(kwh_kernel_stat* st, int cap) override {
         sync();
@@ -1592,7 +1674,7 @@ struct SimT final : kwh_sim {
                 std::snprintf(st[n].name, sizeof(st[n].name), "%s", kKernelNames[i]);
                 st[n].launches = stat_launch[i];
                 st[n].total_ms = stat_ms[i];
-                st[n].bytes_per_launch = stat_bytes[i];
+                st[n].bytes_per_launch = stat_bytes[i] / (double)stat_launch[i];
             }
             ++n;
         }
@@ -1820,6 +1902,13 @@ int kwh_sim_set_kernel_timing(kwh_sim* sim, int enable) {
     return guarded([&] {
         KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
         sim->set_timing(enable != 0);
+    });
+}
+
+int kwh_sim_set_kernel_timing_only(kwh_sim* sim, const char* kernel_name) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && kernel_name, KWH_EINVAL, "NULL argument");
+        sim->set_timing_only(kernel_name);
     });
 }
 

@@ -1,4 +1,4 @@
-// kwh_sq.h -- register-resident kernels for axis lengths L = R*R (R = 16: 256, R = 8: 64, R = 4: 16).
+// kwh_sq.h -- register-resident kernels for axis lengths L = R*R (R = 16: 256, R = 8: 64).
 //
 // A length-L transform is two radix-R stages. Lane j (0..R-1) of a transform holds elements
 // n = j + R*r (r = 0..R-1) in registers:
@@ -11,6 +11,22 @@
 #pragma once
 #include "kwh_kernels.h"
 
+// Occupancy floors (waves per SIMD) of the fused row kernels: without them the register
+// allocator spends the whole register file on load hoisting and drops a wave per SIMD. The
+// density kernels that also accumulate rho0 * div u stay at two.
+#ifndef KWH_DENSITY_WAVES
+#define KWH_DENSITY_WAVES 3
+#endif
+#ifndef KWH_VELOCITY_WAVES
+#define KWH_VELOCITY_WAVES 4
+#endif
+#ifndef KWH_MIDDLE_WAVES
+#define KWH_MIDDLE_WAVES 3
+#endif
+#ifndef KWH_EOS_WAVES
+#define KWH_EOS_WAVES 3
+#endif
+
 namespace kwh {
 
 template <typename T, int R, bool INV>
@@ -22,16 +38,19 @@ __device__ __forceinline__ void dftR(cx<T>* v) {
 // Contiguous-axis transforms of a row pair (a + i b), lanes j of transform w in a workgroup.
 // LDS per transform: L + R complex (one pad slot per R elements).
 // ------------------------------------------------------------------------------------------
-// Per-lane twiddles W_L^(j*r), r = 1..R-1, loaded once per kernel (same for every transform of
-// this lane; the inverse uses the conjugates).
+// Twiddles W_L^(j*r) of the second stage, in an LDS table laid out [r][j] (R*R elements at the
+// start of the workgroup's LDS): lane j reads slot j + R*r, a base + immediate address, and the
+// table costs no registers between transforms. The inverse uses the conjugates. The table is
+// filled cooperatively at kernel start; the barrier inside the first transform orders the fill
+// before any read.
 template <typename T, int R>
 struct LaneTw {
-    cx<T> w[R];
-    __device__ __forceinline__ void load(const cx<T>* __restrict__ tw, int j) {
-        w[0] = cx<T>{T(1), T(0)};
-#pragma unroll
-        for (int r = 1; r < R; ++r) w[r] = tw[j * r];
+    const cx<T>* p;
+    __device__ __forceinline__ void load(cx<T>* tab_lds, const cx<T>* __restrict__ tw, int j) {
+        for (int i = threadIdx.x; i < R * R; i += blockDim.x) tab_lds[i] = tw[(i % R) * (i / R)];
+        p = tab_lds + j;
     }
+    __device__ __forceinline__ cx<T> w(int r) const { return p[R * r]; }
 };
 
 template <typename T, int R>
@@ -54,7 +73,7 @@ struct SqRow {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             cx<T> x = rd[(R + 1) * r];
-            if (r > 0) x = cmul(x, INV ? conjv(tw.w[r]) : tw.w[r]);
+            if (r > 0) x = cmul(x, INV ? conjv(tw.w(r)) : tw.w(r));
             v[r] = x;
         }
         dftR<T, R, INV>(v);
@@ -178,8 +197,11 @@ __global__ void __launch_bounds__(256) sq_z_forward_real(const T* __restrict__ s
                                                          int P, AxisTabs<T> tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds = ldsb + R * R;
     SqZPos<R> ps(P);
+    LaneTw<T, R> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
     cx<T> v[R];
 #pragma unroll
@@ -188,8 +210,6 @@ __global__ void __launch_bounds__(256) sq_z_forward_real(const T* __restrict__ s
         v[r].x = va ? src[(int64_t)ps.ra * S::L + n] * scale : T(0);
         v[r].y = vb ? src[(int64_t)ps.rb * S::L + n] * scale : T(0);
     }
-    LaneTw<T, R> lt;
-    lt.load(tab.tw[2], ps.j);
     S::forward_store(spec, ps.ra, ps.rb, g.nrows, g.mzh, lds + ps.w * S::LP, ps.j, lt, v);
 }
 
@@ -198,10 +218,11 @@ __global__ void __launch_bounds__(256) sq_z_inverse_real(const cx<T>* spec, T* _
                                                          int accumulate, Geo g, int P, AxisTabs<T> tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds = ldsb + R * R;
     SqZPos<R> ps(P);
     LaneTw<T, R> lt;
-    lt.load(tab.tw[2], ps.j);
+    lt.load(ldsb, tab.tw[2], ps.j);
     cx<T> v[R];
     S::load_spectrum(spec, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
     S::template transform<true>(v, lds + ps.w * S::LP, ps.j, lt);
@@ -220,20 +241,58 @@ __global__ void __launch_bounds__(256) sq_z_inverse_real(const cx<T>* spec, T* _
     }
 }
 
+// Collocated velocity along the contiguous axis for a list of rows: dst[row] =
+// C2R_z(coll_z * Z[row]) / L, Z = R2C_z(u) of the updated component (as the velocity kernel
+// leaves it). The shift e^{-i k dz/2} only involves the contiguous axis, so the full 3-D
+// transform of the reference (KSpaceFirstOrder.swift:735-745) reduces to this 1-D one on the
+// rows the sensor reads. Rows are paired in list order; an odd tail pairs with nothing.
+template <typename T, int R>
+__global__ void __launch_bounds__(256) sq_z_colloc_rows(const cx<T>* spec, T* __restrict__ dst,
+                                                        const int* __restrict__ rows, int nlist, Geo g, int P,
+                                                        AxisTabs<T> tab) {
+    using S = SqRow<T, R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    SqZPos<R> ps(P);
+    LaneTw<T, R> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    const bool va = ps.ra < nlist, vb = ps.rb < nlist;
+    const int ra = va ? rows[ps.ra] : g.nrows;   // g.nrows: masked by load_spectrum
+    const int rb = vb ? rows[ps.rb] : g.nrows;
+    cx<T> v[R];
+    S::load_spectrum(spec, ra, rb, g.nrows, g.mzh, ps.j, v);
+    const cx<T>* co = tab.coll[2] + ps.j;
+    const T scale = T(1) / T(S::L);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const cx<T> c = co[R * r];
+        v[r] = cmul(v[r], cx<T>{c.x * scale, c.y * scale});
+    }
+    S::template transform<true>(v, ldsb + R * R + ps.w * S::LP, ps.j, lt);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int n = ps.j + R * r;
+        if (va) dst[(int64_t)ra * S::L + n] = v[r].x;
+        if (vb) dst[(int64_t)rb * S::L + n] = v[r].y;
+    }
+}
+
 // Scheduling fence: keeps the compiler from hoisting the loads of later element groups above
 // earlier ones (bounds live registers; other waves hide the latency).
 #define KWH_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
-constexpr int kGroup = 8;
+constexpr int kGroup = 8;      // density update
+constexpr int kVelGroup = 4;   // velocity update (keeps it within the three-wave register budget)
 
 // Velocity update, per axis: C2R_z(G_a) -> u_a = pml_sg (pml_sg u_a - dt/rho_a g) [+ sources] -> R2C_z.
 template <typename T, int R, bool CORR>
-__global__ void __launch_bounds__(256) sq_z_velocity(VelArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_VELOCITY_WAVES : 1) sq_z_velocity(VelArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* lds_all = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds_all = ldsb + R * R;
     SqZPos<R> ps(P);
     LaneTw<T, R> lt;
-    lt.load(tab.tw[2], ps.j);
+    lt.load(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
@@ -250,12 +309,16 @@ __global__ void __launch_bounds__(256) sq_z_velocity(VelArgs<T> a, Geo g, int P,
         const T* pm = tab.pmlsg[ia];
         const T* ua = u + oa;
         const T* ub = u + ob;
+        const T* dta = dtr.lane(oa, ps.j);
+        const T* dtb = dtr.lane(ob, ps.j);
+        const T* cra = corr.lane(oa, ps.j);
+        const T* crb = corr.lane(ob, ps.j);
         auto upd = [&](int r, T qa, T qb) {
-            T xa = qa * (qa * ua[R * r] - dtr.at(oa + R * r) * v[r].x);
-            T xb = qb * (qb * ub[R * r] - dtr.at(ob + R * r) * v[r].y);
+            T xa = qa * (qa * ua[R * r] - dta[R * r] * v[r].x);
+            T xb = qb * (qb * ub[R * r] - dtb[R * r] * v[r].y);
             if constexpr (CORR) {
-                xa += corr.at(oa + R * r);
-                xb += corr.at(ob + R * r);
+                xa += cra[R * r];
+                xb += crb[R * r];
             }
             v[r].x = va ? xa : T(0);
             v[r].y = vb ? xb : T(0);
@@ -263,9 +326,9 @@ __global__ void __launch_bounds__(256) sq_z_velocity(VelArgs<T> a, Geo g, int P,
         if (ia == 2) {   // staggered PML along z varies with n and is shared by both rows
             const T* pz = pm + ps.j;
 #pragma unroll
-            for (int g0 = 0; g0 < R; g0 += kGroup) {
+            for (int g0 = 0; g0 < R; g0 += kVelGroup) {
 #pragma unroll
-                for (int r = g0; r < g0 + kGroup && r < R; ++r) {
+                for (int r = g0; r < g0 + kVelGroup && r < R; ++r) {
                     const T q = pz[R * r];
                     upd(r, q, q);
                 }
@@ -274,9 +337,9 @@ __global__ void __launch_bounds__(256) sq_z_velocity(VelArgs<T> a, Geo g, int P,
         } else {
             const T qa = pm[axis_pos(g, ra, 0, ia)], qb = pm[axis_pos(g, rb, 0, ia)];
 #pragma unroll
-            for (int g0 = 0; g0 < R; g0 += kGroup) {
+            for (int g0 = 0; g0 < R; g0 += kVelGroup) {
 #pragma unroll
-                for (int r = g0; r < g0 + kGroup && r < R; ++r) upd(r, qa, qb);
+                for (int r = g0; r < g0 + kVelGroup && r < R; ++r) upd(r, qa, qb);
                 KWH_SCHED_FENCE();
             }
         }
@@ -296,23 +359,30 @@ __global__ void __launch_bounds__(256) sq_z_velocity(VelArgs<T> a, Geo g, int P,
     }
 }
 
-// Density update + EOS (MODE 0) or absorption inputs (MODE 1); see z_density. NL: B/A active.
-// Only the total density is accumulated per element; sum_a du_a (needed for Stokes absorption
-// and for rho0 div u) is the inverse transform of sum_a D_a, re-read from the divergence spectra.
+// Density update, per axis: C2R_z(D_a) -> rho_a = pml (pml rho_a - dt rho0 [nl] du_a) [+ sources],
+// followed by the EOS (MODE 0), the EOS with Stokes absorption (MODE 2) or the inputs of the
+// FFT absorption term (MODE 1: rhoT, F1 = R2C_z(rho0 sum_a du_a), F2 = R2C_z(rhoT)). NL: B/A.
+// The total density and, for MODE 1/2, rho0 * sum_a du_a are accumulated in registers while the
+// axes are processed, so each divergence spectrum is read once.
 template <typename T, int R, bool NL, int MODE, bool CORR>
-__global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+__global__ void __launch_bounds__(256, (NL || CORR || sizeof(T) != 4) ? 1 : (MODE == 0 ? KWH_DENSITY_WAVES : 2)) sq_z_density(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
     using S = SqRow<T, R>;
+    constexpr bool ACC = MODE != 0;   // accumulate rho0 * sum_a du_a
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* lds_all = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds_all = ldsb + R * R;
     SqZPos<R> ps(P);
     LaneTw<T, R> lt;
-    lt.load(tab.tw[2], ps.j);
+    lt.load(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
     const int row0 = 2 * xcd_tile(blockIdx.x, gridDim.x) * P;
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+    const T* r0pa = a.rho0.lane(oa, ps.j);
+    const T* r0pb = a.rho0.lane(ob, ps.j);
     cx<T> rt[R];
+    cx<T> sd[ACC ? R : 1];
     cx<T> rp[NL ? R : 1];
     if constexpr (NL) {
 #pragma unroll
@@ -336,10 +406,17 @@ __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, 
         const T cf = a.corr_factor[ax];
         const T* rha = rho + oa;
         const T* rhb = rho + ob;
+        const T* cra = a.corr.lane(oa, ps.j);
+        const T* crb = a.corr.lane(ob, ps.j);
         auto upd = [&](int r, T qa, T qb) {
-            const T r0a = a.rho0.at(oa + R * r);
-            const T r0b = a.rho0.at(ob + R * r);
-            T ma = a.dt * r0a * v[r].x, mb = a.dt * r0b * v[r].y;
+            const T r0a = r0pa[R * r];
+            const T r0b = r0pb[R * r];
+            const T da = r0a * v[r].x, db = r0b * v[r].y;
+            if constexpr (ACC) {
+                if (ax == 0) sd[r] = cx<T>{da, db};
+                else { sd[r].x += da; sd[r].y += db; }
+            }
+            T ma = a.dt * da, mb = a.dt * db;
             if constexpr (NL) {
                 ma *= (T(2) * rp[r].x + r0a) / r0a;
                 mb *= (T(2) * rp[r].y + r0b) / r0b;
@@ -347,8 +424,8 @@ __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, 
             T xa = qa * (qa * rha[R * r] - ma);
             T xb = qb * (qb * rhb[R * r] - mb);
             if constexpr (CORR) {
-                xa += a.corr.at(oa + R * r) * cf;
-                xb += a.corr.at(ob + R * r) * cf;
+                xa += cra[R * r] * cf;
+                xb += crb[R * r] * cf;
             }
             v[r] = cx<T>{xa, xb};
         };
@@ -384,28 +461,29 @@ __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, 
             else { rt[r].x += v[r].x; rt[r].y += v[r].y; }
         }
     }
-    const bool need_div = MODE == 1 || a.stokes;
     cx<T> v[R];
-    if (need_div) {
-        S::load_spectrum_sum(a.D, g.dim, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-        S::template transform<true>(v, lds, ps.j, lt);
-    }
-    if constexpr (MODE == 0) {
+    if constexpr (MODE != 1) {
+        const T* tpa = a.tau.lane(oa, ps.j);
+        const T* tpb = a.tau.lane(ob, ps.j);
+        const T* bpa = a.bona.lane(oa, ps.j);
+        const T* bpb = a.bona.lane(ob, ps.j);
+        const T* cpa = a.c0.lane(oa, ps.j);
+        const T* cpb = a.c0.lane(ob, ps.j);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             T ea = rt[r].x, eb = rt[r].y;
-            const T r0a = a.rho0.at(oa + R * r);
-            const T r0b = a.rho0.at(ob + R * r);
-            if (a.stokes) {
-                ea += a.tau.at(oa + R * r) * r0a * v[r].x;
-                eb += a.tau.at(ob + R * r) * r0b * v[r].y;
+            const T r0a = r0pa[R * r];
+            const T r0b = r0pb[R * r];
+            if constexpr (MODE == 2) {
+                ea += tpa[R * r] * sd[r].x;
+                eb += tpb[R * r] * sd[r].y;
             }
             if constexpr (NL) {
-                ea += a.bona.at(oa + R * r) * (rt[r].x * rt[r].x) / (T(2) * r0a);
-                eb += a.bona.at(ob + R * r) * (rt[r].y * rt[r].y) / (T(2) * r0b);
+                ea += bpa[R * r] * (rt[r].x * rt[r].x) / (T(2) * r0a);
+                eb += bpb[R * r] * (rt[r].y * rt[r].y) / (T(2) * r0b);
             }
-            const T cA = a.c0.at(oa + R * r);
-            const T cB = a.c0.at(ob + R * r);
+            const T cA = cpa[R * r];
+            const T cB = cpb[R * r];
             const T pA = cA * cA * ea, pB = cB * cB * eb;
             if (va) a.p[oa + R * r] = pA;
             if (vb) a.p[ob + R * r] = pB;
@@ -414,11 +492,7 @@ __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, 
         S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
     } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const T r0a = a.rho0.at(oa + R * r);
-            const T r0b = a.rho0.at(ob + R * r);
-            v[r] = cx<T>{va ? r0a * v[r].x : T(0), vb ? r0b * v[r].y : T(0)};
-        }
+        for (int r = 0; r < R; ++r) v[r] = cx<T>{va ? sd[r].x : T(0), vb ? sd[r].y : T(0)};
         S::forward_store(a.F1, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -430,44 +504,72 @@ __global__ void __launch_bounds__(256) sq_z_density(RhoArgs<T> a, Geo g, int P, 
     }
 }
 
+// EOS with FFT absorption: p = c^2 (rhoT + tau C2R_z(A1) - eta C2R_z(A2) [+ B/A rhoT^2 / (2 rho0)]),
+// P = R2C_z(p). Both spectra are loaded before the first transform (one memory phase).
 template <typename T, int R>
-__global__ void __launch_bounds__(256) sq_z_eos(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_EOS_WAVES : 1) sq_z_eos(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
     SqZPos<R> ps(P);
     LaneTw<T, R> lt;
-    lt.load(tab.tw[2], ps.j);
-    lds += ps.w * S::LP;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    cx<T>* lds = ldsb + R * R + ps.w * S::LP;
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
     cx<T> e[R], v[R];
     S::load_spectrum(a.A1, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, e);
+    if (a.A2) S::load_spectrum(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
     S::template transform<true>(e, lds, ps.j, lt);
-    if (a.A2) {
-        S::load_spectrum(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-        S::template transform<true>(v, lds, ps.j, lt);
-    }
+    if (a.A2) S::template transform<true>(v, lds, ps.j, lt);
+    KWH_SCHED_FENCE();
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+    const T* cpa = a.c0.lane(oa, ps.j);
+    const T* cpb = a.c0.lane(ob, ps.j);
+    const T* tpa = a.tau.lane(oa, ps.j);
+    const T* tpb = a.tau.lane(ob, ps.j);
+    const T* epa = a.eta.lane(oa, ps.j);
+    const T* epb = a.eta.lane(ob, ps.j);
+    const T* bpa = a.bona.lane(oa, ps.j);
+    const T* bpb = a.bona.lane(ob, ps.j);
+    const T* rpa = a.rho0.lane(oa, ps.j);
+    const T* rpb = a.rho0.lane(ob, ps.j);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int64_t ca = oa + R * r, cb = ob + R * r;
-        const T rta = a.rhoT[ca], rtb = a.rhoT[cb];
-        T ea = rta + a.tau.at(ca) * e[r].x;
-        T eb = rtb + a.tau.at(cb) * e[r].y;
-        if (a.A2) {
-            ea -= a.eta.at(ca) * v[r].x;
-            eb -= a.eta.at(cb) * v[r].y;
+    for (int g0 = 0; g0 < R; g0 += kGroup) {
+#pragma unroll
+        for (int r = g0; r < g0 + kGroup && r < R; ++r) {
+            const int64_t ca = oa + R * r, cb = ob + R * r;
+            const T rta = a.rhoT[ca], rtb = a.rhoT[cb];
+            const T cA = cpa[R * r], cB = cpb[R * r];
+            T tauA, etaA, tauB, etaB;
+            if (a.tau_eta_from_c) {
+                const T la = log2(cA), lb = log2(cB);
+                tauA = a.tk * exp2(a.te * la);
+                tauB = a.tk * exp2(a.te * lb);
+                etaA = a.ek * exp2(a.ee * la);
+                etaB = a.ek * exp2(a.ee * lb);
+            } else {
+                tauA = tpa[R * r];
+                tauB = tpb[R * r];
+                etaA = epa[R * r];
+                etaB = epb[R * r];
+            }
+            T ea = rta + tauA * e[r].x;
+            T eb = rtb + tauB * e[r].y;
+            if (a.A2) {
+                ea -= etaA * v[r].x;
+                eb -= etaB * v[r].y;
+            }
+            if (a.nonlinear) {
+                ea += bpa[R * r] * (rta * rta) / (T(2) * rpa[R * r]);
+                eb += bpb[R * r] * (rtb * rtb) / (T(2) * rpb[R * r]);
+            }
+            const T pA = cA * cA * ea, pB = cB * cB * eb;
+            if (va) a.p[ca] = pA;
+            if (vb) a.p[cb] = pB;
+            e[r] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
         }
-        if (a.nonlinear) {
-            ea += a.bona.at(ca) * (rta * rta) / (T(2) * a.rho0.at(ca));
-            eb += a.bona.at(cb) * (rtb * rtb) / (T(2) * a.rho0.at(cb));
-        }
-        const T cA = a.c0.at(ca), cB = a.c0.at(cb);
-        const T pA = cA * cA * ea, pB = cB * cB * eb;
-        if (va) a.p[ca] = pA;
-        if (vb) a.p[cb] = pB;
-        e[r] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
+        KWH_SCHED_FENCE();
     }
     S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, e);
 }
@@ -493,7 +595,7 @@ struct SqCol {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             cx<T> x = rd[r * (R * NTR + 1)];
-            if (r > 0) x = cmul(x, INV ? conjv(tw.w[r]) : tw.w[r]);
+            if (r > 0) x = cmul(x, INV ? conjv(tw.w(r)) : tw.w(r));
             v[r] = x;
         }
         dftR<T, R, INV>(v);
@@ -505,10 +607,11 @@ template <typename T, int R, int NTR, bool INV>
 __global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __restrict__ tw) {
     using S = SqCol<T, R, NTR>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds = ldsb + R * R;
     const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
     LaneTw<T, R> lt;
-    lt.load(tw, j);
+    lt.load(ldsb, tw, j);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t c = (int64_t)tile * NTR + w;
     const bool valid = c < a.ncols;
@@ -539,13 +642,14 @@ __global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __
 // Middle pass: forward along the middle axis, operators, inverse, store. Every table-using
 // operator of one input reads the same real k-space table (checked on the host), loaded once.
 template <typename T, int R, int NTR>
-__global__ void __launch_bounds__(256) sq_middle(MidArgs a, AxisTabs<T> tab, Geo g) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_MIDDLE_WAVES : 1) sq_middle(MidArgs a, AxisTabs<T> tab, Geo g) {
     using S = SqCol<T, R, NTR>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds = ldsb + R * R;
     const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
     LaneTw<T, R> lt;
-    lt.load(tab.tw[a.axis], j);
+    lt.load(ldsb, tab.tw[a.axis], j);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t c = (int64_t)tile * NTR + w;
     const bool valid = c < a.ncols;
@@ -559,6 +663,8 @@ __global__ void __launch_bounds__(256) sq_middle(MidArgs a, AxisTabs<T> tab, Geo
     const int64_t hstride = a.axis == 0 ? (int64_t)g.m1 * g.mzh : (int64_t)g.mzh;
     const int64_t hbase = (a.axis == 0 ? cc : (int64_t)i0c * g.m1 * g.mzh + i2c) + (int64_t)j * hstride;
     const int64_t hstep = (int64_t)R * hstride;
+    T kap[R];
+    const T* kap_src = nullptr;   // table currently held in kap (inputs sharing one table load it once)
     for (int f = 0; f < a.nin; ++f) {
         cx<T> X[R];
         {
@@ -573,14 +679,14 @@ __global__ void __launch_bounds__(256) sq_middle(MidArgs a, AxisTabs<T> tab, Geo
         const T* ktab = nullptr;
         for (int o = 0; o < a.nop; ++o)
             if (a.op[o].in == f && a.op[o].table) { ktab = reinterpret_cast<const T*>(a.op[o].table); break; }
-        T kap[R];
-        if (ktab) {
+        if (ktab && ktab != kap_src) {
             const T* q = ktab + hbase;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 kap[r] = *q * scale;
                 q += hstep;
             }
+            kap_src = ktab;
         }
         for (int o = 0; o < a.nop; ++o) {
             const MidOp op = a.op[o];

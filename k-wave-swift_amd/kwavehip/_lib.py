@@ -23,7 +23,7 @@ EXPORTED = [
     "kwh_abi_version", "kwh_last_error", "kwh_sim_validate", "kwh_ctx_create", "kwh_ctx_destroy", "kwh_sim_create",
     "kwh_sim_run", "kwh_sim_launch", "kwh_sim_sync", "kwh_sim_output_shape", "kwh_sim_read_output",
     "kwh_sim_steps_done", "kwh_sim_state_bytes", "kwh_sim_save_state", "kwh_sim_load_state",
-    "kwh_sim_set_kernel_timing", "kwh_sim_kernel_stats", "kwh_sim_set_graph", "kwh_sim_destroy",
+    "kwh_sim_set_kernel_timing", "kwh_sim_set_kernel_timing_only", "kwh_sim_kernel_stats", "kwh_sim_set_graph", "kwh_sim_destroy",
 ]
 
 
@@ -89,6 +89,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.kwh_sim_save_state.argtypes = [vp, vp, C.c_size_t]
     lib.kwh_sim_load_state.argtypes = [vp, vp, C.c_size_t]
     lib.kwh_sim_set_kernel_timing.argtypes = [vp, C.c_int]
+    lib.kwh_sim_set_kernel_timing_only.argtypes = [vp, C.c_char_p]
     lib.kwh_sim_kernel_stats.argtypes = [vp, C.POINTER(KernelStat), C.c_int32, C.POINTER(C.c_int32)]
     lib.kwh_sim_set_graph.argtypes = [vp, C.c_int]
     lib.kwh_sim_destroy.argtypes = [vp]

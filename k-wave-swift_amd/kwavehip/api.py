@@ -415,8 +415,12 @@ class Simulation:
         buf = C.create_string_buffer(blob, len(blob))
         L.check(self.lib.kwh_sim_load_state(self.handle, buf, len(blob)))
 
-    def set_kernel_timing(self, on: bool):
-        L.check(self.lib.kwh_sim_set_kernel_timing(self.handle, 1 if on else 0))
+    def set_kernel_timing(self, on: bool, only: Optional[str] = None):
+        """HIP-event timing of every kernel launch, or of the launches of kernel `only`."""
+        if on and only:
+            L.check(self.lib.kwh_sim_set_kernel_timing_only(self.handle, only.encode()))
+        else:
+            L.check(self.lib.kwh_sim_set_kernel_timing(self.handle, 1 if on else 0))
 
     def kernel_stats(self):
         cnt = C.c_int32()

@@ -418,6 +418,32 @@ def test_deterministic_replay():
 
 
 # ------------------------------------------------------------------------------------------ square-length fast path
+@pytest.mark.parametrize("absorbing,mode,nonlinear", [
+    (False, AbsorptionMode.powerLaw, False), (False, AbsorptionMode.powerLaw, True),
+    (True, AbsorptionMode.stokes, False), (True, AbsorptionMode.stokes, True),
+    (True, AbsorptionMode.noDispersion, False), (True, AbsorptionMode.powerLaw, False)])
+def test_2d_64_register_resident_density_variants(absorbing, mode, nonlinear):
+    """Every density-kernel variant on the register-resident 64^2 path: EOS, EOS with Stokes
+    absorption, FFT absorption with and without dispersion; linear and B/A."""
+    n = 64
+    med = _hetero_2d(n, seed=11, absorbing=absorbing, nonlinear=nonlinear, mode=mode, y=1.4)
+    grid = KWaveGrid(n, 1e-4, n, 1e-4)
+    grid.setTime(160, 0.3 * 1e-4 / float(np.max(med.soundSpeed)))
+    check(f"2d_64_density_{absorbing}_{mode.value}_{nonlinear}", grid, med,
+          KWaveSource(p0=makeDisc(n, n, radius=6)), KWaveSensor(), ["pFinal"], pmlSize=10)
+
+
+def test_3d_64_register_resident_stokes():
+    """3D Stokes absorption (y = 2) with B/A on the radix-8 x 8 path."""
+    n = 64
+    c = random_field((n, n, n), 51, 6.0, 1400.0, 1600.0)
+    med = KWaveMedium(c, 1000.0, alphaCoeff=0.5, alphaPower=2.0, alphaMode=AbsorptionMode.stokes, bOnA=4.0)
+    grid = KWaveGrid(n, 1e-4, n, 1e-4, n, 1e-4)
+    grid.setTime(50, 0.3 * 1e-4 / float(c.max()))
+    check("3d_64_stokes", grid, med, KWaveSource(p0=makeBall(n, n, n, radius=6)), KWaveSensor(), ["pFinal"],
+          pmlSize=8, workers=8)
+
+
 def test_2d_256_register_resident_path():
     """256^2 runs the register-resident radix-16 x 16 kernels on both axes."""
     n = 256
