@@ -75,6 +75,40 @@ def measured_traffic(workload, dtype, kernel):
     return {"bytes": rec["fetch_bytes"] + rec["write_bytes"], "source": rec["summary"]}
 
 
+def precision_check(grid, med, src, sens, opts):
+    """The whole workload (all nt steps) in FP32 and in FP64 on the GPU: the FP64 throughput and
+    the FP32 error against FP64 (relL2 of the sensor series; pFinal residual normalised by
+    max_t |p| at the sensor, as the FP32 reference tests do)."""
+    import dataclasses
+    from kwavehip import DTypePrecision, Simulation
+    outs, rate = {}, {}
+    for name, dt in (("f32", DTypePrecision.float32), ("f64", DTypePrecision.float64)):
+        sim = Simulation(grid, med, src, sens, dataclasses.replace(opts, dtype=dt))
+        sim.launch(0, 1)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.launch(1, grid.nt)
+        sim.sync()
+        rate[name] = grid.totalGridPoints * (grid.nt - 1) / (time.perf_counter() - t0) / 1e6
+        outs[name] = sim.outputs()
+        sim.close()
+    o32, o64 = outs["f32"], outs["f64"]
+    def rel_l2(a, b):
+        a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+        return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b * b)), 1e-300))
+    err = {}
+    for f in ("p", "ux", "uy", "uz"):
+        a, b = getattr(o32, f), getattr(o64, f)
+        if a is not None and b is not None:
+            err[f"{f}_relL2"] = rel_l2(a, b)
+    if o32.pFinal is not None and o64.p is not None:
+        d = np.asarray(o32.pFinal, np.float64) - o64.pFinal
+        err["pFinal_rms_over_max_p"] = float(np.sqrt(np.mean(d * d)) / max(np.abs(o64.p).max(), 1e-300))
+    return {"steps": grid.nt, "f64_Mcell_steps_per_s": round(rate["f64"], 2),
+            "f32_Mcell_steps_per_s_full_run": round(rate["f32"], 2),
+            "error_fp32_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()}}
+
+
 def cpu_baseline(grid, med, src, sens, opts, steps, workers):
     """Time the float64 oracle (numpy/scipy pocketfft, `workers` threads) on `steps` steps."""
     from oracle import kwave_oracle as O
@@ -93,6 +127,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -135,7 +170,6 @@ def main():
     t_read = time.perf_counter()
     outputs = sim.outputs()                         # device -> host copy of the recorded series
     read_s = time.perf_counter() - t_read
-    del outputs
     N = grid.totalGridPoints
     value = N * args.steps / elapsed / 1e6
     ms_step = elapsed / args.steps * 1e3
@@ -164,6 +198,13 @@ def main():
                   "frac_of_8TBps": round(model_bytes / (ms_step * 1e-3) / HBM_PEAK, 4)}
     sim.close()
 
+    del outputs
+    fp64 = None
+    if args.dtype == "f32" and not args.no_fp64_check:
+        print(f"[bench] full {grid.nt}-step runs in FP32 and FP64 for the precision comparison", file=sys.stderr,
+              flush=True)
+        fp64 = precision_check(grid, med, src, sens, opts)
+
     cpu = None
     if not args.no_cpu_baseline:
         workers = min(os.cpu_count() or 1, 16)
@@ -180,7 +221,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": args.dtype, "data": "synthetic (seeded random media)",
         "config": dict(info, dtype=args.dtype),
-        "roofline": roof, "step_model": step_model, "kernels": kernels, "cpu_baseline": cpu,
+        "roofline": roof, "step_model": step_model, "kernels": kernels, "cpu_baseline": cpu, "fp64_gpu": fp64,
         "setup_s": round(setup_s, 2),
         # whole job seen from the host boundary: upload + setup, the timed steps, read-back
         "host_boundary": {"create_s": round(create_s, 3), "read_s": round(read_s, 3),

@@ -424,7 +424,7 @@ template <typename T> struct VelArgs {
     SMap<T> dtr[3];        // dt / rho0 staggered along a
     SMap<T> corr[3];       // additive k-space-corrected source contribution (zero scalar when absent)
     SparseSrc<T> sp[3];
-    int t;
+    const int* ctr;        // device step counters {t, record slot}
 };
 
 template <typename T>
@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(256) z_velocity(VelArgs<T> a, Geo g, ZTile zt0
         })
         if (sparse) {
             __syncthreads();
-            apply_sparse<T>(a.sp[ax], g, zt, R, a.t, tid, nthr);
+            apply_sparse<T>(a.sp[ax], g, zt, R, a.ctr[0], tid, nthr);
             __syncthreads();
             KWH_CELL_LOOP(zt, g, tid, nthr, { u[cell] = comp(R[n * zt.pitch + w], part); })
         }
@@ -484,7 +484,7 @@ template <typename T> struct RhoArgs {
     cx<T>* F2;
     T dt;
     int mode, nonlinear, stokes, dispersion;
-    int t;
+    const int* ctr;        // device step counters {t, record slot}
 };
 
 template <typename T>
@@ -532,7 +532,7 @@ __global__ void __launch_bounds__(256) z_density(RhoArgs<T> a, Geo g, ZTile zt0,
         })
         if (sparse) {
             __syncthreads();
-            apply_sparse<T>(a.sp[ax], g, zt, R, a.t, tid, nthr);
+            apply_sparse<T>(a.sp[ax], g, zt, R, a.ctr[0], tid, nthr);
             __syncthreads();
             KWH_CELL_LOOP(zt, g, tid, nthr, {
                 const T v = comp(R[n * zt.pitch + w], part);
@@ -726,10 +726,27 @@ __global__ void maxabs_kernel(const T* x, int64_t n, unsigned long long* out) {
 // Scatter scaled source values into a zeroed real field (k-space-corrected additive sources).
 template <typename T>
 __global__ void scatter_kernel(T* dst, const int64_t* index, const T* signal, const T* base, int rows, int len, int n,
-                               int t, T factor) {
+                               const int* ctr, T factor) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = ctr[0];
     if (e >= n || t >= len) return;
     dst[index[e]] = signal[(int64_t)(rows == 1 ? 0 : e) * len + t] * base[e] * factor;
+}
+
+// End of a time step: t += 1 and, when the step recorded, slot += 1. Kernels read both from
+// here, so a captured step graph replays unchanged.
+__global__ void set_counters_kernel(int* ctr, int t, int slot) {
+    if (threadIdx.x == 0) {
+        ctr[0] = t;
+        ctr[1] = slot;
+    }
+}
+
+__global__ void advance_kernel(int* ctr, int recorded) {
+    if (threadIdx.x == 0) {
+        ctr[0] += 1;
+        ctr[1] += recorded;
+    }
 }
 
 // Sensor gather + streaming aggregates (Recording.swift:37-45, :139-156).
@@ -743,8 +760,7 @@ template <typename T> struct GatherArgs {
     T* vmax;
     T* vmin;
     double* sum2;
-    int t;
-    int first;             // first recorded step: initialise max/min
+    const int* ctr;        // device step counters {t, record slot}; slot 0 initialises max/min
 };
 
 // Up to four fields (p, ux, uy, uz) sampled by one launch: blockIdx.y selects the field.
@@ -755,6 +771,8 @@ template <typename T> struct GatherSet {
 template <typename T>
 __global__ void gather_kernel(GatherSet<T> set) {
     const GatherArgs<T>& a = set.f[blockIdx.y];
+    const int slot = a.ctr[1];
+    const bool first = slot == 0;
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)gridDim.x * blockDim.x) {
         T v;
         if (a.weight) {
@@ -763,17 +781,19 @@ __global__ void gather_kernel(GatherSet<T> set) {
         } else {
             v = a.field[a.idx[s]];
         }
-        if (a.series) a.series[(int64_t)a.t * a.n + s] = v;
-        if (a.vmax) a.vmax[s] = a.first ? v : fmax(a.vmax[s], v);
-        if (a.vmin) a.vmin[s] = a.first ? v : fmin(a.vmin[s], v);
-        if (a.sum2) a.sum2[s] = (a.first ? 0.0 : a.sum2[s]) + (double)v * (double)v;
+        if (a.series) a.series[(int64_t)slot * a.n + s] = v;
+        if (a.vmax) a.vmax[s] = first ? v : fmax(a.vmax[s], v);
+        if (a.vmin) a.vmin[s] = first ? v : fmin(a.vmin[s], v);
+        if (a.sum2) a.sum2[s] = (first ? 0.0 : a.sum2[s]) + (double)v * (double)v;
     }
 }
 
 // Gather through output positions (a subset of sensor rows, e.g. one directivity angle).
 template <typename T>
 __global__ void gather_pos_kernel(const T* field, const int64_t* idx, const int64_t* pos, int64_t n, int64_t nsens,
-                                  T* series, T* vmax, T* vmin, double* sum2, int t, int first) {
+                                  T* series, T* vmax, T* vmin, double* sum2, const int* ctr) {
+    const int t = ctr[1];
+    const bool first = t == 0;
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
         const T v = field[idx[s]];
         const int64_t q = pos[s];

@@ -229,6 +229,11 @@ struct SimT final : kwh_sim {
     int nzrows = 0;
     DBuf ser_p, ser_u[3], pmax, pmin, psum2, umax[3], usum2[3];
     int rec_steps = 0;
+    // device step counters {t, record slot}, advanced by a kernel at the end of every step
+    DBuf ctr;
+    // captured step graphs, keyed by which k-space-corrected sources the step launches
+    struct StepGraph { hipGraph_t graph; hipGraphExec_t exec; };
+    std::vector<std::pair<unsigned, StepGraph>> step_graphs;
     // directivity
     int ndir = 0;
     std::vector<std::unique_ptr<DBuf>> dir_w, dir_idx, dir_pos;
@@ -244,10 +249,8 @@ struct SimT final : kwh_sim {
     int64_t stat_launch[kNames] = {};
     double stat_ms[kNames] = {};
     double stat_bytes[kNames] = {};
-    // graph
+    // graph replay of the time step
     bool use_graph = false;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
 
     size_t lds_budget = 64 * 1024;
     // one-element device buffers behind scalar SMaps (branch-free at() in the kernels)
@@ -266,8 +269,10 @@ struct SimT final : kwh_sim {
     bool use_sq = true;
 
     ~SimT() override {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& sg : step_graphs) {
+            (void)hipGraphExecDestroy(sg.second.exec);
+            (void)hipGraphDestroy(sg.second.graph);
+        }
         for (auto& e : ev_pool) (void)hipEventDestroy(e);
         for (auto& pd : pending) {
             (void)hipEventDestroy(pd.a);
@@ -1154,6 +1159,7 @@ struct SimT final : kwh_sim {
         int nspec = dim;
         for (int a = 0; a < dim; ++a) if (need_u[a]) ++nspec;
         for (int k = 0; k < std::max(nspec, 2); ++k) spec[k].alloc(g.NH * sizeof(C));
+        ctr.alloc(2 * sizeof(int));
         reset_state();
     }
 
@@ -1176,8 +1182,8 @@ struct SimT final : kwh_sim {
         fill<T>(scr_real.as<T>(), T(0), g.N);
         timed(K_SC, 0, [&] {
             hipLaunchKernelGGL((scatter_kernel<T>), dim3(grid_blocks(s.n)), dim3(256), 0, stream, scr_real.as<T>(),
-                               s.index.template as<int64_t>(), s.signal.template as<T>(), s.base.template as<T>(), s.rows, s.len, (int)s.n, t,
-                               T(factor));
+                               s.index.template as<int64_t>(), s.signal.template as<T>(), s.base.template as<T>(), s.rows, s.len, (int)s.n,
+                               (const int*)ctr.p, T(factor));
         });
         forward_to_middle(scr_real.as<T>(), scr_spec.as<C>(), T(1));
         middle(K_MT, {scr_spec.p}, {MidOp{OP_TAB, 0, 0, scr_spec.p, skappa.p}});
@@ -1253,7 +1259,7 @@ struct SimT final : kwh_sim {
                 va.corr[a] = cu[a] ? mk_map(cu[a]) : zero_map;
                 va.sp[a] = sparse_of(usrc[a], a);
             }
-            va.t = t;
+            va.ctr = (const int*)ctr.p;
             double bytes = 0;
             for (int a = 0; a < dim; ++a)
                 bytes += 2 * spec_bytes() + 2 * real_bytes() + mapb(dtrs[a].p) + (cu[a] ? real_bytes() : 0);
@@ -1343,7 +1349,7 @@ struct SimT final : kwh_sim {
             ra.nonlinear = nonlinear;
             ra.stokes = stokes;
             ra.dispersion = dispersion;
-            ra.t = t;
+            ra.ctr = (const int*)ctr.p;
             const int nbuf = 4 + (nonlinear ? 1 : 0);
             double bytes = dim * (spec_bytes() + 2 * real_bytes()) + mapb(rho0s.p) + (cp ? real_bytes() : 0);
             if (nonlinear) bytes += mapb(bonas.p);
@@ -1419,10 +1425,8 @@ struct SimT final : kwh_sim {
         P_valid = true;
     }
 
-    void record_step(int t) {
+    void record_step(int) {
         if (!(rec_p || rec_u)) return;
-        const int first = rec_steps == 0;
-        const int slot = rec_steps;
         if (rec_p) {
             if (ndir > 0) {
                 // directivity: per unique angle Re ifft2(fft2(p) W) sampled at that angle's rows
@@ -1432,7 +1436,7 @@ struct SimT final : kwh_sim {
                     middle(K_MT, {scr_spec.p}, {MidOp{OP_TAB, 0, 0, tmp, dir_w[k]->p}});
                     inverse_from_middle(tmp, dir_field.as<T>(), T(1), false);
                     gather(dir_field.as<T>(), dir_idx[k]->as<int64_t>(), nullptr, 1, dir_count[k], ser_p.as<T>(),
-                           pmax.as<T>(), pmin.as<T>(), psum2.as<double>(), slot, first, dir_pos[k]->as<int64_t>());
+                           pmax.as<T>(), pmin.as<T>(), psum2.as<double>(), dir_pos[k]->as<int64_t>());
                 }
             }
         }
@@ -1441,41 +1445,76 @@ struct SimT final : kwh_sim {
         int nf = 0;
         if (rec_p && ndir == 0)
             set.f[nf++] = GatherArgs<T>{p.as<T>(), sidx.as<int64_t>(), sw.as<T>(), ncorner, nsens, ser_p.as<T>(),
-                                        pmax.as<T>(), pmin.as<T>(), psum2.as<double>(), slot, first};
+                                        pmax.as<T>(), pmin.as<T>(), psum2.as<double>(), (const int*)ctr.p};
         for (int a = 0; a < dim; ++a)
             if (need_u[a])
                 set.f[nf++] = GatherArgs<T>{ucol[a].as<T>(), sidx.as<int64_t>(), sw.as<T>(), ncorner, nsens,
-                                            ser_u[a].as<T>(), umax[a].as<T>(), nullptr, usum2[a].as<double>(), slot,
-                                            first};
+                                            ser_u[a].as<T>(), umax[a].as<T>(), nullptr, usum2[a].as<double>(),
+                                            (const int*)ctr.p};
         if (nf > 0)
             timed(K_GA, (double)nf * nsens * sizeof(T) * 2, [&] {
                 hipLaunchKernelGGL((gather_kernel<T>), dim3(grid_blocks(nsens), nf), dim3(256), 0, stream, set);
             });
-        rec_steps++;
     }
 
+    // Subset gather through output positions (directivity rows).
     void gather(const T* field, const int64_t* idx, const T* w, int nc, int64_t n, T* series, T* vmax, T* vmin,
-                double* sum2, int slot, int first, const int64_t* outpos) {
-        if (outpos) {
-            // subset gather: write through positions (directivity rows)
-            timed(K_GA, 0, [&] {
-                hipLaunchKernelGGL((gather_pos_kernel<T>), dim3(grid_blocks(n)), dim3(256), 0, stream, field, idx, outpos, n,
-                                   nsens, series, vmax, vmin, sum2, slot, first);
-            });
-            return;
-        }
-        GatherSet<T> set{};
-        set.f[0] = GatherArgs<T>{field, idx, w, nc, n, series, vmax, vmin, sum2, slot, first};
-        timed(K_GA, (double)n * sizeof(T) * 2, [&] {
-            hipLaunchKernelGGL((gather_kernel<T>), dim3(grid_blocks(n)), dim3(256), 0, stream, set);
+                double* sum2, const int64_t* outpos) {
+        (void)w;
+        (void)nc;
+        timed(K_GA, 0, [&] {
+            hipLaunchKernelGGL((gather_pos_kernel<T>), dim3(grid_blocks(n)), dim3(256), 0, stream, field, idx, outpos, n,
+                               nsens, series, vmax, vmin, sum2, (const int*)ctr.p);
         });
     }
 
-    void step(int t) {
+    // The launches of step t, ending with the device counter advance. Host-side mirrors
+    // (steps, rec_steps, P_valid) are updated by the caller.
+    void step_launches(int t) {
         if (t == 0 && has_p0) init_step();
         else main_step(t);
         record_step(t);
+        const int recorded = (rec_p || rec_u) ? 1 : 0;
+        hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, stream, (int*)ctr.p, recorded);
+        HIPCHK(hipGetLastError());
+    }
+    void step_done(int t) {
+        if (rec_p || rec_u) ++rec_steps;
+        P_valid = true;
         steps = t + 1;
+    }
+
+    // Which k-space-corrected additive sources step t launches (the only host-side decision in
+    // a step: everything else reads t from the device counters).
+    unsigned step_signature(int t) const {
+        unsigned key = 0;
+        for (int a = 0; a < dim; ++a)
+            if (usrc[a].present && usrc[a].mode == KWH_SRC_ADDITIVE && t < usrc[a].len) key |= 1u << a;
+        if (psrc.present && psrc.mode == KWH_SRC_ADDITIVE && t < psrc.len) key |= 8u;
+        return key;
+    }
+
+    void step(int t) {
+        if (use_graph && !timing && !(t == 0 && has_p0)) {
+            const unsigned key = step_signature(t);
+            const StepGraph* sg = nullptr;
+            for (auto& e : step_graphs)
+                if (e.first == key) sg = &e.second;
+            if (!sg) {
+                // captured once per signature, replayed for every later step with it
+                StepGraph ng{};
+                HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                step_launches(t);
+                HIPCHK(hipStreamEndCapture(stream, &ng.graph));
+                HIPCHK(hipGraphInstantiate(&ng.exec, ng.graph, nullptr, nullptr, 0));
+                step_graphs.emplace_back(key, ng);
+                sg = &step_graphs.back().second;
+            }
+            HIPCHK(hipGraphLaunch(sg->exec, stream));
+        } else {
+            step_launches(t);
+        }
+        step_done(t);
     }
 
     // ------------------------------------------------------------------ API
@@ -1487,6 +1526,8 @@ struct SimT final : kwh_sim {
     void launch(int t0, int t1) override {
         check_range(t0, t1);
         HIPCHK(hipSetDevice(dev));
+        hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, stream, (int*)ctr.p, t0, rec_steps);
+        HIPCHK(hipGetLastError());
         for (int t = t0; t < t1; ++t) step(t);
     }
     void sync() override {
@@ -1680,7 +1721,10 @@ struct SimT final : kwh_sim {
         }
         return n;
     }
-    void set_graph(bool on) override { use_graph = on; }
+    void set_graph(bool on) override {
+        sync();
+        use_graph = on;
+    }
 };
 
 }  // namespace

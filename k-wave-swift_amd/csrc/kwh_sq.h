@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_VELOCITY_WAVES : 1) 
         }
         if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{(int)(xcd_tile(blockIdx.x, gridDim.x) * P), P, 0, 0}))
             sq_apply_sparse<T, R>(a.sp[ax], g, 2 * xcd_tile(blockIdx.x, gridDim.x) * P, 2 * P, lds_all, ps.w, ps.j,
-                                  a.t, v);
+                                  a.ctr[0], v);
         {
             T* wa = u + oa;
             T* wb = u + ob;
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(256, (NL || CORR || sizeof(T) != 4) ? 1 : (MOD
             }
         }
         if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{row0 / 2, P, 0, 0}))
-            sq_apply_sparse<T, R>(a.sp[ax], g, row0, 2 * P, lds_all, ps.w, ps.j, a.t, v);
+            sq_apply_sparse<T, R>(a.sp[ax], g, row0, 2 * P, lds_all, ps.w, ps.j, a.ctr[0], v);
         T* wa = rho + oa;
         T* wb = rho + ob;
 #pragma unroll

@@ -417,6 +417,69 @@ def test_deterministic_replay():
     assert np.array_equal(a.pFinal, b.pFinal)
 
 
+def _outputs_equal(a, b, fields):
+    for f in fields:
+        x, y = getattr(a, f), getattr(b, f)
+        assert x is not None and y is not None, f
+        assert np.array_equal(x, y), (f, float(np.abs(np.asarray(x, np.float64) - y).max()))
+
+
+def _run_chunks(make, chunks):
+    """Run a fresh Simulation in [t0, t1) chunks, graph replay on or off per chunk."""
+    sim = make()
+    try:
+        for t0, t1, graph in chunks:
+            sim.set_graph(graph)
+            sim.run(t0, t1)
+        return sim.outputs()
+    finally:
+        sim.close()
+
+
+def test_graph_replay_matches_eager_2d_sources():
+    """Captured step graphs (one per set of active k-space-corrected sources) replayed for every
+    step, alternating with eager chunks: bitwise equal to an all-eager run. The additive source
+    ends at step 40, so the replay switches graphs mid-run; the 64^2 grid takes the
+    register-resident kernels."""
+    n = 64
+    grid = KWaveGrid(n, 1e-4, n, 1e-4)
+    grid.setTime(120, 0.3 * 1e-4 / 1600.0)
+    med = _hetero_2d(n, seed=21)
+    mask = np.zeros((n, n), np.float32)
+    mask[10, 20:44] = 1
+    sig = (1e5 * toneBurst(1 / grid.dt, 2e6, 2)[:40]).astype(np.float32)
+    src = KWaveSource(pMask=mask, p=sig, pMode=SourceMode.additive, p0=makeDisc(n, n, radius=5))
+    line = np.zeros((n, n), np.float32)
+    line[44, 8:56] = 1
+    sens = KWaveSensor(mask=line, record=RecordField.p | RecordField.pMax | RecordField.pRms | RecordField.ux |
+                       RecordField.uy | RecordField.pFinal)
+    make = lambda: Simulation(grid, med, src, sens, SimulationOptions(pmlSize=10))
+    print("\n[case] graph replay vs eager, 2D 64^2 with an additive source", flush=True)
+    eager = _run_chunks(make, [(0, 120, False)])
+    mixed = _run_chunks(make, [(0, 30, True), (30, 55, False), (55, 90, True), (90, 120, True)])
+    _outputs_equal(eager, mixed, ["p", "pMax", "pRms", "ux", "uy", "pFinal"])
+
+
+def test_graph_replay_matches_eager_3d_velocity_source():
+    """3D 32^3 (generic kernels): additive and dirichlet velocity sources, graph replay from step 0
+    (no p0, so step 0 is a captured step too) against eager."""
+    n = 32
+    grid = KWaveGrid(n, 1e-4, n, 1e-4, n, 1e-4)
+    grid.setTime(60, 2e-8)
+    mask = np.zeros((n, n, n), np.float32)
+    mask[8, 12:20, 12:20] = 1
+    sig = (0.3 * toneBurst(1 / grid.dt, 3e6, 2)).astype(np.float32)
+    src = KWaveSource(uMask=mask, ux=sig, uz=sig, uMode=SourceMode.additive)
+    sens_mask = np.zeros((n, n, n), np.float32)
+    sens_mask[20, :, 16] = 1
+    sens = KWaveSensor(mask=sens_mask, record=RecordField.p | RecordField.ux | RecordField.uz | RecordField.uMax)
+    make = lambda: Simulation(grid, KWaveMedium(1500.0, 1000.0), src, sens, SimulationOptions(pmlSize=6))
+    print("\n[case] graph replay vs eager, 3D 32^3 velocity source", flush=True)
+    eager = _run_chunks(make, [(0, 60, False)])
+    graph = _run_chunks(make, [(0, 60, True)])
+    _outputs_equal(eager, graph, ["p", "ux", "uz", "uxMax", "uzMax"])
+
+
 # ------------------------------------------------------------------------------------------ square-length fast path
 @pytest.mark.parametrize("absorbing,mode,nonlinear", [
     (False, AbsorptionMode.powerLaw, False), (False, AbsorptionMode.powerLaw, True),
