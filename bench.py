@@ -109,6 +109,145 @@ def precision_check(grid, med, src, sens, opts):
             "error_fp32_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()}}
 
 
+def scan_line_cases(n_runs=48, nt=800, seed=7):
+    """Per-scan-line transmits: 48 independent 3D FP32 runs, sizes drawn from {96^3, 128x96x96,
+    160x128x128}, speckle media c = 1540 (1 + U[-0.02, 0.02]) m/s, rho = 1000 (1 + U[-0.02, 0.02]),
+    a 32-element linear array (width 2, length 24, kerf 0) at x = 10 driven with a 2 MHz 2-cycle
+    burst, steering U[-20, 20] deg, focus U[15, 40] mm, elevation focus 20 mm, nt = 800, receive on
+    the active voxels (p), combined per element."""
+    from kwavehip import (KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, KWaveTransducer,
+                          KWaveTransducerGeometry, RecordField, SimulationOptions, SourceMode, toneBurst)
+    sizes = [(96, 96, 96), (128, 96, 96), (160, 128, 128)]
+    pick = np.random.default_rng(seed).integers(0, 3, n_runs)
+    dx = 1e-4
+    cases = []
+    for r in range(n_runs):
+        shape = sizes[pick[r]]
+        rng = np.random.default_rng(100 + r)
+        c = (1540.0 * (1 + rng.uniform(-0.02, 0.02, shape))).astype(np.float32)
+        rho = (1000.0 * (1 + rng.uniform(-0.02, 0.02, shape))).astype(np.float32)
+        grid = KWaveGrid(shape[0], dx, shape[1], dx, shape[2], dx)
+        grid.setTime(nt, 0.3 * dx / float(c.max()))
+        beam = np.random.default_rng(200 + r)
+        steer, focus = beam.uniform(-20.0, 20.0), beam.uniform(15e-3, 40e-3)
+        geom = KWaveTransducerGeometry(numberElements=32, elementWidth=2, elementLength=24, elementSpacing=0,
+                                       position=(10, (shape[1] - 64) // 2, (shape[2] - 24) // 2), dy=dx, dz=dx)
+        burst = toneBurst(1 / grid.dt, 2e6, 2).astype(np.float32)
+        tr = KWaveTransducer(geom, shape, grid.dt, 1540.0, focusDistance=focus, elevationFocusDistance=20e-3,
+                             steeringAngle=steer, inputSignal=burst)
+        mask, row, delays, scales = tr.velocity_source()
+        src = KWaveSource(uMask=mask, ux=row, uDelay=delays, uScale=scales, uMode=SourceMode.additive)
+        sens = KWaveSensor(mask=mask, record=RecordField.p)
+        cases.append(((grid, KWaveMedium(c, rho), src, sens, SimulationOptions(pmlSize=10, smoothP0=False)), tr))
+    return cases
+
+
+def batch_comparison(n_runs=48, nt=800):
+    """Batched (one hipGraph per step over all runs) against the same runs launched one after
+    another (kwh_sim_run each), both timed from device-resident inputs to combined traces."""
+    from kwavehip import Simulation, SimulationBatch
+    cases = scan_line_cases(n_runs, nt)
+    cells = sum(c[0][0].totalGridPoints for c in cases) * nt
+    plain_s = 0.0
+    ref = []
+    for args, tr in cases:
+        sim = Simulation(*args)
+        t0 = time.perf_counter()
+        sim.run()
+        traces = sim.transducer_traces(tr)
+        plain_s += time.perf_counter() - t0
+        ref.append(traces)
+        sim.close()
+    res = {"runs": n_runs, "steps": nt, "cells_total": int(cells // nt),
+           "plain_Mcell_steps_per_s": round(cells / plain_s / 1e6, 1)}
+    for mode, use_graph in (("graphs", True), ("streams", False)):
+        batch = SimulationBatch([c[0] for c in cases])
+        t0 = time.perf_counter()
+        batch.run(use_graph=use_graph)
+        traces = [sim.transducer_traces(tr) for sim, (_, tr) in zip(batch.sims, cases)]
+        batch_s = time.perf_counter() - t0
+        same = all(np.array_equal(a, b) for a, b in zip(traces, ref))
+        batch.close()
+        res[f"batched_{mode}_Mcell_steps_per_s"] = round(cells / batch_s / 1e6, 1)
+        res[f"speedup_{mode}"] = round(plain_s / batch_s, 3)
+        res[f"traces_identical_{mode}"] = bool(same)
+    res["speedup"] = res["speedup_graphs"]
+    return res
+
+
+def adjoint_case():
+    """2D sound-speed / photoacoustic inversion step: 512^2, dx = 1e-4 m, PML 20, c from a seeded
+    Gaussian random field in [1450, 1650] m/s (seed 11), rho = 1000, p0 = 20 random discs
+    (seed 12, radius 3..12 cells, amplitude U[0.2, 1]), smoothP0 off, 256-point Cartesian circle of
+    radius 22 mm, nt = 1500, L = 1/2 ||p - p_obs||^2 with p_obs from c + 1% random field (seed 13)."""
+    from kwavehip import KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, RecordField, SimulationOptions, makeCartCircle, makeDisc, random_field
+    n, dx = 512, 1e-4
+    grid = KWaveGrid(n, dx, n, dx)
+    c = random_field((n, n), 11, 20.0, 1450.0, 1650.0)
+    grid.setTime(1500, 0.3 * dx / float(c.max()))
+    rng = np.random.default_rng(12)
+    p0 = np.zeros((n, n), np.float32)
+    for _ in range(20):
+        cx, cy = rng.integers(120, n - 120, 2)
+        p0 += float(rng.uniform(0.2, 1.0)) * makeDisc(n, n, cx=int(cx), cy=int(cy), radius=int(rng.integers(3, 13)))
+    pert = random_field((n, n), 13, 20.0, -1.0, 1.0)
+    c_obs = (c.astype(np.float64) * (1.0 + 0.01 * pert)).astype(np.float32)
+    sens = KWaveSensor(mask=makeCartCircle(22e-3, 256), record=RecordField.p)
+    opts = SimulationOptions(pmlSize=20, smoothP0=False)
+    return grid, c, c_obs, p0, sens, opts
+
+
+def adjoint_comparison(n_dirs=16):
+    """Adjoint time (reverse sweep incl. recompute) over forward time, and the c-gradient against
+    central differences of the FP64 forward along random directions that leave max(c) alone."""
+    from kwavehip import DTypePrecision, KWaveMedium, KWaveSource, Simulation
+    import dataclasses
+    grid, c, c_obs, p0, sens, opts = adjoint_case()
+    def forward(cc, dtype=DTypePrecision.float32, ckpt=0):
+        sim = Simulation(grid, KWaveMedium(cc, np.float32(1000.0)), KWaveSource(p0=p0), sens,
+                         dataclasses.replace(opts, dtype=dtype))
+        if ckpt:
+            sim.set_checkpointing(ckpt)
+        t0 = time.perf_counter()
+        sim.run()
+        el = time.perf_counter() - t0
+        return sim, sim.outputs().p.astype(np.float64), el
+    obs, y_obs, _ = forward(c_obs)
+    obs.close()
+    plain, _, _ = forward(c)
+    plain.close()
+    _, _, t_fwd = forward(c)
+    sim, y, t_fwd_ckpt = forward(c, ckpt=32)
+    t0 = time.perf_counter()
+    gp0, gc = sim.vjp((y - y_obs).astype(np.float32), checkpoint_every=32)
+    t_adj = time.perf_counter() - t0
+    sim.close()
+    # central differences of the FP64 forward (c enters as float32: steps of 0.25 m/s are exact)
+    sim64, y64, _ = forward(c, DTypePrecision.float64)
+    sim64.close()
+    c64 = c.astype(np.float64)
+    keep = c64 < c64.max() - 20.0
+    g64 = gc.astype(np.float64)
+    scale = float(np.sqrt(np.sum(np.where(keep, g64, 0.0) ** 2)))   # size of <g, v> for random signs v
+    rng = np.random.default_rng(14)
+    errs, rel = [], []
+    for _ in range(n_dirs):
+        v = np.where(keep, np.sign(rng.standard_normal(c.shape)), 0.0)
+        h = 0.125
+        lp = 0.5 * np.sum((forward((c64 + h * v).astype(np.float32), DTypePrecision.float64)[1] - y_obs) ** 2)
+        lm = 0.5 * np.sum((forward((c64 - h * v).astype(np.float32), DTypePrecision.float64)[1] - y_obs) ** 2)
+        fd = (lp - lm) / (2 * h)
+        an = float(np.sum(g64 * v))
+        errs.append(abs(fd - an) / scale)
+        rel.append(abs(fd - an) / max(abs(fd), 1e-300))
+    return {"grid": [512, 512], "steps": grid.nt, "checkpoint_every": 32,
+            "t_forward_s": round(t_fwd, 4), "t_forward_checkpointed_s": round(t_fwd_ckpt, 4),
+            "t_adjoint_s": round(t_adj, 4), "adjoint_over_forward": round(t_adj / t_fwd, 3),
+            "fd_directions": n_dirs, "grad_c_fd_max_err_over_grad_norm": float(f"{max(errs):.3e}"),
+            "grad_c_fd_median_rel_err": float(f"{float(np.median(rel)):.3e}"),
+            "grad_dtype": "f32", "fd_dtype": "f64"}
+
+
 def cpu_baseline(grid, med, src, sens, opts, steps, workers):
     """Time the float64 oracle (numpy/scipy pocketfft, `workers` threads) on `steps` steps."""
     from oracle import kwave_oracle as O
@@ -128,6 +267,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -205,6 +346,18 @@ def main():
               flush=True)
         fp64 = precision_check(grid, med, src, sens, opts)
 
+    batch = None
+    if not args.no_batch and args.workload == "c2":
+        print("[bench] batched scan lines: 48 runs, batched vs sequential", file=sys.stderr, flush=True)
+        batch = batch_comparison()
+        print(f"[bench] batch speedup {batch['speedup']}", file=sys.stderr, flush=True)
+
+    adjoint = None
+    if not args.no_adjoint and args.workload == "c2":
+        print("[bench] adjoint: 2D 512^2, 1500 steps, checkpoints every 32", file=sys.stderr, flush=True)
+        adjoint = adjoint_comparison()
+        print(f"[bench] adjoint/forward {adjoint['adjoint_over_forward']}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         workers = min(os.cpu_count() or 1, 16)
@@ -222,6 +375,7 @@ def main():
         "vs_baseline": None, "dtype": args.dtype, "data": "synthetic (seeded random media)",
         "config": dict(info, dtype=args.dtype),
         "roofline": roof, "step_model": step_model, "kernels": kernels, "cpu_baseline": cpu, "fp64_gpu": fp64,
+        "batched_scan_lines": batch, "adjoint": adjoint,
         "setup_s": round(setup_s, 2),
         # whole job seen from the host boundary: upload + setup, the timed steps, read-back
         "host_boundary": {"create_s": round(create_s, 3), "read_s": round(read_s, 3),

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KWH_ABI_VERSION 1
+#define KWH_ABI_VERSION 2
 
 typedef enum {
     KWH_OK = 0,
@@ -83,7 +83,10 @@ typedef struct {
 } kwh_field;
 
 /* One time-varying source field (SourceOp, KSpaceFirstOrder.swift:129-201).
- * signal is [rows][len], rows == 1 (broadcast) or rows == n_points. */
+ * signal is [rows][len], rows == 1 (broadcast) or rows == n_points.
+ * Delayed copies (a transducer drive, KWaveTransducer.swift:185-258): with delay != NULL the
+ * signal is one row and point k plays scale[k] * signal[t - delay[k]] (zero outside the row;
+ * scale NULL = 1); the per-point rows, len + max(delay) samples long, are built on the device. */
 typedef struct {
     const int64_t* index;     /* ascending flat indices of mask nonzeros; NULL = absent */
     int64_t n_points;
@@ -91,6 +94,8 @@ typedef struct {
     int32_t rows;
     int32_t len;
     int32_t mode;             /* kwh_source_mode */
+    const int32_t* delay;     /* [n_points] samples >= 0, or NULL */
+    const float* scale;       /* [n_points], or NULL */
 } kwh_source_field;
 
 /* Flattened KWaveGrid + KWaveMedium + KWaveSource + KWaveSensor + SimulationOptions. */
@@ -131,6 +136,7 @@ typedef struct {
 
 typedef struct kwh_ctx kwh_ctx;
 typedef struct kwh_sim kwh_sim;
+typedef struct kwh_batch kwh_batch;
 
 /* Called between step chunks. t is the last completed step index, p_host the pressure field
  * copied to host when the callback is due (every cb_interval steps), else NULL. Replaces the
@@ -197,8 +203,57 @@ int kwh_sim_set_kernel_timing(kwh_sim* sim, int enable);
 int kwh_sim_set_kernel_timing_only(kwh_sim* sim, const char* kernel_name);
 int kwh_sim_kernel_stats(kwh_sim* sim, kwh_kernel_stat* stats, int32_t capacity, int32_t* count);
 
-/* Capture one time step into a hipGraph and replay it for subsequent steps (1 = on). */
+/* Replay the time step from captured hipGraphs (1 = on): one graph per set of active
+ * k-space-corrected sources, captured on first use; results are bitwise those of eager runs. */
 int kwh_sim_set_graph(kwh_sim* sim, int enable);
+
+/* Receive combination on the device (combineSensorData, KWaveTransducer.swift:263-289):
+ * out[e][t] = scale * sum over sensor points k with element[k] == e of
+ * series[k][t + delay[k]] (terms with t + delay[k] beyond the recorded steps are dropped).
+ * field is a recorded series (KWH_OUT_P / _UX / _UY / _UZ); element[] in [0, n_elements),
+ * delay[] >= 0, both [n_sensor]. dst is [n_elements][steps recorded], sim dtype. */
+int kwh_sim_element_traces(kwh_sim* sim, int32_t field, const int32_t* element, const int32_t* delay,
+                           int32_t n_elements, double scale, void* dst, size_t dst_bytes);
+
+/* Time-reversal reconstruction (timeReversal, TimeReversal.swift:20-51): the grid, medium and
+ * options of desc, its binary sensor mask as a Dirichlet pressure source playing `recorded`
+ * ([n_sensor][n_recorded_steps], float, host) reversed in time (on the device); runs the loop and
+ * writes max(0, compensation_factor * pFinal) to p0_out (grid-shaped, desc dtype). Every source
+ * and recording field of desc other than the sensor mask is ignored. */
+int kwh_time_reversal(kwh_ctx* ctx, const kwh_sim_desc* desc, const float* recorded, int32_t n_recorded_steps,
+                      double compensation_factor, void* p0_out);
+
+/* Autotuner (no counterpart in the reference): before the first step, times one full time step
+ * under each launch-schedule candidate (tile columns of the strided / middle passes, row pairs
+ * per workgroup of the row kernels, LDS budget of the generic FFT passes) on this sim's buffers,
+ * keeps the fastest and caches it per grid shape and precision for the process. report (may be
+ * NULL) receives the candidates' step times. The state is reset afterwards. */
+int kwh_sim_autotune(kwh_sim* sim, char* report, size_t report_bytes);
+
+/* Adjoint (reverse-time) gradient; no counterpart in the reference.
+ * kwh_sim_set_checkpointing: save (p, u, rho) after every `every`-th step of later runs
+ * (0 = off), so a following kwh_sim_vjp needs no forward pass of its own.
+ * kwh_sim_vjp: for the p0 problem on the lossless linear loop (no time-varying sources, no
+ * smoothing, pressure series recorded), given dL/dy [n_sensor][nt] of the recorded pressure
+ * (sim dtype), writes dL/dp0 and dL/dc (grid-shaped, sim dtype). The gradient is the partial
+ * derivative at fixed c_ref = max(c) (kappa and the PML held fixed). If the sim has not just
+ * completed nt steps with checkpoints every `checkpoint_every` steps, the forward pass is run
+ * first. The reverse sweep recomputes each checkpoint segment; afterwards the sim is back at its
+ * end-of-run state, outputs unchanged. */
+int kwh_sim_set_checkpointing(kwh_sim* sim, int32_t every);
+int kwh_sim_vjp(kwh_sim* sim, const void* dl_dp, int32_t checkpoint_every, void* grad_p0, void* grad_c0);
+
+/* Batched independent simulations of uneven sizes (per-scan-line transmits) on one device.
+ * kwh_batch_create validates and sets up n sims (descs[i] as for kwh_sim_create).
+ * kwh_batch_run advances every sim to its nt, each on its own stream: use_graph = 1 replays each
+ * sim's captured step graph, interleaving the sims step by step; use_graph = 0 enqueues each sim's
+ * steps eagerly. Outputs are read per sim (kwh_batch_read_output, or the borrowed
+ * handle from kwh_batch_sim with the kwh_sim_* readers). */
+int kwh_batch_create(kwh_ctx* ctx, const kwh_sim_desc* descs, int32_t n, kwh_batch** out);
+int kwh_batch_run(kwh_batch* batch, int32_t use_graph);
+int kwh_batch_sim(kwh_batch* batch, int32_t i, kwh_sim** out);
+int kwh_batch_read_output(kwh_batch* batch, int32_t i, int32_t field, void* dst, size_t dst_bytes);
+void kwh_batch_destroy(kwh_batch* batch);
 
 void kwh_sim_destroy(kwh_sim* sim);
 

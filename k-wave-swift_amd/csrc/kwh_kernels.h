@@ -811,6 +811,49 @@ __global__ void rms_kernel(const double* sum2, T* out, int64_t n, double inv_nt)
         out[s] = T(sqrt(sum2[s] * inv_nt));
 }
 
+// Receive combination (KWaveTransducer.swift:263-289) over a time-major series [nt][n]:
+// out[e][t] = scale * sum_{k in element e} series[t + delay[k]][k], t + delay[k] < nt.
+// rowptr/rows list the sensor points of each element (CSR).
+template <typename T>
+__global__ void element_traces_kernel(const T* series, int64_t n, int nt, const int* rowptr, const int* rows,
+                                      const int* delay, int nel, double scale, T* out) {
+    const int64_t total = (int64_t)nel * nt;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int e = (int)(i / nt);
+        const int t = (int)(i - (int64_t)e * nt);
+        double acc = 0.0;
+        for (int q = rowptr[e]; q < rowptr[e + 1]; ++q) {
+            const int k = rows[q];
+            const int ts = t + delay[k];
+            if (ts < nt) acc += (double)series[(int64_t)ts * n + k];
+        }
+        out[i] = T(acc * scale);
+    }
+}
+
+// Time reversal (TimeReversal.swift:33-50): reverse every source row in time, in place ...
+template <typename T>
+__global__ void reverse_rows_kernel(T* sig, int rows, int len) {
+    const int64_t half = (int64_t)rows * (len / 2);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < half; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / (len / 2);
+        const int t = (int)(i - r * (len / 2));
+        T* row = sig + r * len;
+        const T a = row[t];
+        row[t] = row[len - 1 - t];
+        row[len - 1 - t] = a;
+    }
+}
+
+// ... and the estimate max(0, factor * pFinal).
+template <typename T>
+__global__ void scaled_positive_kernel(const T* p, T* out, T factor, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T v = factor * p[i];
+        out[i] = v > T(0) ? v : T(0);
+    }
+}
+
 // Intensity average (Recording.swift:113-124) from stored series [nt][n]:
 // I = (1/nt) sum_t p[t] * Re(ifft(e^{i w/2} fft(u)))[t] = (1/nt^2) Re sum_k conj(P_k) S_k U_k.
 // One thread per sensor point, direct DFT in float64 (cost n * nt^2).

@@ -158,6 +158,20 @@ __global__ void absorption_kernel(const T* c0, T cs, const T* alpha, T as, int64
     }
 }
 
+// Delayed copies of one drive row (a transducer transmit): rows[k][t] = scale[k] *
+// sig[t - delay[k]] for 0 <= t - delay[k] < len, else 0; rows is [n][len_eff].
+template <typename T>
+__global__ void delayed_rows_kernel(const float* sig, int len, const int* delay, const float* scale, int64_t n,
+                                    int len_eff, T* rows) {
+    const int64_t total = n * (int64_t)len_eff;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = i / len_eff;
+        const int s = (int)(i - k * len_eff) - delay[k];
+        const double a = scale ? (double)scale[k] : 1.0;
+        rows[i] = (s >= 0 && s < len) ? T(a * (double)sig[s]) : T(0);
+    }
+}
+
 template <typename T>
 __global__ void gather_values_kernel(const T* f, const int64_t* idx, int64_t n, double* out) {
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x)

@@ -20,6 +20,7 @@
 #include "kwh_kernels.h"
 #include "kwh_setup.h"
 #include "kwh_sq.h"
+#include "kwh_adjoint.h"
 
 using namespace kwh;
 
@@ -154,6 +155,18 @@ struct kwh_sim {
     virtual void set_timing_only(const char* name) = 0;
     virtual int kernel_stats(kwh_kernel_stat* st, int cap) = 0;
     virtual void set_graph(bool on) = 0;
+    virtual void element_traces(int field, const int32_t* element, const int32_t* delay, int nel, double scale,
+                                void* dst, size_t bytes) = 0;
+    virtual void set_checkpointing(int every) = 0;
+    virtual std::string autotune() = 0;
+    virtual void time_reversal_result(double factor, void* dst) = 0;
+    virtual void vjp(const void* dldy, int every, void* grad_p0, void* grad_c0) = 0;
+    // batched runs (kwh_batch)
+    virtual int total_steps() const = 0;
+    virtual void step_done(int t) = 0;
+    virtual void set_counters_on(hipStream_t s) = 0;
+    virtual void graph_step(int t) = 0;   // replay step t's graph on the sim's own stream
+    virtual int device() const = 0;
 };
 
 namespace {
@@ -231,7 +244,12 @@ struct SimT final : kwh_sim {
     int rec_steps = 0;
     // device step counters {t, record slot}, advanced by a kernel at the end of every step
     DBuf ctr;
-    // captured step graphs, keyed by which k-space-corrected sources the step launches
+    // checkpoints of (p, u_a, rho_a) after every ckpt_every-th step, for the adjoint
+    int ckpt_every = 0;
+    bool smoothed_media = false, smoothed_p0 = false;   // the adjoint does not cover smoothing
+    DBuf ckpt;
+    std::vector<char> ckpt_have;
+    // captured step graphs, keyed by step_key(t)
     struct StepGraph { hipGraph_t graph; hipGraphExec_t exec; };
     std::vector<std::pair<unsigned, StepGraph>> step_graphs;
     // directivity
@@ -252,7 +270,6 @@ struct SimT final : kwh_sim {
     // graph replay of the time step
     bool use_graph = false;
 
-    size_t lds_budget = 64 * 1024;
     // one-element device buffers behind scalar SMaps (branch-free at() in the kernels)
     DBuf scal_pool;
     int scal_n = 0;
@@ -266,11 +283,16 @@ struct SimT final : kwh_sim {
         return SMap<T>{nullptr, v, d, 0};
     }
     int sqR[3] = {0, 0, 0};   // R when the internal axis length is R*R with a register-resident kernel
+    // launch schedule chosen by the autotuner: halve the tile columns of the strided / middle
+    // passes (ntr_shift), halve the row pairs per workgroup of the row kernels (p_shift), LDS
+    // budget of the generic Stockham passes
+    struct Schedule { int ntr_shift = 0, p_shift = 0; size_t lds = 64 * 1024; };
+    Schedule sched;
     bool use_sq = true;
 
     ~SimT() override {
         for (auto& sg : step_graphs) {
-            (void)hipGraphExecDestroy(sg.second.exec);
+            if (sg.second.exec) (void)hipGraphExecDestroy(sg.second.exec);
             (void)hipGraphDestroy(sg.second.graph);
         }
         for (auto& e : ev_pool) (void)hipEventDestroy(e);
@@ -325,7 +347,7 @@ struct SimT final : kwh_sim {
         const int L = g.m2;
         const int npairs = (g.nrows + 1) / 2;
         int ntr = 32;
-        while (ntr > 1 && ((size_t)nbuf * L * (ntr + 1) * sizeof(C) > lds_budget || ntr / 2 >= npairs)) ntr >>= 1;
+        while (ntr > 1 && ((size_t)nbuf * L * (ntr + 1) * sizeof(C) > sched.lds || ntr / 2 >= npairs)) ntr >>= 1;
         ZCfg c{};
         c.zt.ntr = ntr;
         c.zt.lg_ntr = ilog2(ntr);
@@ -346,7 +368,7 @@ struct SimT final : kwh_sim {
         else { c.stride = g.mzh; c.inner = g.mzh; c.ncols = (int64_t)g.m0 * g.mzh; }
         const int L = std::max(c.L, 1);
         int ntr = 64;
-        while (ntr > 1 && ((size_t)nbuf * L * ntr * sizeof(C) + 64 * sizeof(int64_t) > lds_budget || ntr / 2 >= c.ncols))
+        while (ntr > 1 && ((size_t)nbuf * L * ntr * sizeof(C) + 64 * sizeof(int64_t) > sched.lds || ntr / 2 >= c.ncols))
             ntr >>= 1;
         c.lg_ntr = ilog2(ntr);
         c.pitch = ntr;
@@ -367,8 +389,9 @@ struct SimT final : kwh_sim {
         std::vector<void*> v(fields);
         strided_v(axis, inv, v);
     }
-    // columns per tile for the register-resident kernels: 256 lanes (128 in float64)
-    int sq_ntr(int R) const { return (sizeof(T) == 8 ? 128 : 256) / R; }
+    // columns per tile for the register-resident kernels: 256 lanes (128 in float64), or the
+    // autotuner's choice (half of that)
+    int sq_ntr(int R) const { return ((sizeof(T) == 8 ? 128 : 256) / R) >> sched.ntr_shift; }
     void strided_v(int axis, bool inv, const std::vector<void*>& fields) {
         SCfg c = scfg(axis, 2);
         if (c.L <= 1) return;
@@ -471,7 +494,7 @@ struct SimT final : kwh_sim {
     SqZ sqz() const {
         const int R = sqR[2];
         SqZ z{};
-        z.P = (sizeof(T) == 8 ? 128 : 256) / R;
+        z.P = ((sizeof(T) == 8 ? 128 : 256) / R) >> sched.p_shift;
         const size_t per_pair = (size_t)(R * R + R + 1) * sizeof(C);
         const size_t twb = (size_t)R * R * sizeof(C);   // LDS twiddle table
         while (z.P > 1 && z.P * per_pair + twb > 65536) z.P /= 2;
@@ -530,7 +553,7 @@ struct SimT final : kwh_sim {
         dev = device;
         HIPCHK(hipSetDevice(dev));
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        if (const char* e = std::getenv("KWH_LDS_BUDGET")) lds_budget = (size_t)std::atol(e);
+        if (const char* e = std::getenv("KWH_LDS_BUDGET")) sched.lds = (size_t)std::atol(e);
         scal_pool.alloc((size_t)64 * kScalarRep * sizeof(T));
         zero_map = mk_scalar(T(0));
         taus = etas = bonas = zero_map;
@@ -580,6 +603,7 @@ struct SimT final : kwh_sim {
         build_axis_tables_fft();
         if (d.smooth_c0 && c0s.p) smooth_inplace(c0m.as<T>());
         if (d.smooth_rho0 && rho0s.p) smooth_inplace(rho0m.as<T>());
+        smoothed_media = (d.smooth_c0 && c0s.p) || (d.smooth_rho0 && rho0s.p);
 
         // c_ref = max(c0) (:786-788)
         c_ref = c0s.p ? max_abs(c0m.as<T>(), N) : (double)c0s.s;
@@ -868,7 +892,7 @@ struct SimT final : kwh_sim {
         {
             const int L = g.m2, npairs = (g.nrows + 1) / 2;
             int ntr = 32;
-            while (ntr > 1 && ((size_t)2 * L * (ntr + 1) * cs > lds_budget || ntr / 2 >= npairs)) ntr >>= 1;
+            while (ntr > 1 && ((size_t)2 * L * (ntr + 1) * cs > sched.lds || ntr / 2 >= npairs)) ntr >>= 1;
             ZTile zt{0, ntr, ilog2(ntr), ntr > 1 ? ntr + 1 : 1};
             int thr = std::max(64, std::min(256, ((ntr * L / 8 + 63) / 64) * 64));
             const size_t lds = (size_t)2 * L * zt.pitch * cs;
@@ -886,7 +910,7 @@ struct SimT final : kwh_sim {
                 if (axis == 0) { a.stride = (int64_t)g.m1 * g.mzh; a.inner = a.stride; a.ncols = a.stride; }
                 else { a.stride = g.mzh; a.inner = g.mzh; a.ncols = (int64_t)g.m0 * g.mzh; }
                 int nt_ = 64;
-                while (nt_ > 1 && ((size_t)2 * La * nt_ * cs + 64 * 8 > lds_budget || nt_ / 2 >= a.ncols)) nt_ >>= 1;
+                while (nt_ > 1 && ((size_t)2 * La * nt_ * cs + 64 * 8 > sched.lds || nt_ / 2 >= a.ncols)) nt_ >>= 1;
                 a.lg_ntr = ilog2(nt_);
                 a.pitch = nt_;
                 const int th = std::max(64, std::min(256, ((nt_ * La / 8 + 63) / 64) * 64));
@@ -911,7 +935,7 @@ struct SimT final : kwh_sim {
                 if (ax == 0) { a.stride = (int64_t)g.m1 * g.mzh; a.inner = a.stride; a.ncols = a.stride; }
                 else { a.stride = g.mzh; a.inner = g.mzh; a.ncols = (int64_t)g.m0 * g.mzh; }
                 int nt_ = 64;
-                while (nt_ > 1 && ((size_t)3 * a.L * nt_ * cs + 64 * 8 > lds_budget || nt_ / 2 >= a.ncols)) nt_ >>= 1;
+                while (nt_ > 1 && ((size_t)3 * a.L * nt_ * cs + 64 * 8 > sched.lds || nt_ / 2 >= a.ncols)) nt_ >>= 1;
                 a.lg_ntr = ilog2(nt_);
                 a.pitch = nt_;
                 a.scale = 1.0 / (double)g.N;
@@ -970,9 +994,28 @@ struct SimT final : kwh_sim {
         for (int a = 0; a < 3; ++a)
             sd.factor[a] = (sf.mode == KWH_SRC_DIRICHLET || a >= dim) ? 1.0 : 1.0 / dphys[a];
         upload(sd.base, base);
-        std::vector<T> sig((size_t)sf.rows * sf.len);
-        for (size_t i = 0; i < sig.size(); ++i) sig[i] = T(sf.signal[i]);
-        upload(sd.signal, sig);
+        if (sf.delay) {
+            // delayed copies of one drive row, expanded into per-point rows on the device
+            int maxd = 0;
+            for (int64_t i = 0; i < sf.n_points; ++i) maxd = std::max(maxd, (int)sf.delay[i]);
+            const int len_eff = sf.len + maxd;
+            DBuf sig1, dly, scl;
+            upload(sig1, std::vector<float>(sf.signal, sf.signal + sf.len));
+            upload(dly, std::vector<int>(sf.delay, sf.delay + sf.n_points));
+            if (sf.scale) upload(scl, std::vector<float>(sf.scale, sf.scale + sf.n_points));
+            sd.signal.alloc((size_t)sf.n_points * len_eff * sizeof(T));
+            hipLaunchKernelGGL((delayed_rows_kernel<T>), dim3(grid_blocks((int64_t)sf.n_points * len_eff)), dim3(256), 0,
+                               stream, sig1.as<float>(), sf.len, dly.as<int>(), scl.p ? scl.as<float>() : nullptr,
+                               sf.n_points, len_eff, sd.signal.template as<T>());
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream));
+            sd.rows = (int)sf.n_points;
+            sd.len = len_eff;
+        } else {
+            std::vector<T> sig((size_t)sf.rows * sf.len);
+            for (size_t i = 0; i < sig.size(); ++i) sig[i] = T(sf.signal[i]);
+            upload(sd.signal, sig);
+        }
         if (sf.mode != KWH_SRC_ADDITIVE) {
             std::vector<int> rp(g.nrows + 1, 0);
             for (int64_t i = 0; i < sf.n_points; ++i) rp[idx[i] / g.m2 + 1]++;
@@ -1006,6 +1049,7 @@ struct SimT final : kwh_sim {
             std::vector<T> h(d.p0.data, d.p0.data + g.N);
             upload(p0s, h);
             if (d.smooth_p0) smooth_inplace(p0s.as<T>());
+            smoothed_p0 = d.smooth_p0 != 0;
         }
         KWH_REQUIRE(has_p0 || psrc.present || usrc[0].present || usrc[1].present || usrc[2].present ||
                         d.p_source.index != nullptr || d.u_source[0].index != nullptr,
@@ -1174,6 +1218,7 @@ struct SimT final : kwh_sim {
         P_valid = true;
         steps = 0;
         rec_steps = 0;
+        std::fill(ckpt_have.begin(), ckpt_have.end(), 0);
     }
 
     // ------------------------------------------------------------------ the time step
@@ -1478,43 +1523,395 @@ struct SimT final : kwh_sim {
         hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, stream, (int*)ctr.p, recorded);
         HIPCHK(hipGetLastError());
     }
-    void step_done(int t) {
+    void step_done(int t) override {
         if (rec_p || rec_u) ++rec_steps;
         P_valid = true;
         steps = t + 1;
     }
 
-    // Which k-space-corrected additive sources step t launches (the only host-side decision in
-    // a step: everything else reads t from the device counters).
-    unsigned step_signature(int t) const {
+    // What step t launches beyond the fixed step: which k-space-corrected additive sources are
+    // active (bits 0-3) and whether it is the t = 0 p0 step (bit 4). Everything else reads t
+    // from the device counters, so steps with one key share one graph.
+    unsigned step_key(int t) const {
         unsigned key = 0;
         for (int a = 0; a < dim; ++a)
             if (usrc[a].present && usrc[a].mode == KWH_SRC_ADDITIVE && t < usrc[a].len) key |= 1u << a;
         if (psrc.present && psrc.mode == KWH_SRC_ADDITIVE && t < psrc.len) key |= 8u;
+        if (t == 0 && has_p0) key |= 16u;
         return key;
+    }
+    StepGraph& graph_for(int t) {
+        const unsigned key = step_key(t);
+        for (auto& e : step_graphs)
+            if (e.first == key) return e.second;
+        StepGraph ng{};
+        HIPCHK(hipSetDevice(dev));
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        step_launches(t);
+        HIPCHK(hipStreamEndCapture(stream, &ng.graph));
+        step_graphs.emplace_back(key, ng);
+        return step_graphs.back().second;
+    }
+    void graph_step(int t) override {
+        StepGraph& sg = graph_for(t);
+        if (!sg.exec) HIPCHK(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphLaunch(sg.exec, stream));
+        step_done(t);
+    }
+    int total_steps() const override { return nt; }
+    int device() const override { return dev; }
+    void set_counters_on(hipStream_t s) override {   // nullptr: the sim's own stream
+        HIPCHK(hipSetDevice(dev));
+        hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, s ? s : stream, (int*)ctr.p, steps, rec_steps);
+        HIPCHK(hipGetLastError());
     }
 
     void step(int t) {
-        if (use_graph && !timing && !(t == 0 && has_p0)) {
-            const unsigned key = step_signature(t);
-            const StepGraph* sg = nullptr;
-            for (auto& e : step_graphs)
-                if (e.first == key) sg = &e.second;
-            if (!sg) {
-                // captured once per signature, replayed for every later step with it
-                StepGraph ng{};
-                HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                step_launches(t);
-                HIPCHK(hipStreamEndCapture(stream, &ng.graph));
-                HIPCHK(hipGraphInstantiate(&ng.exec, ng.graph, nullptr, nullptr, 0));
-                step_graphs.emplace_back(key, ng);
-                sg = &step_graphs.back().second;
-            }
-            HIPCHK(hipGraphLaunch(sg->exec, stream));
+        if (use_graph && !timing) {
+            StepGraph& sg = graph_for(t);
+            if (!sg.exec) HIPCHK(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0));
+            HIPCHK(hipGraphLaunch(sg.exec, stream));
         } else {
             step_launches(t);
         }
         step_done(t);
+        if (ckpt_every > 0 && t % ckpt_every == 0) save_checkpoint(t / ckpt_every);
+    }
+
+    // ------------------------------------------------------------------ time reversal
+    // Runs the loop with the p source rows reversed in time on the device (the recorded data
+    // re-injected as a Dirichlet boundary), then writes max(0, factor * pFinal).
+    void time_reversal_result(double factor, void* dst) override {
+        KWH_REQUIRE(psrc.present && psrc.mode == KWH_SRC_DIRICHLET, KWH_EINVAL, "time reversal needs its Dirichlet source");
+        HIPCHK(hipSetDevice(dev));
+        hipLaunchKernelGGL((reverse_rows_kernel<T>), dim3(grid_blocks((int64_t)psrc.rows * std::max(psrc.len / 2, 1))),
+                           dim3(256), 0, stream, psrc.signal.template as<T>(), psrc.rows, psrc.len);
+        HIPCHK(hipGetLastError());
+        launch(0, nt);
+        DBuf out;
+        out.alloc(g.N * sizeof(T));
+        hipLaunchKernelGGL((scaled_positive_kernel<T>), dim3(grid_blocks(g.N)), dim3(256), 0, stream, p.as<T>(),
+                           out.as<T>(), T(factor), g.N);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dst, out.p, g.N * sizeof(T), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // ------------------------------------------------------------------ autotuner
+    // Times one full time step (row, strided and middle passes together) under each launch
+    // schedule candidate on this sim's own buffers and keeps the fastest; the choice is cached
+    // per grid shape, dimension and precision for later sims in the process. State is reset.
+    std::string autotune() override {
+        KWH_REQUIRE(steps == 0, KWH_ESTATE, "autotune before the first step");
+        HIPCHK(hipSetDevice(dev));
+        sync();
+        static std::vector<std::pair<std::string, Schedule>> cache;
+        char keyb[96];
+        std::snprintf(keyb, sizeof(keyb), "%d:%d:%d:%d:%zu:%d%d%d", dim, g.m0, g.m1, g.m2, sizeof(T), sqR[0], sqR[1],
+                      sqR[2]);
+        const std::string key(keyb);
+        std::string report;
+        for (auto& e : cache)
+            if (e.first == key) {
+                sched = e.second;
+                report = "cached schedule for " + key + "\n";
+                return report;
+            }
+        const bool sq_cols = sizeof(T) == 4 && (sqR[mid_axis] || (plain1 && sqR[1]));
+        bool generic = false;
+        const int m[3] = {g.m0, g.m1, g.m2};
+        for (int ax = 0; ax < 3; ++ax) generic |= m[ax] > 1 && sqR[ax] == 0;
+        std::vector<Schedule> cands;
+        for (int ns = 0; ns <= (sq_cols ? 1 : 0); ++ns)
+            for (int ps = 0; ps <= (sqR[2] ? 1 : 0); ++ps)
+                for (int lb = 0; lb <= (generic ? 1 : 0); ++lb) {
+                    Schedule c;
+                    c.ntr_shift = ns;
+                    c.p_shift = ps;
+                    c.lds = lb ? 32 * 1024 : 64 * 1024;
+                    cands.push_back(c);
+                }
+        for (auto& sg : step_graphs) {
+            if (sg.second.exec) (void)hipGraphExecDestroy(sg.second.exec);
+            (void)hipGraphDestroy(sg.second.graph);
+        }
+        step_graphs.clear();
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const bool tm = timing;
+        timing = false;
+        float best = 1e30f;
+        Schedule chosen = sched;
+        for (const Schedule& c : cands) {
+            sched = c;
+            reset_state();
+            main_step(1);
+            HIPCHK(hipEventRecord(e0, stream));
+            for (int i = 0; i < 3; ++i) main_step(1);
+            HIPCHK(hipEventRecord(e1, stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            ms /= 3;
+            char line[160];
+            std::snprintf(line, sizeof(line), "ntr_shift=%d p_shift=%d lds=%zu: %.4f ms/step\n", c.ntr_shift, c.p_shift,
+                          c.lds, ms);
+            report += line;
+            if (ms < best) {
+                best = ms;
+                chosen = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        timing = tm;
+        sched = chosen;
+        reset_state();
+        sync();
+        cache.emplace_back(key, chosen);
+        char line[160];
+        std::snprintf(line, sizeof(line), "chosen: ntr_shift=%d p_shift=%d lds=%zu\n", chosen.ntr_shift, chosen.p_shift,
+                      chosen.lds);
+        return report + line;
+    }
+
+    // ------------------------------------------------------------------ adjoint (VJP)
+    size_t ckpt_slot_bytes() const { return (size_t)(1 + 2 * dim) * g.N * sizeof(T); }
+    void set_checkpointing(int every) override {
+        KWH_REQUIRE(every >= 0, KWH_EINVAL, "checkpoint interval must be >= 0");
+        sync();
+        ckpt_every = every;
+        ckpt_have.clear();
+        ckpt.alloc(0);
+        if (every > 0) {
+            const int n = (nt + every - 1) / every;
+            ckpt.alloc((size_t)n * ckpt_slot_bytes());
+            ckpt_have.assign(n, 0);
+        }
+    }
+    // state fields in checkpoint order
+    std::vector<T*> ckpt_fields() {
+        std::vector<T*> f = {p.as<T>()};
+        for (int a = 0; a < dim; ++a) f.push_back(u[a].as<T>());
+        for (int a = 0; a < dim; ++a) f.push_back(rho[a].as<T>());
+        return f;
+    }
+    void copy_state(char* base, bool to_ckpt) {
+        const size_t fb = g.N * sizeof(T);
+        int i = 0;
+        for (T* f : ckpt_fields()) {
+            char* c = base + (size_t)i++ * fb;
+            HIPCHK(hipMemcpyAsync(to_ckpt ? (void*)c : (void*)f, to_ckpt ? (const void*)f : (const void*)c, fb,
+                                  hipMemcpyDeviceToDevice, stream));
+        }
+    }
+    void save_checkpoint(int k) {
+        copy_state(reinterpret_cast<char*>(ckpt.p) + (size_t)k * ckpt_slot_bytes(), true);
+        ckpt_have[k] = 1;
+    }
+    // spec[0] = R2C_z(p), the derived state a step starts from
+    void rebuild_P() {
+        if (sqR[2]) {
+            zlaunch(K_ZFR, real_bytes() + spec_bytes(), sq_z_forward_real<T, 16>, sq_z_forward_real<T, 8>,
+                    (const T*)p.as<T>(), spec[0].as<C>(), T(1), g);
+        } else {
+            ZCfg zc = zcfg(2);
+            hipLaunchKernelGGL((z_forward_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, p.as<T>(),
+                               spec[0].as<C>(), T(1), g, zc.zt, tab);
+            HIPCHK(hipGetLastError());
+        }
+        P_valid = true;
+    }
+
+    // dst (= or +=) sum_a D-_a[scale r_a s_a^spow lu_a]
+    void adj_sum_dneg(T* dst, bool accumulate, const std::vector<DBuf>& lu, int spow, T scale, DBuf& w) {
+        std::vector<const void*> ins;
+        std::vector<MidOp> ops;
+        for (int a = 0; a < dim; ++a) {
+            const int ia = a + g.off;
+            hipLaunchKernelGGL((adj_prediv_kernel<T>), dim3(grid_blocks(g.N)), dim3(256), 0, stream, w.as<T>(),
+                               lu[a].as<T>(), dtrs[a], tab.pmlsg[ia], ia, spow, scale, g);
+            HIPCHK(hipGetLastError());
+            forward_to_middle(w.as<T>(), spec[a].as<C>(), T(1));
+            ins.push_back(spec[a].p);
+            ops.push_back(MidOp{OP_DDXN_KAPPA, ia, a, spec[a].p, kappa.p});
+        }
+        middle(K_MD, ins, ops);
+        for (int a = 0; a < dim; ++a) inverse_from_middle(spec[a].as<C>(), dst, T(1), accumulate || a > 0);
+    }
+
+    // Gradient of <dL/dy, y> (y = the recorded pressure series) with respect to p0 and c.
+    void vjp(const void* dldy, int every, void* grad_p0, void* grad_c0) override {
+        KWH_REQUIRE(dldy && grad_p0 && grad_c0, KWH_EINVAL, "NULL argument");
+        KWH_REQUIRE(every > 0, KWH_EINVAL, "checkpoint interval must be > 0");
+        KWH_REQUIRE(has_p0, KWH_EUNSUPPORTED, "the gradient is defined for the p0 problem");
+        KWH_REQUIRE(!absorbing && !nonlinear, KWH_EUNSUPPORTED, "the adjoint covers the lossless linear loop");
+        KWH_REQUIRE(!psrc.present && !usrc[0].present && !usrc[1].present && !usrc[2].present, KWH_EUNSUPPORTED,
+                    "the adjoint does not cover time-varying sources");
+        KWH_REQUIRE(!smoothed_media && !smoothed_p0, KWH_EUNSUPPORTED,
+                    "the adjoint needs smoothP0 / smoothC0 / smoothRho0 off");
+        KWH_REQUIRE(store_p && (record & KWH_REC_P) && ndir == 0, KWH_EINVAL,
+                    "the adjoint needs the pressure series recorded (no directivity)");
+        HIPCHK(hipSetDevice(dev));
+        const int K = every;
+        const int nseg = (nt + K - 1) / K;
+        bool have = ckpt_every == K && steps == nt && (int)ckpt_have.size() == nseg;
+        for (int k = 0; have && k < nseg; ++k) have = ckpt_have[k] != 0;
+        if (!have) {   // forward pass with checkpoints
+            const bool gr = use_graph;
+            set_checkpointing(K);
+            reset_state();
+            launch(0, nt);
+            use_graph = gr;
+        }
+        sync();
+        const int64_t N = g.N;
+        // keep the final state (the outputs pFinal / uFinal read it) across the reverse sweep
+        DBuf fin;
+        fin.alloc(ckpt_slot_bytes());
+        copy_state(reinterpret_cast<char*>(fin.p), true);
+        // cotangent, time-major on the device
+        std::vector<T> h((size_t)nt * nsens);
+        const T* src = reinterpret_cast<const T*>(dldy);
+        for (int64_t k = 0; k < nsens; ++k)
+            for (int t = 0; t < nt; ++t) h[(size_t)t * nsens + k] = src[(size_t)k * nt + t];
+        DBuf dg;
+        upload(dg, h);
+        std::vector<DBuf> lu(dim), lr(dim);
+        for (int a = 0; a < dim; ++a) {
+            lu[a].alloc(N * sizeof(T));
+            lr[a].alloc(N * sizeof(T));
+            fill<T>(lu[a].as<T>(), T(0), N);
+            fill<T>(lr[a].as<T>(), T(0), N);
+        }
+        DBuf lp, w, gc, hist, t0;
+        lp.alloc(N * sizeof(T));
+        w.alloc(N * sizeof(T));
+        gc.alloc(N * sizeof(T));
+        fill<T>(gc.as<T>(), T(0), N);
+        hist.alloc((size_t)K * N * sizeof(T));
+        auto scatter = [&](int n) {
+            hipLaunchKernelGGL((adj_scatter_kernel<T>), dim3(grid_blocks(nsens * ncorner)), dim3(256), 0, stream,
+                               lp.as<T>(), dg.as<T>() + (size_t)n * nsens, sidx.as<int64_t>(),
+                               cart ? sw.as<T>() : nullptr, ncorner, nsens);
+            HIPCHK(hipGetLastError());
+        };
+        const size_t fb = N * sizeof(T);
+        for (int seg = nseg - 1; seg >= 0; --seg) {
+            const int s0 = seg * K, s1 = std::min(s0 + K, nt);
+            // recompute the segment's pressures from its checkpoint
+            copy_state(reinterpret_cast<char*>(ckpt.p) + (size_t)seg * ckpt_slot_bytes(), false);
+            rebuild_P();
+            HIPCHK(hipMemcpyAsync(hist.p, p.p, fb, hipMemcpyDeviceToDevice, stream));
+            for (int t = s0 + 1; t < s1; ++t) {
+                main_step(t);
+                HIPCHK(hipMemcpyAsync(reinterpret_cast<char*>(hist.p) + (size_t)(t - s0) * fb, p.p, fb,
+                                      hipMemcpyDeviceToDevice, stream));
+            }
+            for (int n = s1 - 1; n >= std::max(s0, 1); --n) {
+                // lp = S^T g^n + sum_a D-_a[r_a s_a lu_a]
+                if (n < nt - 1) adj_sum_dneg(lp.as<T>(), false, lu, 1, T(1), w);
+                else fill<T>(lp.as<T>(), T(0), N);
+                scatter(n);
+                hipLaunchKernelGGL((adj_grad_c_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream, gc.as<T>(),
+                                   (const T*)lp.as<T>(), (const T*)(hist.as<T>() + (size_t)(n - s0) * N), c0s, N);
+                HIPCHK(hipGetLastError());
+                // lr_a = c^2 lp + q_a^2 lr_a;  lu_a = s_a^2 lu_a + D+_a[m q_a lr_a]
+                std::vector<const void*> ins;
+                std::vector<MidOp> ops;
+                for (int a = 0; a < dim; ++a) {
+                    const int ia = a + g.off;
+                    hipLaunchKernelGGL((adj_rho_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream, lr[a].as<T>(),
+                                       w.as<T>(), (const T*)lp.as<T>(), c0s, rho0s, T(dt), tab.pml[ia], ia, g);
+                    HIPCHK(hipGetLastError());
+                    forward_to_middle(w.as<T>(), spec[a].as<C>(), T(1));
+                    ins.push_back(spec[a].p);
+                    ops.push_back(MidOp{OP_DDXP_KAPPA, ia, a, spec[a].p, kappa.p});
+                }
+                middle(K_MG, ins, ops);
+                for (int a = 0; a < dim; ++a) {
+                    const int ia = a + g.off;
+                    hipLaunchKernelGGL((adj_u_decay_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream,
+                                       lu[a].as<T>(), tab.pmlsg[ia], ia, g);
+                    HIPCHK(hipGetLastError());
+                    inverse_from_middle(spec[a].as<C>(), lu[a].as<T>(), T(1), true);
+                }
+            }
+        }
+        // n = 0: the p0 override
+        if (nt > 1) adj_sum_dneg(lp.as<T>(), false, lu, 1, T(1), w);
+        else fill<T>(lp.as<T>(), T(0), N);
+        scatter(0);
+        t0.alloc(N * sizeof(T));
+        if (nt > 1) adj_sum_dneg(t0.as<T>(), false, lu, 2, T(0.5), w);
+        else fill<T>(t0.as<T>(), T(0), N);
+        DBuf gp0;
+        gp0.alloc(N * sizeof(T));
+        AdjFinalArgs<T> fa{};
+        fa.gp0 = gp0.as<T>();
+        fa.gc = gc.as<T>();
+        fa.lp0 = lp.as<T>();
+        fa.t0 = t0.as<T>();
+        for (int a = 0; a < dim; ++a) {
+            fa.lr[a] = lr[a].as<T>();
+            fa.q[a] = tab.pml[a + g.off];
+        }
+        fa.p0 = p0s.as<T>();
+        fa.c0 = c0s;
+        fa.dim = dim;
+        hipLaunchKernelGGL((adj_final_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream, fa, g);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(grad_p0, gp0.p, fb, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(grad_c0, gc.p, fb, hipMemcpyDeviceToHost, stream));
+        // back to the end-of-run state
+        copy_state(reinterpret_cast<char*>(fin.p), false);
+        rebuild_P();
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // combineSensorData on the device (KWaveTransducer.swift:263-289).
+    void element_traces(int field, const int32_t* element, const int32_t* delay, int nel, double scale, void* dst,
+                        size_t bytes) override {
+        const DBuf* ser = nullptr;
+        if (field == KWH_OUT_P) {
+            KWH_REQUIRE(store_p && (record & KWH_REC_P), KWH_EINVAL, "p not recorded");
+            ser = &ser_p;
+        } else if (field >= KWH_OUT_UX && field <= KWH_OUT_UZ) {
+            const int c = field - KWH_OUT_UX;
+            KWH_REQUIRE(c < dim && ser_u[c].p && (record & (KWH_REC_UX << c)), KWH_EINVAL,
+                        "velocity series not recorded");
+            ser = &ser_u[c];
+        } else {
+            KWH_REQUIRE(false, KWH_EINVAL, "element traces need a recorded p or u series");
+        }
+        KWH_REQUIRE(element && delay && nel > 0, KWH_EINVAL, "element / delay arrays missing");
+        KWH_REQUIRE(bytes >= (size_t)nel * rec_steps * sizeof(T), KWH_EINVAL, "destination buffer too small");
+        std::vector<int> rowptr(nel + 1, 0), rows(nsens), dl(nsens);
+        for (int64_t k = 0; k < nsens; ++k) {
+            KWH_REQUIRE(element[k] >= 0 && element[k] < nel, KWH_EINVAL, "element index out of range");
+            KWH_REQUIRE(delay[k] >= 0, KWH_EINVAL, "receive delays must be >= 0");
+            rowptr[element[k] + 1]++;
+            dl[k] = delay[k];
+        }
+        for (int e = 0; e < nel; ++e) rowptr[e + 1] += rowptr[e];
+        std::vector<int> fillp(rowptr.begin(), rowptr.end() - 1);
+        for (int64_t k = 0; k < nsens; ++k) rows[fillp[element[k]]++] = (int)k;
+        HIPCHK(hipSetDevice(dev));
+        sync();
+        DBuf drp, drows, ddl, out;
+        upload(drp, rowptr);
+        upload(drows, rows);
+        upload(ddl, dl);
+        const int64_t total = (int64_t)nel * rec_steps;
+        out.alloc(std::max<int64_t>(total, 1) * sizeof(T));
+        if (total > 0) {
+            hipLaunchKernelGGL((element_traces_kernel<T>), dim3(grid_blocks(total)), dim3(256), 0, stream, ser->as<T>(),
+                               nsens, rec_steps, drp.as<int>(), drows.as<int>(), ddl.as<int>(), nel, scale, out.as<T>());
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(dst, out.p, total * sizeof(T), hipMemcpyDeviceToHost, stream));
+        }
+        HIPCHK(hipStreamSynchronize(stream));
     }
 
     // ------------------------------------------------------------------ API
@@ -1751,6 +2148,40 @@ static int guarded(F&& f) {
     }
 }
 
+// Independent simulations advanced together, each on its own stream (the runtime spreads the
+// streams over the hardware queues, so kernels of small grids overlap). With graphs, every sim
+// replays its own captured step graphs (one per step key); the host issues one graph launch per
+// sim per step, interleaving the sims step by step. Without graphs each sim enqueues its whole
+// run eagerly.
+struct kwh_batch {
+    int dev = 0;
+    std::vector<kwh_sim*> sims;
+
+    ~kwh_batch() {
+        for (kwh_sim* s : sims) delete s;
+    }
+    void run(bool use_graph) {
+        HIPCHK(hipSetDevice(dev));
+        for (kwh_sim* s : sims) s->sync();
+        if (!use_graph) {
+            for (kwh_sim* s : sims) s->launch(s->steps_done(), s->total_steps());
+            for (kwh_sim* s : sims) s->sync();
+            return;
+        }
+        for (kwh_sim* s : sims) s->set_counters_on(nullptr);
+        for (bool any = true; any;) {
+            any = false;
+            for (kwh_sim* s : sims) {
+                const int t = s->steps_done();
+                if (t >= s->total_steps()) continue;
+                s->graph_step(t);
+                any = true;
+            }
+        }
+        for (kwh_sim* s : sims) s->sync();
+    }
+};
+
 static void validate_desc(const kwh_sim_desc* d) {
     KWH_REQUIRE(d != nullptr, KWH_EINVAL, "descriptor is NULL");
     KWH_REQUIRE(d->abi_version == KWH_ABI_VERSION, KWH_EINVAL, "descriptor ABI version mismatch");
@@ -1799,6 +2230,15 @@ static void validate_desc(const kwh_sim_desc* d) {
                     std::string(name) + ": signal must have 1 row or one row per source point");
         KWH_REQUIRE(s.len >= 0, KWH_EINVAL, std::string(name) + ": signal length must be >= 0");
         KWH_REQUIRE(s.mode >= 0 && s.mode <= 2, KWH_EINVAL, std::string(name) + ": invalid source mode");
+        if (s.delay) {
+            KWH_REQUIRE(s.rows == 1, KWH_EINVAL, std::string(name) + ": delayed copies need a single signal row");
+            int64_t mx = 0;
+            for (int64_t i = 0; i < s.n_points; ++i) {
+                KWH_REQUIRE(s.delay[i] >= 0, KWH_EINVAL, std::string(name) + ": delays must be >= 0");
+                mx = std::max<int64_t>(mx, s.delay[i]);
+            }
+            KWH_REQUIRE((int64_t)s.len + mx < (int64_t)1 << 30, KWH_EINVAL, std::string(name) + ": delay too large");
+        }
         for (int64_t i = 0; i < s.n_points; ++i) {
             KWH_REQUIRE(s.index[i] >= 0 && s.index[i] < N, KWH_EINVAL, std::string(name) + ": index outside the grid");
             KWH_REQUIRE(i == 0 || s.index[i] > s.index[i - 1], KWH_EINVAL,
@@ -1971,5 +2411,118 @@ int kwh_sim_set_graph(kwh_sim* sim, int enable) {
 }
 
 void kwh_sim_destroy(kwh_sim* sim) { delete sim; }
+
+int kwh_sim_element_traces(kwh_sim* sim, int32_t field, const int32_t* element, const int32_t* delay,
+                           int32_t n_elements, double scale, void* dst, size_t dst_bytes) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && dst, KWH_EINVAL, "NULL argument");
+        sim->element_traces(field, element, delay, n_elements, scale, dst, dst_bytes);
+    });
+}
+
+int kwh_time_reversal(kwh_ctx* ctx, const kwh_sim_desc* desc, const float* recorded, int32_t n_recorded_steps,
+                      double compensation_factor, void* p0_out) {
+    return guarded([&] {
+        KWH_REQUIRE(ctx && desc && recorded && p0_out, KWH_EINVAL, "NULL argument");
+        KWH_REQUIRE(desc->sensor_index && desc->n_sensor > 0 && !desc->sensor_cartesian, KWH_EINVAL,
+                    "time reversal requires a binary sensor.mask");
+        KWH_REQUIRE(n_recorded_steps > 0, KWH_EINVAL, "recordedPressure must be [numSensorPoints, nt]");
+        kwh_sim_desc d = *desc;
+        d.p0 = kwh_field{nullptr, 0};
+        d.p_source = kwh_source_field{desc->sensor_index, desc->n_sensor, recorded, (int32_t)desc->n_sensor,
+                                      n_recorded_steps, KWH_SRC_DIRICHLET, nullptr, nullptr};
+        for (auto& u : d.u_source) u = kwh_source_field{nullptr, 0, nullptr, 0, 0, 0, nullptr, nullptr};
+        d.sensor_index = nullptr;
+        d.n_sensor = 0;
+        d.sensor_cartesian = nullptr;
+        d.n_cartesian = 0;
+        d.record = 0;
+        d.directivity_angle = nullptr;
+        validate_desc(&d);
+        std::unique_ptr<kwh_sim> sim;
+        if (d.dtype == KWH_FLOAT32) {
+            auto s = std::make_unique<SimT<float>>();
+            s->setup(d, ctx->device);
+            sim = std::move(s);
+        } else {
+            auto s = std::make_unique<SimT<double>>();
+            s->setup(d, ctx->device);
+            sim = std::move(s);
+        }
+        sim->time_reversal_result(compensation_factor, p0_out);
+    });
+}
+
+int kwh_sim_autotune(kwh_sim* sim, char* report, size_t report_bytes) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        const std::string r = sim->autotune();
+        if (report && report_bytes > 0) std::snprintf(report, report_bytes, "%s", r.c_str());
+    });
+}
+
+int kwh_sim_set_checkpointing(kwh_sim* sim, int32_t every) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        sim->set_checkpointing(every);
+    });
+}
+
+int kwh_sim_vjp(kwh_sim* sim, const void* dl_dp, int32_t checkpoint_every, void* grad_p0, void* grad_c0) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        sim->vjp(dl_dp, checkpoint_every, grad_p0, grad_c0);
+    });
+}
+
+int kwh_batch_create(kwh_ctx* ctx, const kwh_sim_desc* descs, int32_t n, kwh_batch** out) {
+    return guarded([&] {
+        KWH_REQUIRE(ctx && descs && out && n > 0, KWH_EINVAL, "NULL argument or empty batch");
+        *out = nullptr;
+        for (int i = 0; i < n; ++i) validate_desc(&descs[i]);
+        auto b = std::make_unique<kwh_batch>();
+        b->dev = ctx->device;
+        HIPCHK(hipSetDevice(b->dev));
+        for (int i = 0; i < n; ++i) {
+            kwh_sim* s = nullptr;
+            if (descs[i].dtype == KWH_FLOAT32) {
+                auto t = std::make_unique<SimT<float>>();
+                t->setup(descs[i], ctx->device);
+                s = t.release();
+            } else {
+                auto t = std::make_unique<SimT<double>>();
+                t->setup(descs[i], ctx->device);
+                s = t.release();
+            }
+            b->sims.push_back(s);
+        }
+        *out = b.release();
+    });
+}
+
+int kwh_batch_run(kwh_batch* batch, int32_t use_graph) {
+    return guarded([&] {
+        KWH_REQUIRE(batch, KWH_EINVAL, "batch is NULL");
+        batch->run(use_graph != 0);
+    });
+}
+
+int kwh_batch_sim(kwh_batch* batch, int32_t i, kwh_sim** out) {
+    return guarded([&] {
+        KWH_REQUIRE(batch && out, KWH_EINVAL, "NULL argument");
+        KWH_REQUIRE(i >= 0 && i < (int)batch->sims.size(), KWH_EINVAL, "batch index out of range");
+        *out = batch->sims[i];
+    });
+}
+
+int kwh_batch_read_output(kwh_batch* batch, int32_t i, int32_t field, void* dst, size_t dst_bytes) {
+    return guarded([&] {
+        KWH_REQUIRE(batch && dst, KWH_EINVAL, "NULL argument");
+        KWH_REQUIRE(i >= 0 && i < (int)batch->sims.size(), KWH_EINVAL, "batch index out of range");
+        batch->sims[i]->read_output(field, dst, dst_bytes);
+    });
+}
+
+void kwh_batch_destroy(kwh_batch* batch) { delete batch; }
 
 }  // extern "C"

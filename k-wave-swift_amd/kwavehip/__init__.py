@@ -4,14 +4,16 @@ The compute runs in libkwavehip.so through the C ABI of include/kwave_hip.h; thi
 host-side mirror of the reference's Swift API (Sources/KWave/...) plus input helpers.
 """
 from .api import (AbsorptionMode, DeviceKind, DirectivityPattern, DTypePrecision, KWaveGrid, KWaveMedium,
-                  KWaveSensor, KWaveSource, RecordField, Simulation, SimulationOptions, SimulationOutput,
-                  SourceMode, flatNonzeroIndices, kspaceFirstOrder)
+                  KWaveSensor, KWaveSource, RecordField, Simulation, SimulationBatch, SimulationOptions,
+                  SimulationOutput, SourceMode, flatNonzeroIndices, kspaceFirstOrder, timeReversal)
 from .inputs import (db2neper, gaussian, getWin, makeBall, makeCartCircle, makeCircle, makeDisc, neper2db,
                      random_field, toneBurst)
+from .transducer import DelayMaskMode, KWaveTransducer, KWaveTransducerGeometry
 
 __all__ = [
     "AbsorptionMode", "DeviceKind", "DirectivityPattern", "DTypePrecision", "KWaveGrid", "KWaveMedium",
-    "KWaveSensor", "KWaveSource", "RecordField", "Simulation", "SimulationOptions", "SimulationOutput",
+    "KWaveSensor", "KWaveSource", "RecordField", "Simulation", "SimulationBatch", "SimulationOptions",
+    "SimulationOutput", "timeReversal", "KWaveTransducer", "KWaveTransducerGeometry", "DelayMaskMode",
     "SourceMode", "flatNonzeroIndices", "kspaceFirstOrder", "db2neper", "gaussian", "getWin", "makeBall",
     "makeCartCircle", "makeCircle", "makeDisc", "neper2db", "random_field", "toneBurst",
 ]

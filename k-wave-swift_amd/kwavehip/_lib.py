@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK, EINVAL, ENOMEM, EDEVICE, ENONFINITE, EUNSUPPORTED, ESTATE = 0, -1, -2, -3, -4, -5, -6
 
@@ -23,7 +23,10 @@ EXPORTED = [
     "kwh_abi_version", "kwh_last_error", "kwh_sim_validate", "kwh_ctx_create", "kwh_ctx_destroy", "kwh_sim_create",
     "kwh_sim_run", "kwh_sim_launch", "kwh_sim_sync", "kwh_sim_output_shape", "kwh_sim_read_output",
     "kwh_sim_steps_done", "kwh_sim_state_bytes", "kwh_sim_save_state", "kwh_sim_load_state",
-    "kwh_sim_set_kernel_timing", "kwh_sim_set_kernel_timing_only", "kwh_sim_kernel_stats", "kwh_sim_set_graph", "kwh_sim_destroy",
+    "kwh_sim_set_kernel_timing", "kwh_sim_set_kernel_timing_only", "kwh_sim_kernel_stats", "kwh_sim_set_graph",
+    "kwh_sim_element_traces", "kwh_sim_autotune", "kwh_sim_set_checkpointing", "kwh_sim_vjp", "kwh_sim_destroy",
+    "kwh_batch_create", "kwh_batch_run", "kwh_batch_sim", "kwh_batch_read_output", "kwh_batch_destroy",
+    "kwh_time_reversal",
 ]
 
 
@@ -33,7 +36,8 @@ class Field(C.Structure):
 
 class SourceField(C.Structure):
     _fields_ = [("index", C.POINTER(C.c_int64)), ("n_points", C.c_int64), ("signal", C.POINTER(C.c_float)),
-                ("rows", C.c_int32), ("len", C.c_int32), ("mode", C.c_int32)]
+                ("rows", C.c_int32), ("len", C.c_int32), ("mode", C.c_int32),
+                ("delay", C.POINTER(C.c_int32)), ("scale", C.POINTER(C.c_float))]
 
 
 class SimDesc(C.Structure):
@@ -92,8 +96,20 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.kwh_sim_set_kernel_timing_only.argtypes = [vp, C.c_char_p]
     lib.kwh_sim_kernel_stats.argtypes = [vp, C.POINTER(KernelStat), C.c_int32, C.POINTER(C.c_int32)]
     lib.kwh_sim_set_graph.argtypes = [vp, C.c_int]
+    lib.kwh_sim_element_traces.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
+                                           C.c_double, vp, C.c_size_t]
+    lib.kwh_sim_autotune.argtypes = [vp, C.c_char_p, C.c_size_t]
+    lib.kwh_sim_set_checkpointing.argtypes = [vp, C.c_int32]
+    lib.kwh_sim_vjp.argtypes = [vp, vp, C.c_int32, vp, vp]
     lib.kwh_sim_destroy.argtypes = [vp]
     lib.kwh_sim_destroy.restype = None
+    lib.kwh_time_reversal.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(C.c_float), C.c_int32, C.c_double, vp]
+    lib.kwh_batch_create.argtypes = [vp, C.POINTER(SimDesc), C.c_int32, C.POINTER(vp)]
+    lib.kwh_batch_run.argtypes = [vp, C.c_int32]
+    lib.kwh_batch_sim.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    lib.kwh_batch_read_output.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_size_t]
+    lib.kwh_batch_destroy.argtypes = [vp]
+    lib.kwh_batch_destroy.restype = None
     if lib.kwh_abi_version() != ABI_VERSION:
         raise ImportError("libkwavehip ABI version mismatch")
     _lib = lib

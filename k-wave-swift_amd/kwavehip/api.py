@@ -175,6 +175,13 @@ class KWaveSource:
     uy: object = None
     uz: object = None
     uMode: SourceMode = SourceMode.additive
+    # Delayed copies (a transducer drive; not in the reference's KWaveSource): with a delay per
+    # mask point, p / ux / uy / uz is one row and point k plays scale[k] * row[t - delay[k]],
+    # expanded on the device (KWaveTransducer.velocity_source() produces these).
+    pDelay: object = None
+    pScale: object = None
+    uDelay: object = None
+    uScale: object = None
 
 
 @dataclass
@@ -263,19 +270,32 @@ def build_desc(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor
         keep.append(a)
         return L.Field(a.ctypes.data_as(C.POINTER(C.c_float)), a.size)
 
-    def src_field(mask, signal, mode):
+    def src_field(mask, signal, mode, delay=None, scale=None):
         if mask is None or signal is None:
-            return L.SourceField(None, 0, None, 0, 0, 0)
+            return L.SourceField(None, 0, None, 0, 0, 0, None, None)
         idx = flatNonzeroIndices(mask)
         sig = np.asarray(signal, dtype=np.float32)
         sig = sig.reshape(1, -1) if sig.ndim <= 1 else sig.reshape(sig.shape[0], -1)
         sig = np.ascontiguousarray(sig)
         keep.extend([idx, sig])
         if idx.size == 0:
-            return L.SourceField(None, 0, None, 0, 0, 0)
-        return L.SourceField(idx.ctypes.data_as(C.POINTER(C.c_int64)), idx.size,
-                             sig.ctypes.data_as(C.POINTER(C.c_float)), sig.shape[0], sig.shape[1],
-                             _SRC[SourceMode(mode)])
+            return L.SourceField(None, 0, None, 0, 0, 0, None, None)
+        f = L.SourceField(idx.ctypes.data_as(C.POINTER(C.c_int64)), idx.size,
+                          sig.ctypes.data_as(C.POINTER(C.c_float)), sig.shape[0], sig.shape[1],
+                          _SRC[SourceMode(mode)], None, None)
+        if delay is not None:
+            dl = np.ascontiguousarray(np.asarray(delay, dtype=np.int32).reshape(-1))
+            if dl.size != idx.size:
+                raise ValueError("source delays must have one entry per mask point")
+            keep.append(dl)
+            f.delay = dl.ctypes.data_as(C.POINTER(C.c_int32))
+            if scale is not None:
+                sc = np.ascontiguousarray(np.asarray(scale, dtype=np.float32).reshape(-1))
+                if sc.size != idx.size:
+                    raise ValueError("source scales must have one entry per mask point")
+                keep.append(sc)
+                f.scale = sc.ctypes.data_as(C.POINTER(C.c_float))
+        return f
 
     d = L.SimDesc()
     d.abi_version = L.ABI_VERSION
@@ -296,9 +316,9 @@ def build_desc(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor
     d.alpha_mode = _ALPHA[AbsorptionMode(medium.alphaMode)]
     d.bona = fld(medium.bOnA)
     d.p0 = fld(source.p0)
-    d.p_source = src_field(source.pMask, source.p, source.pMode)
+    d.p_source = src_field(source.pMask, source.p, source.pMode, source.pDelay, source.pScale)
     for a, s in enumerate([source.ux, source.uy, source.uz]):
-        d.u_source[a] = src_field(source.uMask, s, source.uMode)
+        d.u_source[a] = src_field(source.uMask, s, source.uMode, source.uDelay, source.uScale)
     n_sensor = 0
     if sensor is not None and sensor.mask is not None:
         m = np.asarray(sensor.mask)
@@ -347,9 +367,18 @@ class Simulation:
         self.handle = C.c_void_p()
         L.check(self.lib.kwh_sim_create(_context(device), C.byref(d), C.byref(self.handle)))
 
+    @classmethod
+    def _view(cls, lib, handle, grid, dtype, record, n_sensor):
+        """A non-owning view of a sim that belongs to a SimulationBatch."""
+        obj = cls.__new__(cls)
+        obj.lib, obj.handle, obj.grid, obj.dtype = lib, handle, grid, dtype
+        obj.record, obj.n_sensor, obj._keep, obj._owned = record, n_sensor, [], False
+        return obj
+
     def close(self):
         if getattr(self, "handle", None):
-            self.lib.kwh_sim_destroy(self.handle)
+            if getattr(self, "_owned", True):
+                self.lib.kwh_sim_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -431,7 +460,57 @@ class Simulation:
                  "bytes_per_launch": arr[i].bytes_per_launch} for i in range(cnt.value)]
 
     def set_graph(self, on: bool):
+        """Replay the time step from captured hipGraphs (bitwise equal to eager launches)."""
         L.check(self.lib.kwh_sim_set_graph(self.handle, 1 if on else 0))
+
+    def element_traces(self, field: int, element, delay, n_elements: int, scale: float) -> np.ndarray:
+        """Receive combination on the device (combineSensorData, KWaveTransducer.swift:263-289):
+        out[e, t] = scale * sum over sensor points k of element e of series[k, t + delay[k]]."""
+        el = np.ascontiguousarray(np.asarray(element, np.int32).reshape(-1))
+        dl = np.ascontiguousarray(np.asarray(delay, np.int32).reshape(-1))
+        if el.size != self.n_sensor or dl.size != self.n_sensor:
+            raise ValueError("element and delay need one entry per sensor point")
+        shape = (C.c_int64 * 3)()
+        nd = C.c_int32()
+        L.check(self.lib.kwh_sim_output_shape(self.handle, field, shape, C.byref(nd)))
+        out = np.empty((int(n_elements), int(shape[1])), dtype=self.dtype)
+        L.check(self.lib.kwh_sim_element_traces(self.handle, field, el.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                dl.ctypes.data_as(C.POINTER(C.c_int32)), int(n_elements),
+                                                float(scale), out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def autotune(self) -> str:
+        """Pick the fastest launch schedule for this grid (before the first step); returns the
+        candidates' measured step times."""
+        buf = C.create_string_buffer(4096)
+        L.check(self.lib.kwh_sim_autotune(self.handle, buf, len(buf)))
+        return buf.value.decode()
+
+    def set_checkpointing(self, every: int):
+        """Save the state every `every` steps during later runs, for vjp() (0 = off)."""
+        L.check(self.lib.kwh_sim_set_checkpointing(self.handle, int(every)))
+
+    def vjp(self, dL_dp: np.ndarray, checkpoint_every: int = 32):
+        """Gradient of a loss of the recorded pressure series y [nSensor, nt]: given dL/dy,
+        returns (dL/dp0, dL/dc) on the grid (lossless linear p0 problem; c_ref held fixed).
+        Runs the forward pass first unless the sim has just run all nt steps with checkpoints
+        every `checkpoint_every` steps; the reverse sweep recomputes between checkpoints."""
+        g = np.ascontiguousarray(np.asarray(dL_dp, dtype=self.dtype))
+        if g.shape != (self.n_sensor, self.grid.nt):
+            raise ValueError(f"dL/dp must be [nSensor, nt] = {(self.n_sensor, self.grid.nt)}")
+        shape = tuple(self.grid.size)
+        gp0 = np.empty(shape, dtype=self.dtype)
+        gc = np.empty(shape, dtype=self.dtype)
+        L.check(self.lib.kwh_sim_vjp(self.handle, g.ctypes.data_as(C.c_void_p), int(checkpoint_every),
+                                     gp0.ctypes.data_as(C.c_void_p), gc.ctypes.data_as(C.c_void_p)))
+        return gp0, gc
+
+    def transducer_traces(self, transducer, field: int = L.OUT_P) -> np.ndarray:
+        """Per-element receive traces of a sensor placed on the transducer's active voxels."""
+        element, delay = transducer.receive_combination()
+        g = transducer.geometry
+        return self.element_traces(field, element, delay, transducer.numberActiveElements,
+                                   1.0 / (g.elementWidth * g.elementLength))
 
     def outputs(self) -> SimulationOutput:
         """finalizeRecording (Recording.swift:129-168): only the recorded fields are set."""
@@ -447,6 +526,74 @@ class Simulation:
             if self.has_output(k):
                 setattr(out, nm, self.output(k))
         return out
+
+
+def timeReversal(grid: KWaveGrid, medium: KWaveMedium, sensor: KWaveSensor, recordedPressure,
+                 compensationFactor: float = 2.0, options: SimulationOptions = None, device: int = 0) -> np.ndarray:
+    """Time-reversal reconstruction of p0 (TimeReversal.swift:20-51): the recording re-injected,
+    reversed in time, as a Dirichlet pressure source on sensor.mask; returns
+    max(0, compensationFactor * pFinal). Flip and positivity run on the device."""
+    options = options or SimulationOptions()
+    if sensor is None or sensor.mask is None:
+        raise ValueError("time reversal requires sensor.mask")
+    if grid.nt <= 0:
+        raise ValueError("set grid time (makeTime/setTime) before time reversal")
+    rec = np.ascontiguousarray(np.asarray(recordedPressure, dtype=np.float32))
+    if rec.ndim != 2:
+        raise ValueError("recordedPressure must be [numSensorPoints, nt]")
+    d, keep, n_sensor = build_desc(grid, medium, KWaveSource(), KWaveSensor(mask=sensor.mask), options)
+    if rec.shape[0] != n_sensor:
+        raise ValueError("recordedPressure rows must match the sensor mask points")
+    lib = L.load()
+    out = np.empty(tuple(grid.size), dtype=np.float64 if options.dtype == DTypePrecision.float64 else np.float32)
+    L.check(lib.kwh_time_reversal(_context(device), C.byref(d), rec.ctypes.data_as(C.POINTER(C.c_float)),
+                                  rec.shape[1], float(compensationFactor), out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+class SimulationBatch:
+    """Independent simulations of uneven sizes advanced together on one device (kwh_batch_*):
+    per-scan-line transmits, for instance. `run(use_graph=True)` launches one hipGraph per time
+    step whose branches are the sims' steps; `sims[i]` reads outputs like a Simulation."""
+
+    def __init__(self, cases, device: int = 0):
+        self.lib = L.load()
+        n = len(cases)
+        if n == 0:
+            raise ValueError("empty batch")
+        descs = (L.SimDesc * n)()
+        self._keep = []
+        meta = []
+        for i, (grid, medium, source, sensor, options) in enumerate(cases):
+            options = options or SimulationOptions()
+            d, keep, n_sensor = build_desc(grid, medium, source, sensor, options)
+            descs[i] = d
+            self._keep.append(keep)
+            meta.append((grid, np.float64 if options.dtype == DTypePrecision.float64 else np.float32,
+                         RecordField(d.record), n_sensor))
+        self.handle = C.c_void_p()
+        L.check(self.lib.kwh_batch_create(_context(device), descs, n, C.byref(self.handle)))
+        self.sims = []
+        for i, (grid, dt, rec, ns) in enumerate(meta):
+            h = C.c_void_p()
+            L.check(self.lib.kwh_batch_sim(self.handle, i, C.byref(h)))
+            self.sims.append(Simulation._view(self.lib, h, grid, dt, rec, ns))
+
+    def run(self, use_graph: bool = True):
+        L.check(self.lib.kwh_batch_run(self.handle, 1 if use_graph else 0))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            for s in self.sims:
+                s.handle = None
+            self.lib.kwh_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def kspaceFirstOrder(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor: KWaveSensor,

@@ -61,7 +61,7 @@ private func field(_ a: MLXArray?, _ arena: HostArena) -> kwh_field {
 
 private func sourceField(mask: MLXArray?, signal: MLXArray?, mode: SourceMode, _ arena: HostArena) -> kwh_source_field {
     guard let mask, let signal else {
-        return kwh_source_field(index: nil, n_points: 0, signal: nil, rows: 0, len: 0, mode: 0)
+        return kwh_source_field(index: nil, n_points: 0, signal: nil, rows: 0, len: 0, mode: 0, delay: nil, scale: nil)
     }
     let idx = hostIndices(mask)
     let rows = signal.ndim <= 1 ? 1 : signal.shape[0]
@@ -73,17 +73,17 @@ private func sourceField(mask: MLXArray?, signal: MLXArray?, mode: SourceMode, _
     case .additiveNoCorrection: m = 2
     }
     return kwh_source_field(index: arena.copy(idx), n_points: Int64(idx.count), signal: arena.copy(sig),
-                            rows: Int32(rows), len: Int32(sig.count / max(rows, 1)), mode: m)
+                            rows: Int32(rows), len: Int32(sig.count / max(rows, 1)), mode: m, delay: nil, scale: nil)
 }
 
 private var sharedContext: OpaquePointer? = nil
 
-func kspaceFirstOrderHIP(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource,
-                         sensor: KWaveSensor, options: SimulationOptions) -> SimulationOutput {
+/// Flattens the API values into a descriptor whose pointers live in `arena`.
+private func makeDescriptor(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor: KWaveSensor,
+                            options: SimulationOptions, arena: HostArena) -> (kwh_sim_desc, Int) {
     precondition(grid.nt > 0, "call grid.makeTime(...) before running the solver")
     precondition(source.p0 != nil || source.pMask != nil || source.uMask != nil,
                  "\(grid.dim)D solver requires source.p0 or a time-varying p/u source")
-    let arena = HostArena()
     var d = kwh_sim_desc()
     d.abi_version = Int32(KWH_ABI_VERSION)
     d.dim = Int32(grid.dim)
@@ -133,10 +133,22 @@ func kspaceFirstOrderHIP(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSour
         }
     }
     d.record = UInt32(sensor.record.rawValue)
+    return (d, nSensor)
+}
 
+private func createSim(_ d: inout kwh_sim_desc) -> OpaquePointer? {
     if sharedContext == nil { check(kwh_ctx_create(0, &sharedContext), "kwh_ctx_create") }
     var sim: OpaquePointer? = nil
     check(kwh_sim_create(sharedContext, &d, &sim), "kwh_sim_create")
+    return sim
+}
+
+func kspaceFirstOrderHIP(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource,
+                         sensor: KWaveSensor, options: SimulationOptions) -> SimulationOutput {
+    let arena = HostArena()
+    var (d, nSensor) = makeDescriptor(grid: grid, medium: medium, source: source, sensor: sensor,
+                                      options: options, arena: arena)
+    let sim = createSim(&d)
     defer { kwh_sim_destroy(sim) }
 
     if options.progress == nil && options.fieldMonitor == nil {
@@ -187,4 +199,53 @@ func kspaceFirstOrderHIP(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSour
     out.uxFinal = read(KWH_OUT_UX_FINAL); out.uyFinal = read(KWH_OUT_UY_FINAL); out.uzFinal = read(KWH_OUT_UZ_FINAL)
     out.ixAvg = read(KWH_OUT_IX_AVG); out.iyAvg = read(KWH_OUT_IY_AVG); out.izAvg = read(KWH_OUT_IZ_AVG)
     return out
+}
+
+/// Forward / VJP pair with the contract of a torch.autograd.Function (no counterpart in the
+/// reference): `forward()` runs the loop saving checkpoints and returns the recorded pressure
+/// [nSensor, nt]; `vjp(dL/dp)` returns (dL/dp0, dL/dc) for the lossless linear p0 problem, at
+/// fixed c_ref = max(c). Float32 path (options.dtype .float32).
+public final class KSpaceFirstOrderHIPVJP {
+    private let arena = HostArena()
+    private let sim: OpaquePointer?
+    private let grid: KWaveGrid
+    private let nSensor: Int
+    private let checkpointEvery: Int32
+
+    public init(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor: KWaveSensor,
+                options: SimulationOptions, checkpointEvery: Int = 32) {
+        var (d, n) = makeDescriptor(grid: grid, medium: medium, source: source, sensor: sensor,
+                                    options: options, arena: arena)
+        precondition(d.dtype == 0, "the VJP pair runs in float32")
+        self.sim = createSim(&d)
+        self.grid = grid
+        self.nSensor = n
+        self.checkpointEvery = Int32(checkpointEvery)
+    }
+    deinit { kwh_sim_destroy(sim) }
+
+    public func forward() -> MLXArray {
+        check(kwh_sim_set_checkpointing(sim, checkpointEvery), "kwh_sim_set_checkpointing")
+        check(kwh_sim_run(sim, 0, Int32(grid.nt), nil, 0, nil), "kwh_sim_run")
+        var h = [Float](repeating: 0, count: nSensor * grid.nt)
+        h.withUnsafeMutableBytes { b in
+            check(kwh_sim_read_output(sim, Int32(KWH_OUT_P.rawValue), b.baseAddress, b.count), "read p")
+        }
+        return MLXArray(h).reshaped([nSensor, grid.nt])
+    }
+
+    public func vjp(_ dLdp: MLXArray) -> (p0: MLXArray, c: MLXArray) {
+        let g = dLdp.asType(.float32).reshaped([nSensor * grid.nt]).asArray(Float.self)
+        var gp0 = [Float](repeating: 0, count: grid.totalGridPoints)
+        var gc = [Float](repeating: 0, count: grid.totalGridPoints)
+        g.withUnsafeBytes { gb in
+            gp0.withUnsafeMutableBytes { pb in
+                gc.withUnsafeMutableBytes { cb in
+                    check(kwh_sim_vjp(sim, gb.baseAddress, checkpointEvery, pb.baseAddress, cb.baseAddress),
+                          "kwh_sim_vjp")
+                }
+            }
+        }
+        return (MLXArray(gp0).reshaped(grid.size), MLXArray(gc).reshaped(grid.size))
+    }
 }

@@ -17,7 +17,7 @@ from oracle import kwave_oracle as O
 from kwavehip import (AbsorptionMode, DirectivityPattern, DTypePrecision, KWaveGrid, KWaveMedium,
                       KWaveSensor, KWaveSource, RecordField, Simulation, SimulationOptions, SourceMode,
                       kspaceFirstOrder, makeBall, makeCartCircle, makeCircle, makeDisc, random_field,
-                      toneBurst)
+                      timeReversal, toneBurst)
 from kwavehip import _lib as L
 
 pytestmark = pytest.mark.gpu
@@ -478,6 +478,58 @@ def test_graph_replay_matches_eager_3d_velocity_source():
     eager = _run_chunks(make, [(0, 60, False)])
     graph = _run_chunks(make, [(0, 60, True)])
     _outputs_equal(eager, graph, ["p", "ux", "uz", "uxMax", "uzMax"])
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (256, 256), (32, 32, 32), (64, 64, 64)])
+def test_autotuned_schedule_is_bitwise_neutral(shape):
+    """The autotuner only changes launch geometry (tile columns, row pairs per workgroup, LDS
+    budget); every transform's arithmetic is unchanged, so results are bitwise those of the
+    default schedule."""
+    dim = len(shape)
+    args = [v for n in shape for v in (n, 1e-4)]
+    grid = KWaveGrid(*args)
+    grid.setTime(30, 2e-8)
+    p0 = makeDisc(*shape, radius=4) if dim == 2 else makeBall(*shape, radius=4)
+    mask = np.zeros(shape, np.float32)
+    mask[(shape[0] * 3) // 4] = 1
+    med = KWaveMedium(random_field(shape, 17, 4.0, 1450.0, 1600.0), 1000.0, alphaCoeff=0.5, alphaPower=1.3)
+    make = lambda: Simulation(grid, med, KWaveSource(p0=p0), KWaveSensor(mask=mask, record=RecordField.p),
+                              SimulationOptions(pmlSize=6))
+    print(f"\n[case] autotune {shape}", flush=True)
+    a = make()
+    a.run()
+    ref = a.outputs()
+    a.close()
+    b = make()
+    report = b.autotune()
+    print("  " + report.replace("\n", "\n  "), flush=True)
+    b.run()
+    out = b.outputs()
+    b.close()
+    assert np.array_equal(out.p, ref.p) and np.array_equal(out.pFinal, ref.pFinal)
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_time_reversal(dtype):
+    """timeReversal (TimeReversal.swift:20-51) on the device, flip and positivity included,
+    against the oracle's loop run with the same recording flipped on the host as a Dirichlet
+    source, then max(0, 2 pFinal)."""
+    n = 64
+    grid = KWaveGrid(n, 1e-4, n, 1e-4)
+    grid.makeTime(1500.0, cfl=0.3)
+    med = KWaveMedium(1500.0, 1000.0)
+    mask = makeCircle(n, n, radius=26)
+    opts = SimulationOptions(pmlSize=10)
+    rec = O.kspace_first_order(grid, med, KWaveSource(p0=makeDisc(n, n, radius=6)), KWaveSensor(mask=mask), opts)["p"]
+    rec = rec.astype(np.float32)
+    print(f"\n[case] time reversal {dtype.value} 64^2, {rec.shape[0]} sensor points x {rec.shape[1]} steps", flush=True)
+    est = timeReversal(grid, med, KWaveSensor(mask=mask), rec, 2.0, SimulationOptions(pmlSize=10, dtype=dtype))
+    src = KWaveSource(pMask=mask, p=np.ascontiguousarray(rec[:, ::-1]), pMode=SourceMode.dirichlet)
+    ref = np.maximum(2.0 * O.kspace_first_order(grid, med, src, KWaveSensor(), opts)["pFinal"], 0.0)
+    rl2, rmax = metrics(est, ref)
+    print(f"  relL2={rl2:.3e} relMax={rmax:.3e}", flush=True)
+    assert est.shape == (n, n) and np.all(est >= 0)
+    assert rl2 <= (1e-4 if dtype == F32 else 1e-9) and rmax <= (1e-3 if dtype == F32 else 1e-8)
 
 
 # ------------------------------------------------------------------------------------------ square-length fast path
