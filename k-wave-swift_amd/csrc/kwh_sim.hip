@@ -285,8 +285,9 @@ struct SimT final : kwh_sim {
     int sqR[3] = {0, 0, 0};   // R when the internal axis length is R*R with a register-resident kernel
     // launch schedule chosen by the autotuner: halve the tile columns of the strided / middle
     // passes (ntr_shift), halve the row pairs per workgroup of the row kernels (p_shift), LDS
-    // budget of the generic Stockham passes
-    struct Schedule { int ntr_shift = 0, p_shift = 0; size_t lds = 64 * 1024; };
+    // budget of the generic Stockham passes and the workgroup count below which they take fewer
+    // transforms per workgroup (min_blocks; 1024 = four per CU)
+    struct Schedule { int ntr_shift = 0, p_shift = 0; size_t lds = 64 * 1024; int min_blocks = 1024; };
     Schedule sched;
     bool use_sq = true;
 
@@ -348,6 +349,7 @@ struct SimT final : kwh_sim {
         const int npairs = (g.nrows + 1) / 2;
         int ntr = 32;
         while (ntr > 1 && ((size_t)nbuf * L * (ntr + 1) * sizeof(C) > sched.lds || ntr / 2 >= npairs)) ntr >>= 1;
+        while (ntr > 1 && (npairs + ntr - 1) / ntr < sched.min_blocks) ntr >>= 1;   // fill the CUs first
         ZCfg c{};
         c.zt.ntr = ntr;
         c.zt.lg_ntr = ilog2(ntr);
@@ -370,6 +372,8 @@ struct SimT final : kwh_sim {
         int ntr = 64;
         while (ntr > 1 && ((size_t)nbuf * L * ntr * sizeof(C) + 64 * sizeof(int64_t) > sched.lds || ntr / 2 >= c.ncols))
             ntr >>= 1;
+        // fill the CUs first, keeping at least 8 adjacent columns (64 B rows in float32) per tile
+        while (ntr > 8 && (c.ncols + ntr - 1) / ntr < sched.min_blocks) ntr >>= 1;
         c.lg_ntr = ilog2(ntr);
         c.pitch = ntr;
         c.blocks = (int)((c.ncols + ntr - 1) / ntr);
@@ -1624,11 +1628,12 @@ struct SimT final : kwh_sim {
         std::vector<Schedule> cands;
         for (int ns = 0; ns <= (sq_cols ? 1 : 0); ++ns)
             for (int ps = 0; ps <= (sqR[2] ? 1 : 0); ++ps)
-                for (int lb = 0; lb <= (generic ? 1 : 0); ++lb) {
+                for (int lb = 0; lb <= (generic ? 2 : 0); ++lb) {
                     Schedule c;
                     c.ntr_shift = ns;
                     c.p_shift = ps;
-                    c.lds = lb ? 32 * 1024 : 64 * 1024;
+                    c.lds = lb == 1 ? 32 * 1024 : 64 * 1024;
+                    c.min_blocks = lb == 2 ? 4096 : 1024;
                     cands.push_back(c);
                 }
         for (auto& sg : step_graphs) {
@@ -1655,8 +1660,8 @@ struct SimT final : kwh_sim {
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             ms /= 3;
             char line[160];
-            std::snprintf(line, sizeof(line), "ntr_shift=%d p_shift=%d lds=%zu: %.4f ms/step\n", c.ntr_shift, c.p_shift,
-                          c.lds, ms);
+            std::snprintf(line, sizeof(line), "ntr_shift=%d p_shift=%d lds=%zu min_blocks=%d: %.4f ms/step\n",
+                          c.ntr_shift, c.p_shift, c.lds, c.min_blocks, ms);
             report += line;
             if (ms < best) {
                 best = ms;
@@ -1671,8 +1676,8 @@ struct SimT final : kwh_sim {
         sync();
         cache.emplace_back(key, chosen);
         char line[160];
-        std::snprintf(line, sizeof(line), "chosen: ntr_shift=%d p_shift=%d lds=%zu\n", chosen.ntr_shift, chosen.p_shift,
-                      chosen.lds);
+        std::snprintf(line, sizeof(line), "chosen: ntr_shift=%d p_shift=%d lds=%zu min_blocks=%d\n", chosen.ntr_shift,
+                      chosen.p_shift, chosen.lds, chosen.min_blocks);
         return report + line;
     }
 
