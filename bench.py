@@ -143,8 +143,10 @@ def scan_line_cases(n_runs=48, nt=800, seed=7):
 
 
 def batch_comparison(n_runs=48, nt=800):
-    """Batched (one hipGraph per step over all runs) against the same runs launched one after
-    another (kwh_sim_run each), both timed from device-resident inputs to combined traces."""
+    """Batched runs against the same runs launched one after another (kwh_sim_run each), both
+    timed from device-resident inputs to combined traces. Batched modes: fused (one launch per
+    pass of the step over all runs, the batched step replayed from a hipGraph), graphs (each run
+    replays its own step graph on its own stream), streams (each run enqueued on its own stream)."""
     from kwavehip import Simulation, SimulationBatch
     cases = scan_line_cases(n_runs, nt)
     cells = sum(c[0][0].totalGridPoints for c in cases) * nt
@@ -160,10 +162,10 @@ def batch_comparison(n_runs=48, nt=800):
         sim.close()
     res = {"runs": n_runs, "steps": nt, "cells_total": int(cells // nt),
            "plain_Mcell_steps_per_s": round(cells / plain_s / 1e6, 1)}
-    for mode, use_graph in (("graphs", True), ("streams", False)):
+    for mode, use_graph, fused in (("fused", True, True), ("graphs", True, False), ("streams", False, False)):
         batch = SimulationBatch([c[0] for c in cases])
         t0 = time.perf_counter()
-        batch.run(use_graph=use_graph)
+        batch.run(use_graph=use_graph, fused=fused)
         traces = [sim.transducer_traces(tr) for sim, (_, tr) in zip(batch.sims, cases)]
         batch_s = time.perf_counter() - t0
         same = all(np.array_equal(a, b) for a, b in zip(traces, ref))
@@ -171,7 +173,7 @@ def batch_comparison(n_runs=48, nt=800):
         res[f"batched_{mode}_Mcell_steps_per_s"] = round(cells / batch_s / 1e6, 1)
         res[f"speedup_{mode}"] = round(plain_s / batch_s, 3)
         res[f"traces_identical_{mode}"] = bool(same)
-    res["speedup"] = res["speedup_graphs"]
+    res["speedup"] = res["speedup_fused"]
     return res
 
 

@@ -245,12 +245,20 @@ int kwh_sim_vjp(kwh_sim* sim, const void* dl_dp, int32_t checkpoint_every, void*
 
 /* Batched independent simulations of uneven sizes (per-scan-line transmits) on one device.
  * kwh_batch_create validates and sets up n sims (descs[i] as for kwh_sim_create).
- * kwh_batch_run advances every sim to its nt, each on its own stream: use_graph = 1 replays each
- * sim's captured step graph, interleaving the sims step by step; use_graph = 0 enqueues each sim's
- * steps eagerly. Outputs are read per sim (kwh_batch_read_output, or the borrowed
- * handle from kwh_batch_sim with the kwh_sim_* readers). */
+ * kwh_batch_run advances every sim to its nt. mode is a set of KWH_BATCH_* flags:
+ *   0                  each sim enqueues its steps eagerly on its own stream;
+ *   KWH_BATCH_GRAPH    each sim replays its captured step graph on its own stream, the sims
+ *                      interleaved step by step;
+ *   KWH_BATCH_FUSED    one launch per pass of the step covers every sim whose step has the same
+ *                      launch sequence and launch geometry (the sims of one grid shape); the
+ *                      shapes run concurrently, each on its own stream; with KWH_BATCH_GRAPH
+ *                      each shape's batched step is captured once and replayed.
+ * Every mode gives results bitwise equal to running the sims one by one. Outputs are read per sim
+ * (kwh_batch_read_output, or the borrowed handle from kwh_batch_sim with the kwh_sim_* readers). */
+#define KWH_BATCH_GRAPH 1
+#define KWH_BATCH_FUSED 2
 int kwh_batch_create(kwh_ctx* ctx, const kwh_sim_desc* descs, int32_t n, kwh_batch** out);
-int kwh_batch_run(kwh_batch* batch, int32_t use_graph);
+int kwh_batch_run(kwh_batch* batch, int32_t mode);
 int kwh_batch_sim(kwh_batch* batch, int32_t i, kwh_sim** out);
 int kwh_batch_read_output(kwh_batch* batch, int32_t i, int32_t field, void* dst, size_t dst_bytes);
 void kwh_batch_destroy(kwh_batch* batch);

@@ -55,6 +55,16 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
     return (bid % nx) * per + bid / nx;
 }
 
+// Block coordinates of one simulation's share of a launch. A single launch reads them from the
+// hardware and applies the XCD-aware tile order to x; a launch batched over several simulations
+// (kwh_batch.h) hands each its own block range, already in XCD order.
+struct BIdx {
+    int x, nx, y;
+    int remap;
+    __device__ __forceinline__ int tile() const { return remap ? xcd_tile(x, nx) : x; }
+};
+__device__ __forceinline__ BIdx bidx_here() { return BIdx{(int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y, 1}; }
+
 template <typename T> __device__ __forceinline__ T& comp(cx<T>& v, int part) {
     return reinterpret_cast<T*>(&v)[part];
 }
@@ -85,18 +95,18 @@ __device__ __forceinline__ void tile_colbase(int64_t* colbase, int64_t c0, int n
 }
 
 template <typename T, bool INV>
-__global__ void __launch_bounds__(256) strided_pass(StridedArgs a, FftPlan pl, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void strided_pass_body(const BIdx& bi, const StridedArgs& a, const FftPlan& pl, const cx<T>* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int ntr = 1 << a.lg_ntr;
     const int tid = threadIdx.x, nthr = blockDim.x;
     int64_t* colbase = reinterpret_cast<int64_t*>(smem);
     cx<T>* b0 = reinterpret_cast<cx<T>*>(smem + 64 * sizeof(int64_t));
     cx<T>* b1 = b0 + (size_t)a.L * a.pitch;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = bi.tile();
     const int64_t c0 = (int64_t)tile * ntr;
     tile_colbase<T>(colbase, c0, ntr, a.ncols, a.inner, a.L, tid);
     __syncthreads();
-    cx<T>* data = reinterpret_cast<cx<T>*>(a.fields[blockIdx.y]);
+    cx<T>* data = reinterpret_cast<cx<T>*>(a.fields[bi.y]);
     GlobAcc<T> g{data, colbase, a.stride};
     if (pl.nstages >= 2) {
         fft_run<T, INV>(pl, g, g, b0, b1, a.pitch, a.lg_ntr, tw, tid, nthr);
@@ -106,6 +116,10 @@ __global__ void __launch_bounds__(256) strided_pass(StridedArgs a, FftPlan pl, c
         __syncthreads();
         for (int i = tid; i < (a.L << a.lg_ntr); i += nthr) g.st(i & (ntr - 1), i >> a.lg_ntr, B.ld(i & (ntr - 1), i >> a.lg_ntr));
     }
+}
+template <typename T, bool INV>
+__global__ void __launch_bounds__(256) strided_pass(StridedArgs a, FftPlan pl, const cx<T>* __restrict__ tw) {
+    strided_pass_body<T, INV>(bidx_here(), a, pl, tw);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -176,7 +190,7 @@ __device__ __forceinline__ cx<T> mid_op_value(const MidOp& op, const AxisTabs<T>
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) middle_pass(MidArgs a, AxisTabs<T> tab, Geo g) {
+__device__ __forceinline__ void middle_pass_body(const BIdx& bi, const MidArgs& a, const AxisTabs<T>& tab, const Geo& g) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int ntr = 1 << a.lg_ntr;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -185,7 +199,7 @@ __global__ void __launch_bounds__(256) middle_pass(MidArgs a, AxisTabs<T> tab, G
     const size_t bl = (size_t)(a.L > 0 ? a.L : 1) * a.pitch;
     cx<T>* b1 = b0 + bl;
     cx<T>* X = b1 + bl;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = bi.tile();
     const int64_t c0 = (int64_t)tile * ntr;
     tile_colbase<T>(colbase, c0, ntr, a.ncols, a.inner, a.L, tid);
     __syncthreads();
@@ -228,6 +242,10 @@ __global__ void __launch_bounds__(256) middle_pass(MidArgs a, AxisTabs<T> tab, G
             __syncthreads();
         }
     }
+}
+template <typename T>
+__global__ void __launch_bounds__(256) middle_pass(MidArgs a, AxisTabs<T> tab, Geo g) {
+    middle_pass_body<T>(bidx_here(), a, tab, g);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -380,12 +398,12 @@ __device__ __forceinline__ bool sparse_in_tile(const int* rowptr, const Geo& g, 
 
 // Plain forward R2C along z of a real field (times `scale`) into `spec`.
 template <typename T>
-__global__ void __launch_bounds__(256) z_forward_real(const T* __restrict__ src, cx<T>* spec, T scale, Geo g,
-                                                      ZTile zt0, AxisTabs<T> tab) {
+__device__ __forceinline__ void z_forward_real_body(const BIdx& bi, const T* __restrict__ src, cx<T>* spec, T scale, const Geo& g,
+                                                      const ZTile& zt0, const AxisTabs<T>& tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     ZTile zt = zt0;
-    zt.q0 = xcd_tile(blockIdx.x, gridDim.x) * zt.ntr;
+    zt.q0 = bi.tile() * zt.ntr;
     cx<T>* b0 = reinterpret_cast<cx<T>*>(smem);
     cx<T>* b1 = b0 + (size_t)g.m2 * zt.pitch;
     KWH_CELL_LOOP(zt, g, tid, nthr, { comp(b0[n * zt.pitch + w], part) = src[cell] * scale; })
@@ -397,15 +415,20 @@ __global__ void __launch_bounds__(256) z_forward_real(const T* __restrict__ src,
     }
     z_forward<T>(spec, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);
 }
+template <typename T>
+__global__ void __launch_bounds__(256) z_forward_real(const T* __restrict__ src, cx<T>* spec, T scale, Geo g,
+                                                      ZTile zt0, AxisTabs<T> tab) {
+    z_forward_real_body<T>(bidx_here(), src, spec, scale, g, zt0, tab);
+}
 
 // Plain inverse C2R along z: dst = scale * C2R(spec) (accumulate adds to dst).
 template <typename T>
-__global__ void __launch_bounds__(256) z_inverse_real(const cx<T>* spec, T* __restrict__ dst, T scale, int accumulate,
-                                                      Geo g, ZTile zt0, AxisTabs<T> tab) {
+__device__ __forceinline__ void z_inverse_real_body(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst, T scale, int accumulate,
+                                                      const Geo& g, const ZTile& zt0, const AxisTabs<T>& tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     ZTile zt = zt0;
-    zt.q0 = xcd_tile(blockIdx.x, gridDim.x) * zt.ntr;
+    zt.q0 = bi.tile() * zt.ntr;
     cx<T>* b0 = reinterpret_cast<cx<T>*>(smem);
     cx<T>* b1 = b0 + (size_t)g.m2 * zt.pitch;
     cx<T>* R = z_inverse<T>(spec, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);
@@ -414,6 +437,11 @@ __global__ void __launch_bounds__(256) z_inverse_real(const cx<T>* spec, T* __re
         const T v = comp(R[n * zt.pitch + w], part) * scale;
         dst[cell] = accumulate ? dst[cell] + v : v;
     })
+}
+template <typename T>
+__global__ void __launch_bounds__(256) z_inverse_real(const cx<T>* spec, T* __restrict__ dst, T scale, int accumulate,
+                                                      Geo g, ZTile zt0, AxisTabs<T> tab) {
+    z_inverse_real_body<T>(bidx_here(), spec, dst, scale, accumulate, g, zt0, tab);
 }
 
 // Velocity update: u_a = pml_sg_a (pml_sg_a u_a - dt/rho_a * g_a) [+ sources]; then U_a = R2C_z(u_a)
@@ -428,11 +456,11 @@ template <typename T> struct VelArgs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) z_velocity(VelArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {
+__device__ __forceinline__ void z_velocity_body(const BIdx& bi, const VelArgs<T>& a, const Geo& g, const ZTile& zt0, const AxisTabs<T>& tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     ZTile zt = zt0;
-    zt.q0 = xcd_tile(blockIdx.x, gridDim.x) * zt.ntr;
+    zt.q0 = bi.tile() * zt.ntr;
     cx<T>* b0 = reinterpret_cast<cx<T>*>(smem);
     cx<T>* b1 = b0 + (size_t)g.m2 * zt.pitch;
     for (int ax = 0; ax < g.dim; ++ax) {
@@ -463,6 +491,10 @@ __global__ void __launch_bounds__(256) z_velocity(VelArgs<T> a, Geo g, ZTile zt0
         __syncthreads();
     }
 }
+template <typename T>
+__global__ void __launch_bounds__(256) z_velocity(VelArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {
+    z_velocity_body<T>(bidx_here(), a, g, zt0, tab);
+}
 
 // Density update + EOS (KSpaceFirstOrder.swift:680-702, sources :716-720).
 // mode 0: p = c^2 (rhoT [+ stokes] [+ B/A]) written, and P = R2C_z(p) stored in outP.
@@ -488,17 +520,20 @@ template <typename T> struct RhoArgs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) z_density(RhoArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {
+__device__ __forceinline__ void z_density_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, const ZTile& zt0, const AxisTabs<T>& tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     ZTile zt = zt0;
-    zt.q0 = xcd_tile(blockIdx.x, gridDim.x) * zt.ntr;
+    zt.q0 = bi.tile() * zt.ntr;
     const size_t bl = (size_t)g.m2 * zt.pitch;
     cx<T>* b0 = reinterpret_cast<cx<T>*>(smem);
     cx<T>* b1 = b0 + bl;
-    cx<T>* DIV = b1 + bl;   // sum of du (a/b rows in .x/.y)
-    cx<T>* RT = DIV + bl;   // new total density
-    cx<T>* RP = RT + bl;    // previous total density (nonlinear only)
+    // LDS: b0, b1, RT = new total density, DIV = sum of du (Stokes / absorption inputs only),
+    // RP = previous total density (nonlinear only); a/b rows in .x/.y
+    const bool need_div = a.mode != 0 || a.stokes;
+    cx<T>* RT = b1 + bl;
+    cx<T>* DIV = need_div ? RT + bl : nullptr;
+    cx<T>* RP = RT + (need_div ? 2 : 1) * bl;
     if (a.nonlinear) {
         KWH_CELL_LOOP(zt, g, tid, nthr, {
             T s = 0;
@@ -525,10 +560,12 @@ __global__ void __launch_bounds__(256) z_density(RhoArgs<T> a, Geo g, ZTile zt0,
             v += a.corr.at(cell) * cf;
             s = v;
             if (!sparse) rho[cell] = v;
-            T& dv = comp(DIV[n * zt.pitch + w], part);
+            if (need_div) {
+                T& dv = comp(DIV[n * zt.pitch + w], part);
+                dv = ax == 0 ? du : dv + du;
+            }
             T& rt = comp(RT[n * zt.pitch + w], part);
-            if (ax == 0) { dv = du; if (!sparse) rt = v; }
-            else { dv += du; if (!sparse) rt += v; }
+            if (!sparse) rt = ax == 0 ? v : rt + v;
         })
         if (sparse) {
             __syncthreads();
@@ -570,6 +607,10 @@ __global__ void __launch_bounds__(256) z_density(RhoArgs<T> a, Geo g, ZTile zt0,
         }
     }
 }
+template <typename T>
+__global__ void __launch_bounds__(256) z_density(RhoArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {
+    z_density_body<T>(bidx_here(), a, g, zt0, tab);
+}
 
 // EOS with power-law absorption/dispersion (KSpaceFirstOrder.swift:836-845, :694-702):
 // p = c^2 (rhoT + tau * A1 - eta * A2 + B/A rhoT^2 / (2 rho0)); P = R2C_z(p).
@@ -601,11 +642,11 @@ __device__ __forceinline__ void eos_tau_eta(const EosArgs<T>& a, int64_t cell, T
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) z_eos(EosArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {
+__device__ __forceinline__ void z_eos_body(const BIdx& bi, const EosArgs<T>& a, const Geo& g, const ZTile& zt0, const AxisTabs<T>& tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     ZTile zt = zt0;
-    zt.q0 = xcd_tile(blockIdx.x, gridDim.x) * zt.ntr;
+    zt.q0 = bi.tile() * zt.ntr;
     const size_t bl = (size_t)g.m2 * zt.pitch;
     cx<T>* b0 = reinterpret_cast<cx<T>*>(smem);
     cx<T>* b1 = b0 + bl;
@@ -643,6 +684,10 @@ __global__ void __launch_bounds__(256) z_eos(EosArgs<T> a, Geo g, ZTile zt0, Axi
     })
     z_forward<T>(a.outP, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);
 }
+template <typename T>
+__global__ void __launch_bounds__(256) z_eos(EosArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {
+    z_eos_body<T>(bidx_here(), a, g, zt0, tab);
+}
 
 // t = 0 initial-pressure override (KSpaceFirstOrder.swift:724-733):
 // p = p0; rho_a = p0 / (dim c^2); u_a = dt/rho_a / 2 * Re ifft(ddxPos_a kappa fft p0); P = R2C_z(p0).
@@ -658,11 +703,11 @@ template <typename T> struct InitArgs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) z_init(InitArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {
+__device__ __forceinline__ void z_init_body(const BIdx& bi, const InitArgs<T>& a, const Geo& g, const ZTile& zt0, const AxisTabs<T>& tab) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     ZTile zt = zt0;
-    zt.q0 = xcd_tile(blockIdx.x, gridDim.x) * zt.ntr;
+    zt.q0 = bi.tile() * zt.ntr;
     cx<T>* b0 = reinterpret_cast<cx<T>*>(smem);
     cx<T>* b1 = b0 + (size_t)g.m2 * zt.pitch;
     for (int ax = 0; ax < g.dim; ++ax) {
@@ -690,15 +735,23 @@ __global__ void __launch_bounds__(256) z_init(InitArgs<T> a, Geo g, ZTile zt0, A
     }
     z_forward<T>(a.outP, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);
 }
+template <typename T>
+__global__ void __launch_bounds__(256) z_init(InitArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {
+    z_init_body<T>(bidx_here(), a, g, zt0, tab);
+}
 
 // ---------------------------------------------------------------------------------------------
 // Elementwise / utility kernels
 // ---------------------------------------------------------------------------------------------
 
 template <typename T>
-__global__ void fill_kernel(T* p, T v, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void fill_kernel_body(const BIdx& bi, T* p, T v, int64_t n) {
+    for (int64_t i = bi.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)bi.nx * blockDim.x)
         p[i] = v;
+}
+template <typename T>
+__global__ void fill_kernel(T* p, T v, int64_t n) {
+    fill_kernel_body<T>(bidx_here(), p, v, n);
 }
 
 template <typename S, typename D>
@@ -725,12 +778,17 @@ __global__ void maxabs_kernel(const T* x, int64_t n, unsigned long long* out) {
 
 // Scatter scaled source values into a zeroed real field (k-space-corrected additive sources).
 template <typename T>
-__global__ void scatter_kernel(T* dst, const int64_t* index, const T* signal, const T* base, int rows, int len, int n,
+__device__ __forceinline__ void scatter_kernel_body(const BIdx& bi, T* dst, const int64_t* index, const T* signal, const T* base, int rows, int len, int n,
                                const int* ctr, T factor) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = bi.x * blockDim.x + threadIdx.x;
     const int t = ctr[0];
     if (e >= n || t >= len) return;
     dst[index[e]] = signal[(int64_t)(rows == 1 ? 0 : e) * len + t] * base[e] * factor;
+}
+template <typename T>
+__global__ void scatter_kernel(T* dst, const int64_t* index, const T* signal, const T* base, int rows, int len, int n,
+                               const int* ctr, T factor) {
+    scatter_kernel_body<T>(bidx_here(), dst, index, signal, base, rows, len, n, ctr, factor);
 }
 
 // End of a time step: t += 1 and, when the step recorded, slot += 1. Kernels read both from
@@ -742,11 +800,14 @@ __global__ void set_counters_kernel(int* ctr, int t, int slot) {
     }
 }
 
-__global__ void advance_kernel(int* ctr, int recorded) {
+__device__ __forceinline__ void advance_kernel_body(const BIdx& bi, int* ctr, int recorded) {
     if (threadIdx.x == 0) {
         ctr[0] += 1;
         ctr[1] += recorded;
     }
+}
+__global__ void advance_kernel(int* ctr, int recorded) {
+    advance_kernel_body(bidx_here(), ctr, recorded);
 }
 
 // Sensor gather + streaming aggregates (Recording.swift:37-45, :139-156).
@@ -763,17 +824,17 @@ template <typename T> struct GatherArgs {
     const int* ctr;        // device step counters {t, record slot}; slot 0 initialises max/min
 };
 
-// Up to four fields (p, ux, uy, uz) sampled by one launch: blockIdx.y selects the field.
+// Up to four fields (p, ux, uy, uz) sampled by one launch: block y selects the field.
 template <typename T> struct GatherSet {
     GatherArgs<T> f[4];
 };
 
 template <typename T>
-__global__ void gather_kernel(GatherSet<T> set) {
-    const GatherArgs<T>& a = set.f[blockIdx.y];
+__device__ __forceinline__ void gather_kernel_body(const BIdx& bi, const GatherSet<T>& set) {
+    const GatherArgs<T>& a = set.f[bi.y];
     const int slot = a.ctr[1];
     const bool first = slot == 0;
-    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t s = bi.x * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)bi.nx * blockDim.x) {
         T v;
         if (a.weight) {
             v = T(0);
@@ -787,14 +848,18 @@ __global__ void gather_kernel(GatherSet<T> set) {
         if (a.sum2) a.sum2[s] = (first ? 0.0 : a.sum2[s]) + (double)v * (double)v;
     }
 }
+template <typename T>
+__global__ void gather_kernel(GatherSet<T> set) {
+    gather_kernel_body<T>(bidx_here(), set);
+}
 
 // Gather through output positions (a subset of sensor rows, e.g. one directivity angle).
 template <typename T>
-__global__ void gather_pos_kernel(const T* field, const int64_t* idx, const int64_t* pos, int64_t n, int64_t nsens,
+__device__ __forceinline__ void gather_pos_kernel_body(const BIdx& bi, const T* field, const int64_t* idx, const int64_t* pos, int64_t n, int64_t nsens,
                                   T* series, T* vmax, T* vmin, double* sum2, const int* ctr) {
     const int t = ctr[1];
     const bool first = t == 0;
-    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t s = bi.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)bi.nx * blockDim.x) {
         const T v = field[idx[s]];
         const int64_t q = pos[s];
         if (series) series[(int64_t)t * nsens + q] = v;
@@ -802,6 +867,11 @@ __global__ void gather_pos_kernel(const T* field, const int64_t* idx, const int6
         if (vmin) vmin[q] = first ? v : fmin(vmin[q], v);
         if (sum2) sum2[q] = (first ? 0.0 : sum2[q]) + (double)v * (double)v;
     }
+}
+template <typename T>
+__global__ void gather_pos_kernel(const T* field, const int64_t* idx, const int64_t* pos, int64_t n, int64_t nsens,
+                                  T* series, T* vmax, T* vmin, double* sum2, const int* ctr) {
+    gather_pos_kernel_body<T>(bidx_here(), field, idx, pos, n, nsens, series, vmax, vmin, sum2, ctr);
 }
 
 // pRms / uRms = sqrt(sum2 / nt).

@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <list>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -21,6 +23,7 @@
 #include "kwh_setup.h"
 #include "kwh_sq.h"
 #include "kwh_adjoint.h"
+#include "kwh_batch.h"
 
 using namespace kwh;
 
@@ -139,6 +142,34 @@ struct kwh_ctx {
     int device;
 };
 
+// One recorded launch of a step, for launches batched over several simulations: the batched
+// kernel, the simulation's own grid / workgroup / LDS and its packed arguments (a BatchPack).
+struct LaunchRec {
+    const void* fn;
+    unsigned gx, gy, block;
+    size_t lds;
+    bool flex;                          // body runs with any workgroup size
+    std::vector<unsigned char> pack;
+};
+
+template <class B, class... A>
+LaunchRec make_rec(dim3 grid, dim3 block, size_t lds, const A&... a) {
+    using P = BatchPack<A...>;
+    P pk{0, (int)grid.x, 0, 0, std::tuple<A...>(a...)};
+    LaunchRec r{(const void*)&batched_kernel<B, A...>, grid.x, grid.y, block.x, lds, B::kFlex, {}};
+    r.pack.resize(sizeof(P));
+    std::memcpy(r.pack.data(), &pk, sizeof(P));
+    return r;
+}
+
+// The launches of one step of one sim for one step key, with a hash of their kernels and
+// fixed workgroup sizes: sims whose hashes match can share batched launches.
+struct StepRecord {
+    unsigned key = 0;
+    uint64_t signature = 0;
+    std::vector<LaunchRec> launches;
+};
+
 // Abstract sim: the typed implementation follows.
 struct kwh_sim {
     virtual ~kwh_sim() = default;
@@ -167,6 +198,8 @@ struct kwh_sim {
     virtual void set_counters_on(hipStream_t s) = 0;
     virtual void graph_step(int t) = 0;   // replay step t's graph on the sim's own stream
     virtual int device() const = 0;
+    virtual const StepRecord& recorded_step(int t) = 0;   // step t's launches, recorded once per step key
+    virtual uint64_t shape_class() const = 0;            // grid shape and precision
 };
 
 namespace {
@@ -317,7 +350,7 @@ struct SimT final : kwh_sim {
     }
     template <typename F>
     void timed(int id, double bytes, F&& f) {
-        if (!timing || (timing_only >= 0 && id != timing_only)) {
+        if (!timing || recording || (timing_only >= 0 && id != timing_only)) {
             f();
             HIPCHK(hipGetLastError());
             return;
@@ -340,6 +373,19 @@ struct SimT final : kwh_sim {
             ev_pool.push_back(pd.b);
         }
         pending.clear();
+    }
+
+    // ------------------------------------------------------------------ launches
+    // Every launch of the step goes through emit: launched on the sim's stream, or, while a
+    // batched run records this sim's step, appended to `recording`.
+    std::vector<LaunchRec>* recording = nullptr;
+    template <class B, class... A>
+    void emit(dim3 grid, dim3 block, size_t lds, const A&... a) {
+        if (recording) {
+            recording->push_back(make_rec<B>(grid, block, lds, a...));
+            return;
+        }
+        hipLaunchKernelGGL(B::kernel, grid, block, lds, stream, a...);
     }
 
     // ------------------------------------------------------------------ launch geometry
@@ -418,8 +464,8 @@ struct SimT final : kwh_sim {
             const size_t lds = ((size_t)c.L * ntr + R + R * R) * sizeof(C);   // slab + twiddle table
             timed(inv ? K_SI : K_SF, bytes, [&] {
 #define KWH_SQS(RR, NN)                                                                                     \
-    if (inv) hipLaunchKernelGGL((sq_strided<T, RR, NN, true>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]); \
-    else hipLaunchKernelGGL((sq_strided<T, RR, NN, false>), grid, dim3(ntr * R), lds, stream, a, tab.tw[axis]);
+    if (inv) emit<SqStridedB<T, RR, NN, true>>(grid, dim3(ntr * R), lds, a, tab.tw[axis]); \
+    else emit<SqStridedB<T, RR, NN, false>>(grid, dim3(ntr * R), lds, a, tab.tw[axis]);
                 if (R == 16) { if (ntr == 16) { KWH_SQS(16, 16) } else { KWH_SQS(16, 8) } }
                 else { if (ntr == 32) { KWH_SQS(8, 32) } else { KWH_SQS(8, 16) } }
 #undef KWH_SQS
@@ -431,9 +477,9 @@ struct SimT final : kwh_sim {
         dim3 grid(c.blocks, nf);
         timed(inv ? K_SI : K_SF, bytes, [&] {
             if (inv)
-                hipLaunchKernelGGL((strided_pass<T, true>), grid, dim3(c.threads), c.lds, stream, a, tab.plan[axis], tab.tw[axis]);
+                emit<StridedB<T, true>>(grid, dim3(c.threads), c.lds, a, tab.plan[axis], tab.tw[axis]);
             else
-                hipLaunchKernelGGL((strided_pass<T, false>), grid, dim3(c.threads), c.lds, stream, a, tab.plan[axis], tab.tw[axis]);
+                emit<StridedB<T, false>>(grid, dim3(c.threads), c.lds, a, tab.plan[axis], tab.tw[axis]);
         });
     }
 
@@ -476,11 +522,11 @@ struct SimT final : kwh_sim {
             const unsigned blocks = (unsigned)((c.ncols + ntr - 1) / ntr);
             timed(kid, bytes, [&] {
                 if (R == 16) {
-                    if (ntr == 16) hipLaunchKernelGGL((sq_middle<T, 16, 16>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
-                    else hipLaunchKernelGGL((sq_middle<T, 16, 8>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
+                    if (ntr == 16) emit<SqMiddleB<T, 16, 16>>(dim3(blocks), dim3(ntr * R), lds, a, tab, g);
+                    else emit<SqMiddleB<T, 16, 8>>(dim3(blocks), dim3(ntr * R), lds, a, tab, g);
                 } else {
-                    if (ntr == 32) hipLaunchKernelGGL((sq_middle<T, 8, 32>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
-                    else hipLaunchKernelGGL((sq_middle<T, 8, 16>), dim3(blocks), dim3(ntr * R), lds, stream, a, tab, g);
+                    if (ntr == 32) emit<SqMiddleB<T, 8, 32>>(dim3(blocks), dim3(ntr * R), lds, a, tab, g);
+                    else emit<SqMiddleB<T, 8, 16>>(dim3(blocks), dim3(ntr * R), lds, a, tab, g);
                 }
             });
             return;
@@ -488,7 +534,7 @@ struct SimT final : kwh_sim {
         a.lg_ntr = c.lg_ntr;
         a.pitch = c.pitch;
         timed(kid, bytes, [&] {
-            hipLaunchKernelGGL((middle_pass<T>), dim3(c.blocks), dim3(c.threads), c.lds, stream, a, tab, g);
+            emit<MiddleB<T>>(dim3(c.blocks), dim3(c.threads), c.lds, a, tab, g);
         });
     }
 
@@ -508,12 +554,12 @@ struct SimT final : kwh_sim {
         z.lds = (size_t)z.P * per_pair + twb;
         return z;
     }
-    template <typename K, typename... Args>
-    void zlaunch(int kid, double bytes, K kernel16, K kernel8, Args... args) {
+    template <class B16, class B8, typename... Args>
+    void zlaunch(int kid, double bytes, Args... args) {
         SqZ z = sqz();
         timed(kid, bytes, [&] {
-            if (sqR[2] == 16) hipLaunchKernelGGL(kernel16, dim3(z.blocks), dim3(z.threads), z.lds, stream, args..., z.P, tab);
-            else hipLaunchKernelGGL(kernel8, dim3(z.blocks), dim3(z.threads), z.lds, stream, args..., z.P, tab);
+            if (sqR[2] == 16) emit<B16>(dim3(z.blocks), dim3(z.threads), z.lds, args..., z.P, tab);
+            else emit<B8>(dim3(z.blocks), dim3(z.threads), z.lds, args..., z.P, tab);
         });
     }
 
@@ -521,13 +567,13 @@ struct SimT final : kwh_sim {
     // spectrum ready for a middle pass.
     void forward_to_middle(const T* src, C* out, T scale) {
         if (sqR[2]) {
-            zlaunch(K_ZFR, real_bytes() + spec_bytes(), sq_z_forward_real<T, 16>, sq_z_forward_real<T, 8>, src, out, scale, g);
+            zlaunch<SqZForwardRealB<T, 16>, SqZForwardRealB<T, 8>>(K_ZFR, real_bytes() + spec_bytes(), src, out, scale, g);
             if (plain1) strided(1, false, {out});
             return;
         }
         ZCfg zc = zcfg(2);
         timed(K_ZFR, real_bytes() + spec_bytes(), [&] {
-            hipLaunchKernelGGL((z_forward_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, src, out, scale, g,
+            emit<ZForwardRealB<T>>(dim3(zc.blocks), dim3(zc.threads), zc.lds, src, out, scale, g,
                                zc.zt, tab);
         });
         if (plain1) strided(1, false, {out});
@@ -535,20 +581,20 @@ struct SimT final : kwh_sim {
     void inverse_from_middle(C* in, T* dst, T scale, bool accumulate) {
         if (plain1) strided(1, true, {in});
         if (sqR[2]) {
-            zlaunch(K_ZIR, real_bytes() + spec_bytes(), sq_z_inverse_real<T, 16>, sq_z_inverse_real<T, 8>,
+            zlaunch<SqZInverseRealB<T, 16>, SqZInverseRealB<T, 8>>(K_ZIR, real_bytes() + spec_bytes(),
                     (const C*)in, dst, scale, accumulate ? 1 : 0, g);
             return;
         }
         ZCfg zc = zcfg(2);
         timed(K_ZIR, real_bytes() + spec_bytes(), [&] {
-            hipLaunchKernelGGL((z_inverse_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, in, dst, scale,
+            emit<ZInverseRealB<T>>(dim3(zc.blocks), dim3(zc.threads), zc.lds, in, dst, scale,
                                accumulate ? 1 : 0, g, zc.zt, tab);
         });
     }
 
     template <typename U> void fill(U* ptr, U v, int64_t n) {
         timed(K_FI, (double)n * sizeof(U), [&] {
-            hipLaunchKernelGGL((fill_kernel<U>), dim3(grid_blocks(n)), dim3(256), 0, stream, ptr, v, n);
+            emit<FillB<U>>(dim3(grid_blocks(n)), dim3(256), 0, ptr, v, n);
         });
     }
 
@@ -901,7 +947,7 @@ struct SimT final : kwh_sim {
             int thr = std::max(64, std::min(256, ((ntr * L / 8 + 63) / 64) * 64));
             const size_t lds = (size_t)2 * L * zt.pitch * cs;
             const int blocks = (npairs + ntr - 1) / ntr;
-            hipLaunchKernelGGL((z_forward_real<D>), dim3(blocks), dim3(thr), lds, stream, x, s, 1.0, g, zt, tab64);
+            emit<ZForwardRealB<D>>(dim3(blocks), dim3(thr), lds, x, s, 1.0, g, zt, tab64);
             HIPCHK(hipGetLastError());
             auto strided64 = [&](int axis, bool inv) {
                 const int La = axis == 0 ? g.m0 : g.m1;
@@ -920,8 +966,8 @@ struct SimT final : kwh_sim {
                 const int th = std::max(64, std::min(256, ((nt_ * La / 8 + 63) / 64) * 64));
                 const size_t l = (size_t)2 * La * nt_ * cs + 64 * 8;
                 const int bl = (int)((a.ncols + nt_ - 1) / nt_);
-                if (inv) hipLaunchKernelGGL((strided_pass<D, true>), dim3(bl, 1), dim3(th), l, stream, a, tab64.plan[axis], tab64.tw[axis]);
-                else hipLaunchKernelGGL((strided_pass<D, false>), dim3(bl, 1), dim3(th), l, stream, a, tab64.plan[axis], tab64.tw[axis]);
+                if (inv) emit<StridedB<D, true>>(dim3(bl, 1), dim3(th), l, a, tab64.plan[axis], tab64.tw[axis]);
+                else emit<StridedB<D, false>>(dim3(bl, 1), dim3(th), l, a, tab64.plan[axis], tab64.tw[axis]);
                 HIPCHK(hipGetLastError());
             };
             if (plain1) strided64(1, false);
@@ -946,11 +992,11 @@ struct SimT final : kwh_sim {
                 const int th = std::max(64, std::min(256, ((nt_ * a.L / 8 + 63) / 64) * 64));
                 const size_t l = (size_t)3 * a.L * nt_ * cs + 64 * 8;
                 const int bl = (int)((a.ncols + nt_ - 1) / nt_);
-                hipLaunchKernelGGL((middle_pass<D>), dim3(bl), dim3(th), l, stream, a, tab64, g);
+                emit<MiddleB<D>>(dim3(bl), dim3(th), l, a, tab64, g);
                 HIPCHK(hipGetLastError());
             }
             if (plain1) strided64(1, true);
-            hipLaunchKernelGGL((z_inverse_real<D>), dim3(blocks), dim3(thr), lds, stream, s, x, 1.0, 0, g, zt, tab64);
+            emit<ZInverseRealB<D>>(dim3(blocks), dim3(thr), lds, s, x, 1.0, 0, g, zt, tab64);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(stream));
         }
@@ -1230,7 +1276,7 @@ struct SimT final : kwh_sim {
         // Re ifft(source_kappa * fft(scatter(v))) (SourceOp.apply :167-171)
         fill<T>(scr_real.as<T>(), T(0), g.N);
         timed(K_SC, 0, [&] {
-            hipLaunchKernelGGL((scatter_kernel<T>), dim3(grid_blocks(s.n)), dim3(256), 0, stream, scr_real.as<T>(),
+            emit<ScatterB<T>>(dim3(grid_blocks(s.n)), dim3(256), 0, scr_real.as<T>(),
                                s.index.template as<int64_t>(), s.signal.template as<T>(), s.base.template as<T>(), s.rows, s.len, (int)s.n,
                                (const int*)ctr.p, T(factor));
         });
@@ -1263,7 +1309,7 @@ struct SimT final : kwh_sim {
         ia.outP = spec[0].as<C>();
         ZCfg zc = zcfg(2);
         timed(K_ZI, real_bytes() * (1 + 3 * dim) + spec_bytes() * (dim + 1), [&] {
-            hipLaunchKernelGGL((z_init<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ia, g, zc.zt, tab);
+            emit<ZInitB<T>>(dim3(zc.blocks), dim3(zc.threads), zc.lds, ia, g, zc.zt, tab);
         });
         // collocated velocity of the initial state for the t = 0 sample (:735-745)
         for (int a = 0; a < dim; ++a) {
@@ -1315,12 +1361,12 @@ struct SimT final : kwh_sim {
             bool anycorr = false;
             for (int a = 0; a < dim; ++a) anycorr |= cu[a] != nullptr;
             if (sqR[2]) {
-                if (anycorr) zlaunch(K_ZV, bytes, sq_z_velocity<T, 16, true>, sq_z_velocity<T, 8, true>, va, g);
-                else zlaunch(K_ZV, bytes, sq_z_velocity<T, 16, false>, sq_z_velocity<T, 8, false>, va, g);
+                if (anycorr) zlaunch<SqZVelocityB<T, 16, true>, SqZVelocityB<T, 8, true>>(K_ZV, bytes, va, g);
+                else zlaunch<SqZVelocityB<T, 16, false>, SqZVelocityB<T, 8, false>>(K_ZV, bytes, va, g);
             } else {
                 ZCfg zc = zcfg(2);
                 timed(K_ZV, bytes, [&] {
-                    hipLaunchKernelGGL((z_velocity<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, va, g, zc.zt, tab);
+                    emit<ZVelocityB<T>>(dim3(zc.blocks), dim3(zc.threads), zc.lds, va, g, zc.zt, tab);
                 });
             }
         }
@@ -1334,10 +1380,10 @@ struct SimT final : kwh_sim {
             T* dst = ucol[dim - 1].as<T>();
             timed(K_ZCR, (double)nzrows * g.m2 * (sizeof(T) + sizeof(C) / 2), [&] {
                 if (sqR[2] == 16)
-                    hipLaunchKernelGGL((sq_z_colloc_rows<T, 16>), dim3(blocks), dim3(z.threads), z.lds, stream, s, dst,
+                    emit<SqZCollocRowsB<T, 16>>(dim3(blocks), dim3(z.threads), z.lds, s, dst,
                                        (const int*)zrows.p, nzrows, g, z.P, tab);
                 else
-                    hipLaunchKernelGGL((sq_z_colloc_rows<T, 8>), dim3(blocks), dim3(z.threads), z.lds, stream, s, dst,
+                    emit<SqZCollocRowsB<T, 8>>(dim3(blocks), dim3(z.threads), z.lds, s, dst,
                                        (const int*)zrows.p, nzrows, g, z.P, tab);
             });
         }
@@ -1364,13 +1410,13 @@ struct SimT final : kwh_sim {
                 C* s = spec[ucol_slot[a]].as<C>();
                 T* dst = ucol[a].as<T>();
                 if (sqR[2]) {
-                    zlaunch(K_ZIR, real_bytes() + spec_bytes(), sq_z_inverse_real<T, 16>, sq_z_inverse_real<T, 8>,
+                    zlaunch<SqZInverseRealB<T, 16>, SqZInverseRealB<T, 8>>(K_ZIR, real_bytes() + spec_bytes(),
                             (const C*)s, dst, T(1), 0, g);
                     continue;
                 }
                 ZCfg zc = zcfg(2);
                 timed(K_ZIR, real_bytes() + spec_bytes(), [&] {
-                    hipLaunchKernelGGL((z_inverse_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, s, dst, T(1),
+                    emit<ZInverseRealB<T>>(dim3(zc.blocks), dim3(zc.threads), zc.lds, s, dst, T(1),
                                        0, g, zc.zt, tab);
                 });
             }
@@ -1399,7 +1445,7 @@ struct SimT final : kwh_sim {
             ra.stokes = stokes;
             ra.dispersion = dispersion;
             ra.ctr = (const int*)ctr.p;
-            const int nbuf = 4 + (nonlinear ? 1 : 0);
+            const int nbuf = 3 + (fft_abs || stokes ? 1 : 0) + (nonlinear ? 1 : 0);
             double bytes = dim * (spec_bytes() + 2 * real_bytes()) + mapb(rho0s.p) + (cp ? real_bytes() : 0);
             if (nonlinear) bytes += mapb(bonas.p);
             if (fft_abs) bytes += real_bytes() + spec_bytes() * (dispersion ? 2 : 1);
@@ -1408,8 +1454,8 @@ struct SimT final : kwh_sim {
                 const bool cr = cp != nullptr;
                 // MODE 0: EOS; 1: FFT absorption inputs; 2: EOS with Stokes absorption
 #define KWH_ZD(NL, MODE)                                                                                            \
-    (cr ? zlaunch(K_ZD, bytes, sq_z_density<T, 16, NL, MODE, true>, sq_z_density<T, 8, NL, MODE, true>, ra, g) \
-        : zlaunch(K_ZD, bytes, sq_z_density<T, 16, NL, MODE, false>, sq_z_density<T, 8, NL, MODE, false>, ra, g))
+    (cr ? zlaunch<SqZDensityB<T, 16, NL, MODE, true>, SqZDensityB<T, 8, NL, MODE, true>>(K_ZD, bytes, ra, g) \
+        : zlaunch<SqZDensityB<T, 16, NL, MODE, false>, SqZDensityB<T, 8, NL, MODE, false>>(K_ZD, bytes, ra, g))
                 if (nonlinear) {
                     if (fft_abs) KWH_ZD(true, 1);
                     else if (stokes) KWH_ZD(true, 2);
@@ -1423,7 +1469,7 @@ struct SimT final : kwh_sim {
             } else {
                 ZCfg zc = zcfg(nbuf);
                 timed(K_ZD, bytes, [&] {
-                    hipLaunchKernelGGL((z_density<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ra, g, zc.zt, tab);
+                    emit<ZDensityB<T>>(dim3(zc.blocks), dim3(zc.threads), zc.lds, ra, g, zc.zt, tab);
                 });
             }
         }
@@ -1463,11 +1509,11 @@ struct SimT final : kwh_sim {
             double bytes = spec_bytes() * (dispersion ? 3 : 2) + 2 * real_bytes() + mapb(c0s.p) + mapb(taus.p) +
                            (dispersion ? mapb(etas.p) : 0) + (nonlinear ? mapb(bonas.p) + mapb(rho0s.p) : 0);
             if (sqR[2]) {
-                zlaunch(K_ZE, bytes, sq_z_eos<T, 16>, sq_z_eos<T, 8>, ea, g);
+                zlaunch<SqZEosB<T, 16>, SqZEosB<T, 8>>(K_ZE, bytes, ea, g);
             } else {
                 ZCfg zc = zcfg(3);
                 timed(K_ZE, bytes, [&] {
-                    hipLaunchKernelGGL((z_eos<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, ea, g, zc.zt, tab);
+                    emit<ZEosB<T>>(dim3(zc.blocks), dim3(zc.threads), zc.lds, ea, g, zc.zt, tab);
                 });
             }
         }
@@ -1502,7 +1548,7 @@ struct SimT final : kwh_sim {
                                             (const int*)ctr.p};
         if (nf > 0)
             timed(K_GA, (double)nf * nsens * sizeof(T) * 2, [&] {
-                hipLaunchKernelGGL((gather_kernel<T>), dim3(grid_blocks(nsens), nf), dim3(256), 0, stream, set);
+                emit<GatherB<T>>(dim3(grid_blocks(nsens), nf), dim3(256), 0, set);
             });
     }
 
@@ -1512,7 +1558,7 @@ struct SimT final : kwh_sim {
         (void)w;
         (void)nc;
         timed(K_GA, 0, [&] {
-            hipLaunchKernelGGL((gather_pos_kernel<T>), dim3(grid_blocks(n)), dim3(256), 0, stream, field, idx, outpos, n,
+            emit<GatherPosB<T>>(dim3(grid_blocks(n)), dim3(256), 0, field, idx, outpos, n,
                                nsens, series, vmax, vmin, sum2, (const int*)ctr.p);
         });
     }
@@ -1524,7 +1570,7 @@ struct SimT final : kwh_sim {
         else main_step(t);
         record_step(t);
         const int recorded = (rec_p || rec_u) ? 1 : 0;
-        hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, stream, (int*)ctr.p, recorded);
+        emit<AdvanceB>(dim3(1), dim3(64), 0, (int*)ctr.p, recorded);
         HIPCHK(hipGetLastError());
     }
     void step_done(int t) override {
@@ -1562,8 +1608,37 @@ struct SimT final : kwh_sim {
         HIPCHK(hipGraphLaunch(sg.exec, stream));
         step_done(t);
     }
+    std::list<StepRecord> step_records;
+    const StepRecord& recorded_step(int t) override {
+        const unsigned key = step_key(t);
+        for (const StepRecord& r : step_records)
+            if (r.key == key) return r;
+        StepRecord r;
+        r.key = key;
+        recording = &r.launches;
+        try {
+            step_launches(t);
+        } catch (...) {
+            recording = nullptr;
+            throw;
+        }
+        recording = nullptr;
+        uint64_t h = 1469598103934665603ull;   // FNV-1a over (kernel, workgroup size, LDS)
+        auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+        for (const LaunchRec& l : r.launches) {
+            mix((uint64_t)(uintptr_t)l.fn);
+            mix(l.block);
+            mix(l.lds);
+        }
+        r.signature = h;
+        step_records.push_back(std::move(r));
+        return step_records.back();
+    }
     int total_steps() const override { return nt; }
     int device() const override { return dev; }
+    uint64_t shape_class() const override {
+        return ((uint64_t)g.m0 * 65599u + (uint64_t)g.m1) * 65599u * 65599u + (uint64_t)g.m2 * 16u + sizeof(T);
+    }
     void set_counters_on(hipStream_t s) override {   // nullptr: the sim's own stream
         HIPCHK(hipSetDevice(dev));
         hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(64), 0, s ? s : stream, (int*)ctr.p, steps, rec_steps);
@@ -1718,7 +1793,7 @@ struct SimT final : kwh_sim {
     // spec[0] = R2C_z(p), the derived state a step starts from
     void rebuild_P() {
         if (sqR[2]) {
-            zlaunch(K_ZFR, real_bytes() + spec_bytes(), sq_z_forward_real<T, 16>, sq_z_forward_real<T, 8>,
+            zlaunch<SqZForwardRealB<T, 16>, SqZForwardRealB<T, 8>>(K_ZFR, real_bytes() + spec_bytes(),
                     (const T*)p.as<T>(), spec[0].as<C>(), T(1), g);
         } else {
             ZCfg zc = zcfg(2);
@@ -2153,22 +2228,167 @@ static int guarded(F&& f) {
     }
 }
 
-// Independent simulations advanced together, each on its own stream (the runtime spreads the
-// streams over the hardware queues, so kernels of small grids overlap). With graphs, every sim
-// replays its own captured step graphs (one per step key); the host issues one graph launch per
-// sim per step, interleaving the sims step by step. Without graphs each sim enqueues its whole
-// run eagerly.
+// Independent simulations advanced together. Three ways:
+//  * streams (mode 0): each sim enqueues its whole run eagerly on its own stream; the runtime
+//    spreads the streams over the hardware queues, so kernels of small grids overlap;
+//  * per-sim graphs (KWH_BATCH_GRAPH): every sim replays its own captured step graphs (one per
+//    step key), interleaved step by step;
+//  * fused (KWH_BATCH_FUSED): one launch per pass of the step for all sims whose steps have the
+//    same launch sequence and launch geometry (kwh_batch.h) -- in practice the sims of one grid
+//    shape. Each sim's step is recorded once per step key; the batch concatenates the sims'
+//    argument packs per pass, uploads them with a block-owner table, and issues the step as a
+//    short list of launches on the stream of that grid shape (shapes overlap on their own
+//    streams), captured in a hipGraph and replayed with KWH_BATCH_GRAPH. Results are bitwise
+//    those of the single runs: every workgroup runs the same body on the same tile.
 struct kwh_batch {
     int dev = 0;
     std::vector<kwh_sim*> sims;
+    std::vector<hipStream_t> streams;   // fused runs: one per grid shape
+
+    struct FusedLaunch {
+        const void* fn = nullptr;
+        unsigned nblocks = 0, block = 0;
+        size_t lds = 0;
+        DBuf packs, owner;
+    };
+    struct FusedStep {
+        std::vector<std::unique_ptr<FusedLaunch>> launches;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    // keyed by the members of a group and their step keys
+    std::map<std::vector<uint64_t>, std::unique_ptr<FusedStep>> fused;
 
     ~kwh_batch() {
+        for (auto& e : fused) {
+            if (e.second->exec) (void)hipGraphExecDestroy(e.second->exec);
+            if (e.second->graph) (void)hipGraphDestroy(e.second->graph);
+        }
+        fused.clear();
+        for (hipStream_t st : streams) (void)hipStreamDestroy(st);
         for (kwh_sim* s : sims) delete s;
     }
-    void run(bool use_graph) {
+
+    // Pack position j of every member's step into one launch.
+    std::unique_ptr<FusedStep> build(const std::vector<const StepRecord*>& recs) {
+        auto fs = std::make_unique<FusedStep>();
+        const size_t nl = recs[0]->launches.size();
+        for (const StepRecord* r : recs)
+            KWH_REQUIRE(r->launches.size() == nl, KWH_EUNSUPPORTED, "batched step: launch sequences differ");
+        for (size_t j = 0; j < nl; ++j) {
+            auto fl = std::make_unique<FusedLaunch>();
+            const LaunchRec& l0 = recs[0]->launches[j];
+            fl->fn = l0.fn;
+            const size_t psz = l0.pack.size();
+            std::vector<unsigned char> packs;
+            std::vector<int> owner;
+            for (size_t m = 0; m < recs.size(); ++m) {
+                const LaunchRec& l = recs[m]->launches[j];
+                KWH_REQUIRE(l.fn == l0.fn && l.pack.size() == psz && (l.flex || l.block == l0.block), KWH_EUNSUPPORTED,
+                            "batched step: kernels differ");
+                const int start = (int)owner.size();
+                const int64_t nb = (int64_t)l.gx * l.gy;
+                KWH_REQUIRE((int64_t)owner.size() + nb < (int64_t)1 << 31, KWH_EUNSUPPORTED, "batched launch too large");
+                owner.insert(owner.end(), (size_t)nb, (int)m);
+                const size_t off = packs.size();
+                packs.insert(packs.end(), l.pack.begin(), l.pack.end());
+                std::memcpy(packs.data() + off, &start, sizeof(int));   // BatchPack::start
+                fl->block = std::max(fl->block, l.block);
+                fl->lds = std::max(fl->lds, l.lds);
+            }
+            fl->nblocks = (unsigned)owner.size();
+            if (fl->nblocks == 0) continue;
+            fl->packs.alloc(packs.size());
+            HIPCHK(hipMemcpy(fl->packs.p, packs.data(), packs.size(), hipMemcpyHostToDevice));
+            fl->owner.alloc(owner.size() * sizeof(int));
+            HIPCHK(hipMemcpy(fl->owner.p, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice));
+            fs->launches.push_back(std::move(fl));
+        }
+        return fs;
+    }
+
+    void launch_fused(FusedStep& fs, hipStream_t st) {
+        for (auto& fl : fs.launches) {
+            void* pk = fl->packs.p;
+            void* ow = fl->owner.p;
+            void* args[] = {&pk, &ow};
+            HIPCHK(hipLaunchKernel(fl->fn, dim3(fl->nblocks), dim3(fl->block), args, fl->lds, st));
+        }
+    }
+
+    void run_fused(bool use_graph) {
+        const int n = (int)sims.size();
+        // one stream per grid shape (up to kMaxStreams): groups of different shapes overlap, and a
+        // sim always stays on its shape's stream, which orders its steps
+        constexpr int kMaxStreams = 8;
+        std::map<uint64_t, int> shapes;
+        std::vector<int> sidx(n);
+        for (int i = 0; i < n; ++i) {
+            auto it = shapes.find(sims[i]->shape_class());
+            if (it == shapes.end()) it = shapes.emplace(sims[i]->shape_class(), (int)shapes.size()).first;
+            sidx[i] = it->second % kMaxStreams;
+        }
+        while ((int)streams.size() < std::min((int)shapes.size(), kMaxStreams)) {
+            hipStream_t st = nullptr;
+            HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            streams.push_back(st);
+        }
+        for (int i = 0; i < n; ++i) sims[i]->set_counters_on(streams[sidx[i]]);
+        std::vector<const StepRecord*> rec(n);
+        std::vector<int> ts(n);
+        for (;;) {
+            // group the active sims by stream and launch sequence of their current step
+            std::vector<std::pair<std::pair<int, uint64_t>, int>> order;
+            for (int i = 0; i < n; ++i) {
+                ts[i] = sims[i]->steps_done();
+                if (ts[i] >= sims[i]->total_steps()) continue;
+                rec[i] = &sims[i]->recorded_step(ts[i]);
+                order.push_back({{sidx[i], rec[i]->signature}, i});
+            }
+            if (order.empty()) break;
+            std::stable_sort(order.begin(), order.end(),
+                             [](const auto& a, const auto& b) { return a.first < b.first; });
+            for (size_t g0 = 0; g0 < order.size();) {
+                size_t g1 = g0;
+                std::vector<uint64_t> key;
+                std::vector<const StepRecord*> members;
+                while (g1 < order.size() && order[g1].first == order[g0].first) {
+                    const int i = order[g1].second;
+                    key.push_back((uint64_t)i);
+                    key.push_back(rec[i]->key);
+                    members.push_back(rec[i]);
+                    ++g1;
+                }
+                hipStream_t st = streams[order[g0].first.first];
+                auto it = fused.find(key);
+                if (it == fused.end()) it = fused.emplace(key, build(members)).first;
+                FusedStep& fs = *it->second;
+                if (use_graph) {
+                    if (!fs.exec) {
+                        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                        launch_fused(fs, st);
+                        HIPCHK(hipStreamEndCapture(st, &fs.graph));
+                        HIPCHK(hipGraphInstantiate(&fs.exec, fs.graph, nullptr, nullptr, 0));
+                    }
+                    HIPCHK(hipGraphLaunch(fs.exec, st));
+                } else {
+                    launch_fused(fs, st);
+                }
+                g0 = g1;
+            }
+            for (auto& o : order) sims[o.second]->step_done(ts[o.second]);
+        }
+        for (hipStream_t st : streams) HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void run(int mode) {
         HIPCHK(hipSetDevice(dev));
         for (kwh_sim* s : sims) s->sync();
-        if (!use_graph) {
+        if (mode & KWH_BATCH_FUSED) {
+            run_fused((mode & KWH_BATCH_GRAPH) != 0);
+            return;
+        }
+        if (!(mode & KWH_BATCH_GRAPH)) {
             for (kwh_sim* s : sims) s->launch(s->steps_done(), s->total_steps());
             for (kwh_sim* s : sims) s->sync();
             return;
@@ -2505,10 +2725,11 @@ int kwh_batch_create(kwh_ctx* ctx, const kwh_sim_desc* descs, int32_t n, kwh_bat
     });
 }
 
-int kwh_batch_run(kwh_batch* batch, int32_t use_graph) {
+int kwh_batch_run(kwh_batch* batch, int32_t mode) {
     return guarded([&] {
         KWH_REQUIRE(batch, KWH_EINVAL, "batch is NULL");
-        batch->run(use_graph != 0);
+        KWH_REQUIRE((mode & ~(KWH_BATCH_GRAPH | KWH_BATCH_FUSED)) == 0, KWH_EINVAL, "invalid batch run mode");
+        batch->run(mode);
     });
 }
 

@@ -153,11 +153,11 @@ struct SqRow {
 template <int R>
 struct SqZPos {
     int w, j, q, ra, rb;
-    __device__ SqZPos(int P) {
+    __device__ SqZPos(int P, const BIdx& bi) {
         const int tid = threadIdx.x;
         w = tid / R;
         j = tid - w * R;
-        const int tile = xcd_tile(blockIdx.x, gridDim.x);
+        const int tile = bi.tile();
         q = tile * P + w;
         ra = 2 * q;
         rb = ra + 1;
@@ -193,13 +193,13 @@ __device__ void sq_apply_sparse(const SparseSrc<T>& s, const Geo& g, int row0, i
 }
 
 template <typename T, int R>
-__global__ void __launch_bounds__(256) sq_z_forward_real(const T* __restrict__ src, cx<T>* spec, T scale, Geo g,
-                                                         int P, AxisTabs<T> tab) {
+__device__ __forceinline__ void sq_z_forward_real_body(const BIdx& bi, const T* __restrict__ src, cx<T>* spec, T scale, const Geo& g,
+                                                         int P, const AxisTabs<T>& tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
     cx<T>* lds = ldsb + R * R;
-    SqZPos<R> ps(P);
+    SqZPos<R> ps(P, bi);
     LaneTw<T, R> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
@@ -212,15 +212,20 @@ __global__ void __launch_bounds__(256) sq_z_forward_real(const T* __restrict__ s
     }
     S::forward_store(spec, ps.ra, ps.rb, g.nrows, g.mzh, lds + ps.w * S::LP, ps.j, lt, v);
 }
+template <typename T, int R>
+__global__ void __launch_bounds__(256) sq_z_forward_real(const T* __restrict__ src, cx<T>* spec, T scale, Geo g,
+                                                         int P, AxisTabs<T> tab) {
+    sq_z_forward_real_body<T, R>(bidx_here(), src, spec, scale, g, P, tab);
+}
 
 template <typename T, int R>
-__global__ void __launch_bounds__(256) sq_z_inverse_real(const cx<T>* spec, T* __restrict__ dst, T scale,
-                                                         int accumulate, Geo g, int P, AxisTabs<T> tab) {
+__device__ __forceinline__ void sq_z_inverse_real_body(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst, T scale,
+                                                         int accumulate, const Geo& g, int P, const AxisTabs<T>& tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
     cx<T>* lds = ldsb + R * R;
-    SqZPos<R> ps(P);
+    SqZPos<R> ps(P, bi);
     LaneTw<T, R> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     cx<T> v[R];
@@ -240,6 +245,11 @@ __global__ void __launch_bounds__(256) sq_z_inverse_real(const cx<T>* spec, T* _
         }
     }
 }
+template <typename T, int R>
+__global__ void __launch_bounds__(256) sq_z_inverse_real(const cx<T>* spec, T* __restrict__ dst, T scale,
+                                                         int accumulate, Geo g, int P, AxisTabs<T> tab) {
+    sq_z_inverse_real_body<T, R>(bidx_here(), spec, dst, scale, accumulate, g, P, tab);
+}
 
 // Collocated velocity along the contiguous axis for a list of rows: dst[row] =
 // C2R_z(coll_z * Z[row]) / L, Z = R2C_z(u) of the updated component (as the velocity kernel
@@ -247,13 +257,13 @@ __global__ void __launch_bounds__(256) sq_z_inverse_real(const cx<T>* spec, T* _
 // transform of the reference (KSpaceFirstOrder.swift:735-745) reduces to this 1-D one on the
 // rows the sensor reads. Rows are paired in list order; an odd tail pairs with nothing.
 template <typename T, int R>
-__global__ void __launch_bounds__(256) sq_z_colloc_rows(const cx<T>* spec, T* __restrict__ dst,
-                                                        const int* __restrict__ rows, int nlist, Geo g, int P,
-                                                        AxisTabs<T> tab) {
+__device__ __forceinline__ void sq_z_colloc_rows_body(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst,
+                                                        const int* __restrict__ rows, int nlist, const Geo& g, int P,
+                                                        const AxisTabs<T>& tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
-    SqZPos<R> ps(P);
+    SqZPos<R> ps(P, bi);
     LaneTw<T, R> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     const bool va = ps.ra < nlist, vb = ps.rb < nlist;
@@ -276,6 +286,12 @@ __global__ void __launch_bounds__(256) sq_z_colloc_rows(const cx<T>* spec, T* __
         if (vb) dst[(int64_t)rb * S::L + n] = v[r].y;
     }
 }
+template <typename T, int R>
+__global__ void __launch_bounds__(256) sq_z_colloc_rows(const cx<T>* spec, T* __restrict__ dst,
+                                                        const int* __restrict__ rows, int nlist, Geo g, int P,
+                                                        AxisTabs<T> tab) {
+    sq_z_colloc_rows_body<T, R>(bidx_here(), spec, dst, rows, nlist, g, P, tab);
+}
 
 // Scheduling fence: keeps the compiler from hoisting the loads of later element groups above
 // earlier ones (bounds live registers; other waves hide the latency).
@@ -285,12 +301,12 @@ constexpr int kVelGroup = 4;   // velocity update (keeps it within the three-wav
 
 // Velocity update, per axis: C2R_z(G_a) -> u_a = pml_sg (pml_sg u_a - dt/rho_a g) [+ sources] -> R2C_z.
 template <typename T, int R, bool CORR>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_VELOCITY_WAVES : 1) sq_z_velocity(VelArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+__device__ __forceinline__ void sq_z_velocity_body(const BIdx& bi, const VelArgs<T>& a, const Geo& g, int P, const AxisTabs<T>& tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
     cx<T>* lds_all = ldsb + R * R;
-    SqZPos<R> ps(P);
+    SqZPos<R> ps(P, bi);
     LaneTw<T, R> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
@@ -343,8 +359,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_VELOCITY_WAVES : 1) 
                 KWH_SCHED_FENCE();
             }
         }
-        if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{(int)(xcd_tile(blockIdx.x, gridDim.x) * P), P, 0, 0}))
-            sq_apply_sparse<T, R>(a.sp[ax], g, 2 * xcd_tile(blockIdx.x, gridDim.x) * P, 2 * P, lds_all, ps.w, ps.j,
+        if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{(int)(bi.tile() * P), P, 0, 0}))
+            sq_apply_sparse<T, R>(a.sp[ax], g, 2 * bi.tile() * P, 2 * P, lds_all, ps.w, ps.j,
                                   a.ctr[0], v);
         {
             T* wa = u + oa;
@@ -358,6 +374,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_VELOCITY_WAVES : 1) 
         S::forward_store(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
     }
 }
+template <typename T, int R, bool CORR>
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_VELOCITY_WAVES : 1) sq_z_velocity(VelArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    sq_z_velocity_body<T, R, CORR>(bidx_here(), a, g, P, tab);
+}
 
 // Density update, per axis: C2R_z(D_a) -> rho_a = pml (pml rho_a - dt rho0 [nl] du_a) [+ sources],
 // followed by the EOS (MODE 0), the EOS with Stokes absorption (MODE 2) or the inputs of the
@@ -365,19 +385,19 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_VELOCITY_WAVES : 1) 
 // The total density and, for MODE 1/2, rho0 * sum_a du_a are accumulated in registers while the
 // axes are processed, so each divergence spectrum is read once.
 template <typename T, int R, bool NL, int MODE, bool CORR>
-__global__ void __launch_bounds__(256, (NL || CORR || sizeof(T) != 4) ? 1 : (MODE == 0 ? KWH_DENSITY_WAVES : 2)) sq_z_density(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+__device__ __forceinline__ void sq_z_density_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P, const AxisTabs<T>& tab) {
     using S = SqRow<T, R>;
     constexpr bool ACC = MODE != 0;   // accumulate rho0 * sum_a du_a
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
     cx<T>* lds_all = ldsb + R * R;
-    SqZPos<R> ps(P);
+    SqZPos<R> ps(P, bi);
     LaneTw<T, R> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
-    const int row0 = 2 * xcd_tile(blockIdx.x, gridDim.x) * P;
+    const int row0 = 2 * bi.tile() * P;
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
     const T* r0pa = a.rho0.lane(oa, ps.j);
     const T* r0pb = a.rho0.lane(ob, ps.j);
@@ -503,15 +523,19 @@ __global__ void __launch_bounds__(256, (NL || CORR || sizeof(T) != 4) ? 1 : (MOD
         if (a.dispersion) S::forward_store(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, rt);
     }
 }
+template <typename T, int R, bool NL, int MODE, bool CORR>
+__global__ void __launch_bounds__(256, (NL || CORR || sizeof(T) != 4) ? 1 : (MODE == 0 ? KWH_DENSITY_WAVES : 2)) sq_z_density(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    sq_z_density_body<T, R, NL, MODE, CORR>(bidx_here(), a, g, P, tab);
+}
 
 // EOS with FFT absorption: p = c^2 (rhoT + tau C2R_z(A1) - eta C2R_z(A2) [+ B/A rhoT^2 / (2 rho0)]),
 // P = R2C_z(p). Both spectra are loaded before the first transform (one memory phase).
 template <typename T, int R>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_EOS_WAVES : 1) sq_z_eos(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+__device__ __forceinline__ void sq_z_eos_body(const BIdx& bi, const EosArgs<T>& a, const Geo& g, int P, const AxisTabs<T>& tab) {
     using S = SqRow<T, R>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
-    SqZPos<R> ps(P);
+    SqZPos<R> ps(P, bi);
     LaneTw<T, R> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = ldsb + R * R + ps.w * S::LP;
@@ -573,6 +597,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_EOS_WAVES : 1) sq_z_
     }
     S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, e);
 }
+template <typename T, int R>
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_EOS_WAVES : 1) sq_z_eos(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    sq_z_eos_body<T, R>(bidx_here(), a, g, P, tab);
+}
 
 // ------------------------------------------------------------------------------------------
 // Strided / middle passes over a tile of ntr contiguous columns: lane (w, j), w fastest.
@@ -604,7 +632,7 @@ struct SqCol {
 };
 
 template <typename T, int R, int NTR, bool INV>
-__global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void sq_strided_body(const BIdx& bi, const StridedArgs& a, const cx<T>* __restrict__ tw) {
     using S = SqCol<T, R, NTR>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
@@ -612,12 +640,12 @@ __global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __
     const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
     LaneTw<T, R> lt;
     lt.load(ldsb, tw, j);
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = bi.tile();
     const int64_t c = (int64_t)tile * NTR + w;
     const bool valid = c < a.ncols;
     const int64_t cc = valid ? c : 0;
     const int64_t base = (cc / a.inner) * (int64_t)S::L * a.inner + cc % a.inner;
-    cx<T>* data = reinterpret_cast<cx<T>*>(a.fields[blockIdx.y]) + base + (int64_t)j * a.stride;
+    cx<T>* data = reinterpret_cast<cx<T>*>(a.fields[bi.y]) + base + (int64_t)j * a.stride;
     const int64_t step = (int64_t)R * a.stride;
     cx<T> v[R];
     {
@@ -638,11 +666,15 @@ __global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __
         }
     }
 }
+template <typename T, int R, int NTR, bool INV>
+__global__ void __launch_bounds__(256) sq_strided(StridedArgs a, const cx<T>* __restrict__ tw) {
+    sq_strided_body<T, R, NTR, INV>(bidx_here(), a, tw);
+}
 
 // Middle pass: forward along the middle axis, operators, inverse, store. Every table-using
 // operator of one input reads the same real k-space table (checked on the host), loaded once.
 template <typename T, int R, int NTR>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_MIDDLE_WAVES : 1) sq_middle(MidArgs a, AxisTabs<T> tab, Geo g) {
+__device__ __forceinline__ void sq_middle_body(const BIdx& bi, const MidArgs& a, const AxisTabs<T>& tab, const Geo& g) {
     using S = SqCol<T, R, NTR>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
@@ -650,7 +682,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_MIDDLE_WAVES : 1) sq
     const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
     LaneTw<T, R> lt;
     lt.load(ldsb, tab.tw[a.axis], j);
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = bi.tile();
     const int64_t c = (int64_t)tile * NTR + w;
     const bool valid = c < a.ncols;
     const int64_t cc = valid ? c : 0;
@@ -717,6 +749,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_MIDDLE_WAVES : 1) sq
             }
         }
     }
+}
+template <typename T, int R, int NTR>
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? KWH_MIDDLE_WAVES : 1) sq_middle(MidArgs a, AxisTabs<T> tab, Geo g) {
+    sq_middle_body<T, R, NTR>(bidx_here(), a, tab, g);
 }
 
 }  // namespace kwh

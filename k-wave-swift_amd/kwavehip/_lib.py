@@ -9,6 +9,8 @@ import ctypes as C
 import os
 
 ABI_VERSION = 2
+BATCH_GRAPH = 1    # kwh_batch_run mode flags (KWH_BATCH_GRAPH / KWH_BATCH_FUSED)
+BATCH_FUSED = 2
 
 OK, EINVAL, ENOMEM, EDEVICE, ENONFINITE, EUNSUPPORTED, ESTATE = 0, -1, -2, -3, -4, -5, -6
 

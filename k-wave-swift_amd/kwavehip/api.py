@@ -553,8 +553,13 @@ def timeReversal(grid: KWaveGrid, medium: KWaveMedium, sensor: KWaveSensor, reco
 
 class SimulationBatch:
     """Independent simulations of uneven sizes advanced together on one device (kwh_batch_*):
-    per-scan-line transmits, for instance. `run(use_graph=True)` launches one hipGraph per time
-    step whose branches are the sims' steps; `sims[i]` reads outputs like a Simulation."""
+    per-scan-line transmits, for instance. `sims[i]` reads outputs like a Simulation.
+
+    run(fused=True): one launch per pass of the time step covers all sims of one grid shape, the
+    shapes running concurrently on their own streams; with use_graph each shape's batched step
+    is captured once and replayed.
+    run(fused=False): each sim on its own stream, replaying its own step graphs (use_graph) or
+    enqueuing its steps eagerly. All modes give results bitwise equal to single runs."""
 
     def __init__(self, cases, device: int = 0):
         self.lib = L.load()
@@ -579,8 +584,9 @@ class SimulationBatch:
             L.check(self.lib.kwh_batch_sim(self.handle, i, C.byref(h)))
             self.sims.append(Simulation._view(self.lib, h, grid, dt, rec, ns))
 
-    def run(self, use_graph: bool = True):
-        L.check(self.lib.kwh_batch_run(self.handle, 1 if use_graph else 0))
+    def run(self, use_graph: bool = True, fused: bool = True):
+        mode = (L.BATCH_GRAPH if use_graph else 0) | (L.BATCH_FUSED if fused else 0)
+        L.check(self.lib.kwh_batch_run(self.handle, mode))
 
     def close(self):
         if getattr(self, "handle", None):

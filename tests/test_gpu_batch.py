@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import kwave_oracle as O
-from kwavehip import (KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, KWaveTransducer, KWaveTransducerGeometry,
+from kwavehip import (AbsorptionMode, KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, KWaveTransducer, KWaveTransducerGeometry,
                       RecordField, Simulation, SimulationBatch, SimulationOptions, SourceMode, toneBurst)
 from kwavehip import _lib as L
 
@@ -89,13 +89,14 @@ def test_element_traces_match_host_combination():
     assert np.abs(dev - host).max() <= 1e-6 * scale
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_of_uneven_sizes_equals_single_runs(use_graph):
-    """Three sims of different sizes and drives (their sources switch off at different steps,
-    so the batched graph is rebuilt mid-run): bitwise equal to separate runs."""
+@pytest.mark.parametrize("use_graph,fused", [(True, True), (False, True), (True, False), (False, False)])
+def test_batch_of_uneven_sizes_equals_single_runs(use_graph, fused):
+    """Three sims of different sizes and drives (their sources switch off at different steps and
+    their runs end at different steps, so the fused launches regroup mid-run): bitwise equal to
+    separate runs."""
     cases = [scan_line_case((32, 48, 24), 3, 0.0, 8e-3, nt=70), scan_line_case((40, 40, 24), 4, 15.0, np.inf, nt=90),
              scan_line_case((36, 48, 32), 5, -10.0, 6e-3, nt=60)]
-    print(f"\n[case] batch of 3 uneven sims, graph={use_graph}", flush=True)
+    print(f"\n[case] batch of 3 uneven sims, graph={use_graph} fused={fused}", flush=True)
     singles = []
     for grid, med, src, sens, opts, _ in cases:
         s = Simulation(grid, med, src, sens, opts)
@@ -103,7 +104,7 @@ def test_batch_of_uneven_sizes_equals_single_runs(use_graph):
         singles.append(s.outputs())
         s.close()
     batch = SimulationBatch([c[:5] for c in cases])
-    batch.run(use_graph=use_graph)
+    batch.run(use_graph=use_graph, fused=fused)
     for i, (sim, ref) in enumerate(zip(batch.sims, singles)):
         out = sim.outputs()
         assert out.p.shape == ref.p.shape, i
@@ -111,4 +112,50 @@ def test_batch_of_uneven_sizes_equals_single_runs(use_graph):
         assert np.array_equal(out.pFinal, ref.pFinal), i
         tr_out = sim.transducer_traces(cases[i][5])
         assert tr_out.shape == (16, cases[i][0].nt)
+    batch.close()
+
+
+def ivp_case(shape, seed, nt, absorbing=False, nonlinear=False):
+    """2D initial-value problem on a random medium, binary line sensor recording p / pMax / uy."""
+    rng = np.random.default_rng(seed)
+    nx, ny = shape
+    c = (1500.0 * (1 + rng.uniform(-0.05, 0.05, shape))).astype(np.float32)
+    rho = (1000.0 * (1 + rng.uniform(-0.05, 0.05, shape))).astype(np.float32)
+    grid = KWaveGrid(nx, 1e-4, ny, 1e-4)
+    grid.setTime(nt, 0.3 * 1e-4 / float(c.max()))
+    p0 = np.zeros(shape, np.float32)
+    p0[nx // 2 - 3:nx // 2 + 3, ny // 3:ny // 3 + 5] = 1.0
+    mask = np.zeros(shape, np.float32)
+    mask[nx // 4, 2:ny - 2] = 1
+    kw = {}
+    if absorbing:
+        kw.update(alphaCoeff=0.75, alphaPower=1.5, alphaMode=AbsorptionMode.powerLaw)
+    if nonlinear:
+        kw.update(bOnA=6.0)
+    med = KWaveMedium(c, rho, **kw)
+    sens = KWaveSensor(mask=mask, record=RecordField.p | RecordField.pMax | RecordField.uy)
+    return grid, med, KWaveSource(p0=p0), sens, SimulationOptions(pmlSize=8, smoothP0=False)
+
+
+def test_fused_batch_register_resident_mix():
+    """Fused launches over sims of the register-resident path (64^2, 256^2), a generic size,
+    absorbing and nonlinear media: sims with equal launch sequences share launches, the rest run
+    in their own groups; every output bitwise equal to separate runs."""
+    cases = [ivp_case((64, 64), 1, 60), ivp_case((64, 64), 2, 70), ivp_case((256, 256), 3, 40),
+             ivp_case((48, 64), 4, 50), ivp_case((64, 64), 5, 60, absorbing=True),
+             ivp_case((64, 64), 6, 55, absorbing=True, nonlinear=True), ivp_case((256, 256), 7, 30, absorbing=True)]
+    print("\n[case] fused batch, register-resident / generic / absorbing / nonlinear mix", flush=True)
+    singles = []
+    for case in cases:
+        s = Simulation(*case)
+        s.run()
+        singles.append(s.outputs())
+        s.close()
+    batch = SimulationBatch(cases)
+    batch.run(use_graph=True, fused=True)
+    for i, (sim, ref) in enumerate(zip(batch.sims, singles)):
+        out = sim.outputs()
+        for f in ("p", "pMax", "uy", "pFinal"):
+            a, b = getattr(out, f), getattr(ref, f)
+            assert np.array_equal(a, b), (i, f, float(np.abs(a - b).max()))
     batch.close()
