@@ -604,6 +604,7 @@ struct SimT final : kwh_sim {
         HIPCHK(hipSetDevice(dev));
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         if (const char* e = std::getenv("KWH_LDS_BUDGET")) sched.lds = (size_t)std::atol(e);
+        if (const char* e = std::getenv("KWH_MIN_BLOCKS")) sched.min_blocks = std::max(1, std::atoi(e));
         scal_pool.alloc((size_t)64 * kScalarRep * sizeof(T));
         zero_map = mk_scalar(T(0));
         taus = etas = bonas = zero_map;
