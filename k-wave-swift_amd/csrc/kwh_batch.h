@@ -17,7 +17,6 @@
 #include <tuple>
 
 #include "kwh_kernels.h"
-#include "kwh_sq.h"
 
 namespace kwh {
 
@@ -68,20 +67,5 @@ template <typename T> using ScatterB = Body<&scatter_kernel_body<T>, &scatter_ke
 using AdvanceB = Body<&advance_kernel_body, &advance_kernel>;
 template <typename T> using GatherB = Body<&gather_kernel_body<T>, &gather_kernel<T>>;
 template <typename T> using GatherPosB = Body<&gather_pos_kernel_body<T>, &gather_pos_kernel<T>>;
-
-template <typename T, int R> using SqZForwardRealB = Body<&sq_z_forward_real_body<T, R>, &sq_z_forward_real<T, R>, 1, false>;
-template <typename T, int R> using SqZInverseRealB = Body<&sq_z_inverse_real_body<T, R>, &sq_z_inverse_real<T, R>, 1, false>;
-template <typename T, int R> using SqZCollocRowsB = Body<&sq_z_colloc_rows_body<T, R>, &sq_z_colloc_rows<T, R>, 1, false>;
-template <typename T, int R, bool CORR>
-using SqZVelocityB = Body<&sq_z_velocity_body<T, R, CORR>, &sq_z_velocity<T, R, CORR>,
-                          sizeof(T) == 4 ? KWH_VELOCITY_WAVES : 1, false>;
-template <typename T, int R, bool NL, int MODE, bool CORR>
-using SqZDensityB = Body<&sq_z_density_body<T, R, NL, MODE, CORR>, &sq_z_density<T, R, NL, MODE, CORR>,
-                         (NL || CORR || sizeof(T) != 4) ? 1 : (MODE == 0 ? KWH_DENSITY_WAVES : 2), false>;
-template <typename T, int R> using SqZEosB = Body<&sq_z_eos_body<T, R>, &sq_z_eos<T, R>, sizeof(T) == 4 ? KWH_EOS_WAVES : 1, false>;
-template <typename T, int R, int NTR, bool INV>
-using SqStridedB = Body<&sq_strided_body<T, R, NTR, INV>, &sq_strided<T, R, NTR, INV>, 1, false>;
-template <typename T, int R, int NTR>
-using SqMiddleB = Body<&sq_middle_body<T, R, NTR>, &sq_middle<T, R, NTR>, sizeof(T) == 4 ? KWH_MIDDLE_WAVES : 1, false>;
 
 }  // namespace kwh

@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace kwh {
 
 template <typename T> struct cx { T x, y; };
@@ -277,6 +279,100 @@ __device__ void fft_run(const FftPlan& pl, const Src& first, const Dst& last, cx
         (void)ldst;
         if (!isLast) __syncthreads();
         Ns *= R;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Register DFTs of composite length with compile-time twiddles (register-resident kernels)
+// ---------------------------------------------------------------------------------------------
+
+// cos and sin of 2*pi*e/N, evaluated at compile time: reduce to the nearest quadrant,
+// 2*pi*e/N = q*pi/2 + psi with |psi| <= pi/4, Taylor series of sin / cos of psi to the 25th power
+// (truncation error below 1e-25), then rotate by q quarter turns exactly. Multiples of pi/2 give
+// exact 0 and +-1.
+struct CosSin { double c, s; };
+constexpr CosSin ct_cis(long e, long n) {
+    long m = e % n;
+    if (m < 0) m += n;
+    const long q = (8 * m + n) / (2 * n);              // nearest quadrant, 0..4
+    const double psi = 1.57079632679489661923132169163975144 * (double)(4 * m - q * n) / (double)n;
+    const double x2 = psi * psi;
+    double ts = psi, ss = psi, tc = 1.0, sc = 1.0;
+    for (int k = 1; k <= 12; ++k) {
+        ts *= -x2 / (double)((2 * k) * (2 * k + 1));
+        ss += ts;
+        tc *= -x2 / (double)((2 * k - 1) * (2 * k));
+        sc += tc;
+    }
+    switch (q & 3) {
+        case 0: return CosSin{sc, ss};
+        case 1: return CosSin{-ss, sc};
+        case 2: return CosSin{-sc, -ss};
+        default: return CosSin{ss, -sc};
+    }
+}
+
+// W_N^EXP = exp(-2 pi i EXP / N) (forward) or its conjugate (inverse), a compile-time constant.
+template <typename T, int N, int EXP, bool INV>
+__device__ __forceinline__ cx<T> twk() {
+    constexpr CosSin v = ct_cis(EXP, N);
+    return cx<T>{T(v.c), INV ? T(v.s) : T(-v.s)};
+}
+
+// Compile-time loop: f(integral_constant<int, i>) for i = B .. E-1.
+template <int B, int E, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        sfor<B + 1, E>(f);
+    }
+}
+
+// Factor of a composite length N = N1 * N2 (N2 = 3 or 5 when they divide N, else 4 or 8).
+template <int N> struct CtSplit {
+    static constexpr int N2 = N % 3 == 0 ? 3 : (N % 5 == 0 ? 5 : (N >= 64 ? 8 : 4));
+    static constexpr int N1 = N / N2;
+};
+
+template <typename T, int N, bool INV> __device__ __forceinline__ void dftN(cx<T>* v);
+
+// Cooley-Tukey in registers, natural order in and out: with n = n1 + N1 n2 and k = k2 + N2 k1,
+// X[k] = sum_n1 W_N^(n1 k2) W_N1^(n1 k1) sum_n2 x[n] W_N2^(n2 k2).
+template <typename T, int N1, int N2, bool INV>
+__device__ __forceinline__ void dft_ct(cx<T>* v) {
+    constexpr int N = N1 * N2;
+    cx<T> y[N];
+    sfor<0, N1>([&](auto c1) {
+        constexpr int n1 = decltype(c1)::value;
+        cx<T> t[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) t[n2] = v[n1 + N1 * n2];
+        dftN<T, N2, INV>(t);
+        sfor<0, N2>([&](auto c2) {
+            constexpr int k2 = decltype(c2)::value;
+            if constexpr ((n1 * k2) % N == 0) y[n1 * N2 + k2] = t[k2];
+            else y[n1 * N2 + k2] = cmul(t[k2], twk<T, N, n1 * k2, INV>());
+        });
+    });
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+        cx<T> s[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) s[n1] = y[n1 * N2 + k2];
+        dftN<T, N1, INV>(s);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) v[k2 + N2 * k1] = s[k1];
+    }
+}
+
+// DFT of any length whose prime factors are 2, 3 and 5, in registers.
+template <typename T, int N, bool INV>
+__device__ __forceinline__ void dftN(cx<T>* v) {
+    if constexpr (N == 1) {
+    } else if constexpr (N == 2 || N == 3 || N == 4 || N == 5 || N == 8 || N == 16) {
+        dft<T, N, INV>(v);
+    } else {
+        dft_ct<T, CtSplit<N>::N1, CtSplit<N>::N2, INV>(v);
     }
 }
 

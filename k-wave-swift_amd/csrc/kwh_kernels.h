@@ -343,10 +343,11 @@ __device__ __forceinline__ int axis_pos(const Geo& g, int row, int n, int ax) {
 
 // Scalar-or-map accessor. `dp` always points at device memory: the map itself (mask = ~0) or a
 // buffer holding the scalar (mask = 0), so at() is a branch-free load. A scalar's buffer holds
-// kScalarRep copies: lane(off, j) returns a per-lane pointer whose [R*r] elements (j + R*r <
-// kScalarRep) are the map's row elements or the scalar, so the register-resident kernels address
-// every element as base + immediate. `p` / `s` keep the host-side view (p == nullptr: scalar).
-constexpr int kScalarRep = 256;
+// kScalarRep copies: lane(off, j) returns a per-lane pointer whose [J*m] elements (j + J*m <
+// kScalarRep, the longest register-resident row) are the map's row elements or the scalar, so the
+// register-resident kernels address every element as base + immediate. `p` / `s` keep the
+// host-side view (p == nullptr: scalar).
+constexpr int kScalarRep = 512;
 template <typename T> struct SMap {
     const T* p;
     T s;
@@ -793,7 +794,7 @@ __global__ void scatter_kernel(T* dst, const int64_t* index, const T* signal, co
 
 // End of a time step: t += 1 and, when the step recorded, slot += 1. Kernels read both from
 // here, so a captured step graph replays unchanged.
-__global__ void set_counters_kernel(int* ctr, int t, int slot) {
+static __global__ void set_counters_kernel(int* ctr, int t, int slot) {
     if (threadIdx.x == 0) {
         ctr[0] = t;
         ctr[1] = slot;
@@ -806,7 +807,7 @@ __device__ __forceinline__ void advance_kernel_body(const BIdx& bi, int* ctr, in
         ctr[1] += recorded;
     }
 }
-__global__ void advance_kernel(int* ctr, int recorded) {
+static __global__ void advance_kernel(int* ctr, int recorded) {
     advance_kernel_body(bidx_here(), ctr, recorded);
 }
 

@@ -1,0 +1,779 @@
+// kwh_reg.h -- register-resident kernels for axis lengths L = J*E with J | E
+// (J, E) = (4, 8): 32, (8, 8): 64, (4, 24): 96, (8, 16): 128, (4, 40): 160, (8, 24): 192,
+// (16, 16): 256, (16, 32): 512.
+//
+// A length-L transform is a two-step Cooley-Tukey split L = J * E. Lane j (0..J-1) of a
+// transform holds elements n = j + J*m (m = 0..E-1) in registers:
+//   step 1: DFT_E over m in registers -> Y[n1 = j][k2], k2 = 0..E-1, written to LDS;
+//   step 2: lane j takes the columns k2 = j + J*a (a = 0..E/J-1): reads Y[n1][k2] for n1 =
+//           0..J-1, twiddles W_L^(n1*k2), DFT_J over n1 -> X[k2 + E*k1] = element
+//           j + J*(a + (E/J)*k1) of the lane-natural pattern.
+// Input and output of a transform are therefore both in the lane-natural pattern n = j + J*m, so
+// an inverse, an elementwise update and a forward chain without extra LDS traffic, and the
+// outputs of a forward feed the next inverse directly (middle pass). One LDS transpose per
+// transform; every lane busy in both steps. For J = E (64, 256) this is two radix-R stages.
+#pragma once
+#include "kwh_kernels.h"
+
+// Occupancy floors (waves per SIMD) of the fused row kernels with E <= 16 in FP32: without them
+// the register allocator spends the whole register file on load hoisting and drops a wave per
+// SIMD. The density kernels that also accumulate rho0 * div u stay at two. Longer rows (E > 16)
+// hold more registers per lane and carry no floor.
+#ifndef KWH_DENSITY_WAVES
+#define KWH_DENSITY_WAVES 3
+#endif
+#ifndef KWH_VELOCITY_WAVES
+#define KWH_VELOCITY_WAVES 4
+#endif
+#ifndef KWH_MIDDLE_WAVES
+#define KWH_MIDDLE_WAVES 3
+#endif
+#ifndef KWH_EOS_WAVES
+#define KWH_EOS_WAVES 3
+#endif
+
+namespace kwh {
+
+template <typename T, int E>
+constexpr int reg_waves(int w) { return (sizeof(T) == 4 && E <= 16) ? w : 1; }
+
+// Twiddles W_L^(n1*k2) of step 2, in an LDS table laid out [n1][k2] (L elements at the start of
+// the workgroup's LDS): lane j reads slot (j + J*a) + E*n1, a base + immediate address, and the
+// table costs no registers between transforms. The inverse uses the conjugates. The table is
+// filled cooperatively at kernel start; the barrier inside the first transform orders the fill
+// before any read.
+template <typename T, int J, int E>
+struct LaneTw {
+    static constexpr int L = J * E;
+    const cx<T>* p;
+    __device__ __forceinline__ void load(cx<T>* tab_lds, const cx<T>* __restrict__ tw, int j) {
+        for (int i = threadIdx.x; i < L; i += blockDim.x) tab_lds[i] = tw[(i % E) * (i / E)];
+        p = tab_lds + j;
+    }
+    __device__ __forceinline__ cx<T> w(int a, int n1) const { return p[J * a + E * n1]; }
+};
+
+// Step 2 of a transform: lane j reads Y[n1][j + J*a] at rd[n1*RS + J*a*CS] (RS, CS: the LDS
+// strides of n1 and of one k2), twiddles, DFT_J over n1, and keeps X[k2 + E*k1] in
+// v[a + (E/J)*k1].
+template <typename T, int J, int E, int RS, int CS, bool INV>
+__device__ __forceinline__ void reg_step2(cx<T>* v, const cx<T>* rd, const LaneTw<T, J, E>& tw) {
+    constexpr int Q = E / J;
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        cx<T> s[J];
+#pragma unroll
+        for (int n1 = 0; n1 < J; ++n1) {
+            cx<T> x = rd[n1 * RS + J * a * CS];
+            if (n1 > 0) x = cmul(x, INV ? conjv(tw.w(a, n1)) : tw.w(a, n1));
+            s[n1] = x;
+        }
+        dftN<T, J, INV>(s);
+#pragma unroll
+        for (int k1 = 0; k1 < J; ++k1) v[a + Q * k1] = s[k1];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Contiguous-axis transforms of a row pair (a + i b), lanes j of transform w in a workgroup.
+// LDS per transform: LP = L + E + 1 complex.
+// ------------------------------------------------------------------------------------------
+template <typename T, int J, int E>
+struct RegRow {
+    static_assert(E % J == 0 && E % 2 == 0, "lanes per transform must divide the (even) elements per lane");
+    static constexpr int L = J * E;
+    static constexpr int LP = L + E + 1;
+    // padded LDS slot of element n is n + n / J; for the access pattern used (n = j + J*m):
+    // j + (J+1)*m. Step 1 writes Y[n1][k2] at n1*(E+1) + k2.
+    __device__ static __forceinline__ int pidx(int n) { return n + n / J; }
+
+    // Two-step transform of v (lane-natural in, lane-natural out). lds: this transform's slab.
+    template <bool INV>
+    __device__ static __forceinline__ void transform(cx<T>* v, cx<T>* lds, int j, const LaneTw<T, J, E>& tw) {
+        dftN<T, E, INV>(v);
+        cx<T>* wr = lds + j * (E + 1);
+#pragma unroll
+        for (int r = 0; r < E; ++r) wr[r] = v[r];
+        __syncthreads();
+        reg_step2<T, J, E, E + 1, 1, INV>(v, lds + j, tw);
+        __syncthreads();
+    }
+
+    // Load the Hermitian-extended Z = A + iB of rows (ra, rb) in the lane-natural pattern.
+    // Elements n = j + J m < L/2 read k = n from base (j); n >= L/2 read k = L - n from base
+    // (L - j) with negative compile-time offsets, so every load is base + immediate.
+    __device__ static __forceinline__ void load_spectrum(const cx<T>* __restrict__ spec, int ra, int rb, int nrows,
+                                                         int mzh, int j, cx<T>* v) {
+        const T ma = ra < nrows ? T(1) : T(0), mb = rb < nrows ? T(1) : T(0);
+        const cx<T>* sa = spec + (int64_t)(ra < nrows ? ra : 0) * mzh;
+        const cx<T>* sb = spec + (int64_t)(rb < nrows ? rb : 0) * mzh;
+        const cx<T>* la = sa + j;
+        const cx<T>* lb = sb + j;
+        const cx<T>* ha = sa + (L - j);
+        const cx<T>* hb = sb + (L - j);
+#pragma unroll
+        for (int m = 0; m < E / 2; ++m) {
+            const cx<T> A = la[J * m] * ma;
+            const cx<T> B = lb[J * m] * mb;
+            v[m] = cx<T>{A.x - B.y, A.y + B.x};
+        }
+#pragma unroll
+        for (int m = E / 2; m < E; ++m) {
+            const cx<T> A = ha[-J * m] * ma;
+            const cx<T> B = hb[-J * m] * mb;
+            v[m] = cx<T>{A.x + B.y, B.x - A.y};
+        }
+        if (j == 0) {   // DC (n = 0) and Nyquist (n = L/2) bins are real parts only
+            v[0] = cx<T>{sa[0].x * ma, sb[0].x * mb};
+            v[E / 2] = cx<T>{sa[L / 2].x * ma, sb[L / 2].x * mb};
+        }
+    }
+
+    // Forward of v (a in .x, b in .y), split into half spectra, stored to rows (ra, rb).
+    __device__ static __forceinline__ void forward_store(cx<T>* spec, int ra, int rb, int nrows, int mzh, cx<T>* lds,
+                                                         int j, const LaneTw<T, J, E>& tw, cx<T>* v) {
+        transform<false>(v, lds, j, tw);
+        cx<T>* wr = lds + j;
+#pragma unroll
+        for (int m = 0; m < E; ++m) wr[(J + 1) * m] = v[m];
+        __syncthreads();
+        const bool va = ra < nrows, vb = rb < nrows;
+        // k = j + J m: slot j + (J+1) m; mirror L - k: slot (L - j + E - [j>0]) - (J+1) m (k > 0)
+        const cx<T>* pk = lds + j;
+        const cx<T>* pm = lds + (L - j + E - (j > 0 ? 1 : 0));
+        cx<T>* oa = spec + (int64_t)(va ? ra : 0) * mzh + j;
+        cx<T>* ob = spec + (int64_t)(vb ? rb : 0) * mzh + j;
+#pragma unroll
+        for (int m = 0; m <= E / 2; ++m) {
+            const int k = j + J * m;
+            if (k > L / 2) break;
+            const cx<T> zk = pk[(J + 1) * m];
+            const cx<T> zm = k == 0 ? zk : pm[-(J + 1) * m];
+            if (va) oa[J * m] = cx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+            if (vb) ob[J * m] = cx<T>{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+        }
+        __syncthreads();
+    }
+};
+
+// Row pair handled by a lane group: P transforms per workgroup, J lanes each.
+template <int J>
+struct RegZPos {
+    int w, j, q, ra, rb;
+    __device__ RegZPos(int P, const BIdx& bi) {
+        const int tid = threadIdx.x;
+        w = tid / J;
+        j = tid - w * J;
+        const int tile = bi.tile();
+        q = tile * P + w;
+        ra = 2 * q;
+        rb = ra + 1;
+    }
+};
+
+// Sparse sources on register-resident rows: go through LDS (natural order) when the tile has any.
+template <typename T, int J, int E>
+__device__ void reg_apply_sparse(const SparseSrc<T>& s, const Geo& g, int row0, int nrow_tile, cx<T>* lds_all,
+                                 int w, int j, int t, cx<T>* v) {
+    using S = RegRow<T, J, E>;
+    cx<T>* lds = lds_all + w * S::LP;
+#pragma unroll
+    for (int m = 0; m < E; ++m) lds[S::pidx(j + J * m)] = v[m];
+    __syncthreads();
+    if (t < s.len) {
+        const int r1 = min(g.nrows, row0 + nrow_tile);
+        const int e0 = s.rowptr[row0], e1 = s.rowptr[r1];
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+            const int64_t cell = s.index[e];
+            const int row = (int)(cell / g.m2);
+            const int n = (int)(cell - (int64_t)row * g.m2);
+            const int lr = row - row0;
+            const T val = s.signal[(int64_t)(s.rows == 1 ? 0 : e) * s.len + t] * s.base[e] * s.factor;
+            T& slot = comp(lds_all[(lr >> 1) * S::LP + S::pidx(n)], lr & 1);
+            slot = s.mode == 0 ? val : slot + val;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = lds[S::pidx(j + J * m)];
+    __syncthreads();
+}
+
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_z_forward_real_body(const BIdx& bi, const T* __restrict__ src, cx<T>* spec,
+                                                        T scale, const Geo& g, int P, const AxisTabs<T>& tab) {
+    using S = RegRow<T, J, E>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds = ldsb + S::L;
+    RegZPos<J> ps(P, bi);
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    cx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int n = ps.j + J * m;
+        v[m].x = va ? src[(int64_t)ps.ra * S::L + n] * scale : T(0);
+        v[m].y = vb ? src[(int64_t)ps.rb * S::L + n] * scale : T(0);
+    }
+    S::forward_store(spec, ps.ra, ps.rb, g.nrows, g.mzh, lds + ps.w * S::LP, ps.j, lt, v);
+}
+template <typename T, int J, int E>
+__global__ void __launch_bounds__(256) reg_z_forward_real(const T* __restrict__ src, cx<T>* spec, T scale, Geo g,
+                                                          int P, AxisTabs<T> tab) {
+    reg_z_forward_real_body<T, J, E>(bidx_here(), src, spec, scale, g, P, tab);
+}
+
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_z_inverse_real_body(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst,
+                                                        T scale, int accumulate, const Geo& g, int P,
+                                                        const AxisTabs<T>& tab) {
+    using S = RegRow<T, J, E>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds = ldsb + S::L;
+    RegZPos<J> ps(P, bi);
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    cx<T> v[E];
+    S::load_spectrum(spec, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+    S::template transform<true>(v, lds + ps.w * S::LP, ps.j, lt);
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int n = ps.j + J * m;
+        if (va) {
+            const int64_t c = (int64_t)ps.ra * S::L + n;
+            dst[c] = accumulate ? dst[c] + v[m].x * scale : v[m].x * scale;
+        }
+        if (vb) {
+            const int64_t c = (int64_t)ps.rb * S::L + n;
+            dst[c] = accumulate ? dst[c] + v[m].y * scale : v[m].y * scale;
+        }
+    }
+}
+template <typename T, int J, int E>
+__global__ void __launch_bounds__(256) reg_z_inverse_real(const cx<T>* spec, T* __restrict__ dst, T scale,
+                                                          int accumulate, Geo g, int P, AxisTabs<T> tab) {
+    reg_z_inverse_real_body<T, J, E>(bidx_here(), spec, dst, scale, accumulate, g, P, tab);
+}
+
+// Collocated velocity along the contiguous axis for a list of rows: dst[row] =
+// C2R_z(coll_z * Z[row]) / L, Z = R2C_z(u) of the updated component (as the velocity kernel
+// leaves it). The shift e^{-i k dz/2} only involves the contiguous axis, so the full 3-D
+// transform of the reference (KSpaceFirstOrder.swift:735-745) reduces to this 1-D one on the
+// rows the sensor reads. Rows are paired in list order; an odd tail pairs with nothing.
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_z_colloc_rows_body(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst,
+                                                       const int* __restrict__ rows, int nlist, const Geo& g, int P,
+                                                       const AxisTabs<T>& tab) {
+    using S = RegRow<T, J, E>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    RegZPos<J> ps(P, bi);
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    const bool va = ps.ra < nlist, vb = ps.rb < nlist;
+    const int ra = va ? rows[ps.ra] : g.nrows;   // g.nrows: masked by load_spectrum
+    const int rb = vb ? rows[ps.rb] : g.nrows;
+    cx<T> v[E];
+    S::load_spectrum(spec, ra, rb, g.nrows, g.mzh, ps.j, v);
+    const cx<T>* co = tab.coll[2] + ps.j;
+    const T scale = T(1) / T(S::L);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const cx<T> c = co[J * m];
+        v[m] = cmul(v[m], cx<T>{c.x * scale, c.y * scale});
+    }
+    S::template transform<true>(v, ldsb + S::L + ps.w * S::LP, ps.j, lt);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int n = ps.j + J * m;
+        if (va) dst[(int64_t)ra * S::L + n] = v[m].x;
+        if (vb) dst[(int64_t)rb * S::L + n] = v[m].y;
+    }
+}
+template <typename T, int J, int E>
+__global__ void __launch_bounds__(256) reg_z_colloc_rows(const cx<T>* spec, T* __restrict__ dst,
+                                                         const int* __restrict__ rows, int nlist, Geo g, int P,
+                                                         AxisTabs<T> tab) {
+    reg_z_colloc_rows_body<T, J, E>(bidx_here(), spec, dst, rows, nlist, g, P, tab);
+}
+
+// Scheduling fence: keeps the compiler from hoisting the loads of later element groups above
+// earlier ones (bounds live registers; other waves hide the latency).
+#define KWH_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+constexpr int kGroup = 8;      // density update
+constexpr int kVelGroup = 4;   // velocity update (keeps it within the three-wave register budget)
+
+// Velocity update, per axis: C2R_z(G_a) -> u_a = pml_sg (pml_sg u_a - dt/rho_a g) [+ sources] -> R2C_z.
+template <typename T, int J, int E, bool CORR>
+__device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArgs<T>& a, const Geo& g, int P,
+                                                    const AxisTabs<T>& tab) {
+    using S = RegRow<T, J, E>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds_all = ldsb + S::L;
+    RegZPos<J> ps(P, bi);
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    cx<T>* lds = lds_all + ps.w * S::LP;
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+    for (int ax = 0; ax < g.dim; ++ax) {
+        const int ia = ax + g.off;
+        cx<T> v[E];
+        S::load_spectrum(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        S::template transform<true>(v, lds, ps.j, lt);
+        KWH_SCHED_FENCE();
+        T* u = a.u[ax];
+        const SMap<T> dtr = a.dtr[ax];
+        const SMap<T> corr = a.corr[ax];
+        const T* pm = tab.pmlsg[ia];
+        const T* ua = u + oa;
+        const T* ub = u + ob;
+        const T* dta = dtr.lane(oa, ps.j);
+        const T* dtb = dtr.lane(ob, ps.j);
+        const T* cra = corr.lane(oa, ps.j);
+        const T* crb = corr.lane(ob, ps.j);
+        auto upd = [&](int m, T qa, T qb) {
+            T xa = qa * (qa * ua[J * m] - dta[J * m] * v[m].x);
+            T xb = qb * (qb * ub[J * m] - dtb[J * m] * v[m].y);
+            if constexpr (CORR) {
+                xa += cra[J * m];
+                xb += crb[J * m];
+            }
+            v[m].x = va ? xa : T(0);
+            v[m].y = vb ? xb : T(0);
+        };
+        if (ia == 2) {   // staggered PML along z varies with n and is shared by both rows
+            const T* pz = pm + ps.j;
+#pragma unroll
+            for (int g0 = 0; g0 < E; g0 += kVelGroup) {
+#pragma unroll
+                for (int m = g0; m < g0 + kVelGroup && m < E; ++m) {
+                    const T q = pz[J * m];
+                    upd(m, q, q);
+                }
+                KWH_SCHED_FENCE();
+            }
+        } else {
+            const T qa = pm[axis_pos(g, ra, 0, ia)], qb = pm[axis_pos(g, rb, 0, ia)];
+#pragma unroll
+            for (int g0 = 0; g0 < E; g0 += kVelGroup) {
+#pragma unroll
+                for (int m = g0; m < g0 + kVelGroup && m < E; ++m) upd(m, qa, qb);
+                KWH_SCHED_FENCE();
+            }
+        }
+        if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{(int)(bi.tile() * P), P, 0, 0}))
+            reg_apply_sparse<T, J, E>(a.sp[ax], g, 2 * bi.tile() * P, 2 * P, lds_all, ps.w, ps.j, a.ctr[0], v);
+        {
+            T* wa = u + oa;
+            T* wb = u + ob;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                if (va) wa[J * m] = v[m].x;
+                if (vb) wb[J * m] = v[m].y;
+            }
+        }
+        S::forward_store(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+    }
+}
+template <typename T, int J, int E, bool CORR>
+__global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_VELOCITY_WAVES)))
+    reg_z_velocity(VelArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    reg_z_velocity_body<T, J, E, CORR>(bidx_here(), a, g, P, tab);
+}
+
+// Density update, per axis: C2R_z(D_a) -> rho_a = pml (pml rho_a - dt rho0 [nl] du_a) [+ sources],
+// followed by the EOS (MODE 0), the EOS with Stokes absorption (MODE 2) or the inputs of the
+// FFT absorption term (MODE 1: rhoT, F1 = R2C_z(rho0 sum_a du_a), F2 = R2C_z(rhoT)). NL: B/A.
+// The total density and, for MODE 1/2, rho0 * sum_a du_a are accumulated in registers while the
+// axes are processed, so each divergence spectrum is read once.
+template <typename T, int J, int E, bool NL, int MODE, bool CORR>
+constexpr int density_waves() {
+    return (NL || CORR) ? 1 : reg_waves<T, E>(MODE == 0 ? KWH_DENSITY_WAVES : 2);
+}
+template <typename T, int J, int E, bool NL, int MODE, bool CORR>
+__device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P,
+                                                   const AxisTabs<T>& tab) {
+    using S = RegRow<T, J, E>;
+    constexpr bool ACC = MODE != 0;   // accumulate rho0 * sum_a du_a
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds_all = ldsb + S::L;
+    RegZPos<J> ps(P, bi);
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    cx<T>* lds = lds_all + ps.w * S::LP;
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    const int row0 = 2 * bi.tile() * P;
+    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+    const T* r0pa = a.rho0.lane(oa, ps.j);
+    const T* r0pb = a.rho0.lane(ob, ps.j);
+    cx<T> rt[E];
+    cx<T> sd[ACC ? E : 1];
+    cx<T> rp[NL ? E : 1];
+    if constexpr (NL) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            T sa = 0, sb = 0;
+            for (int ax = 0; ax < g.dim; ++ax) {
+                sa += a.rho[ax][oa + J * m];
+                sb += a.rho[ax][ob + J * m];
+            }
+            rp[m] = cx<T>{sa, sb};
+        }
+    }
+    for (int ax = 0; ax < g.dim; ++ax) {
+        const int ia = ax + g.off;
+        cx<T> v[E];
+        S::load_spectrum(a.D[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        S::template transform<true>(v, lds, ps.j, lt);
+        KWH_SCHED_FENCE();
+        T* rho = a.rho[ax];
+        const T* pm = tab.pml[ia];
+        const T cf = a.corr_factor[ax];
+        const T* rha = rho + oa;
+        const T* rhb = rho + ob;
+        const T* cra = a.corr.lane(oa, ps.j);
+        const T* crb = a.corr.lane(ob, ps.j);
+        auto upd = [&](int m, T qa, T qb) {
+            const T r0a = r0pa[J * m];
+            const T r0b = r0pb[J * m];
+            const T da = r0a * v[m].x, db = r0b * v[m].y;
+            if constexpr (ACC) {
+                if (ax == 0) sd[m] = cx<T>{da, db};
+                else { sd[m].x += da; sd[m].y += db; }
+            }
+            T ma = a.dt * da, mb = a.dt * db;
+            if constexpr (NL) {
+                ma *= (T(2) * rp[m].x + r0a) / r0a;
+                mb *= (T(2) * rp[m].y + r0b) / r0b;
+            }
+            T xa = qa * (qa * rha[J * m] - ma);
+            T xb = qb * (qb * rhb[J * m] - mb);
+            if constexpr (CORR) {
+                xa += cra[J * m] * cf;
+                xb += crb[J * m] * cf;
+            }
+            v[m] = cx<T>{xa, xb};
+        };
+        if (ia == 2) {
+            const T* pz = pm + ps.j;
+#pragma unroll
+            for (int g0 = 0; g0 < E; g0 += kGroup) {
+#pragma unroll
+                for (int m = g0; m < g0 + kGroup && m < E; ++m) {
+                    const T q = pz[J * m];
+                    upd(m, q, q);
+                }
+                KWH_SCHED_FENCE();
+            }
+        } else {
+            const T qa = pm[axis_pos(g, ra, 0, ia)], qb = pm[axis_pos(g, rb, 0, ia)];
+#pragma unroll
+            for (int g0 = 0; g0 < E; g0 += kGroup) {
+#pragma unroll
+                for (int m = g0; m < g0 + kGroup && m < E; ++m) upd(m, qa, qb);
+                KWH_SCHED_FENCE();
+            }
+        }
+        if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{row0 / 2, P, 0, 0}))
+            reg_apply_sparse<T, J, E>(a.sp[ax], g, row0, 2 * P, lds_all, ps.w, ps.j, a.ctr[0], v);
+        T* wa = rho + oa;
+        T* wb = rho + ob;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            if (va) wa[J * m] = v[m].x;
+            if (vb) wb[J * m] = v[m].y;
+            if (ax == 0) rt[m] = v[m];
+            else { rt[m].x += v[m].x; rt[m].y += v[m].y; }
+        }
+    }
+    cx<T> v[E];
+    if constexpr (MODE != 1) {
+        const T* tpa = a.tau.lane(oa, ps.j);
+        const T* tpb = a.tau.lane(ob, ps.j);
+        const T* bpa = a.bona.lane(oa, ps.j);
+        const T* bpb = a.bona.lane(ob, ps.j);
+        const T* cpa = a.c0.lane(oa, ps.j);
+        const T* cpb = a.c0.lane(ob, ps.j);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            T ea = rt[m].x, eb = rt[m].y;
+            const T r0a = r0pa[J * m];
+            const T r0b = r0pb[J * m];
+            if constexpr (MODE == 2) {
+                ea += tpa[J * m] * sd[m].x;
+                eb += tpb[J * m] * sd[m].y;
+            }
+            if constexpr (NL) {
+                ea += bpa[J * m] * (rt[m].x * rt[m].x) / (T(2) * r0a);
+                eb += bpb[J * m] * (rt[m].y * rt[m].y) / (T(2) * r0b);
+            }
+            const T cA = cpa[J * m];
+            const T cB = cpb[J * m];
+            const T pA = cA * cA * ea, pB = cB * cB * eb;
+            if (va) a.p[oa + J * m] = pA;
+            if (vb) a.p[ob + J * m] = pB;
+            v[m] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
+        }
+        S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cx<T>{va ? sd[m].x : T(0), vb ? sd[m].y : T(0)};
+        S::forward_store(a.F1, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            if (va) a.rhoT[oa + J * m] = rt[m].x;
+            if (vb) a.rhoT[ob + J * m] = rt[m].y;
+            rt[m] = cx<T>{va ? rt[m].x : T(0), vb ? rt[m].y : T(0)};
+        }
+        if (a.dispersion) S::forward_store(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, rt);
+    }
+}
+template <typename T, int J, int E, bool NL, int MODE, bool CORR>
+__global__ void __launch_bounds__(256, (density_waves<T, J, E, NL, MODE, CORR>()))
+    reg_z_density(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    reg_z_density_body<T, J, E, NL, MODE, CORR>(bidx_here(), a, g, P, tab);
+}
+
+// EOS with FFT absorption: p = c^2 (rhoT + tau C2R_z(A1) - eta C2R_z(A2) [+ B/A rhoT^2 / (2 rho0)]),
+// P = R2C_z(p). Both spectra are loaded before the first transform (one memory phase).
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>& a, const Geo& g, int P,
+                                               const AxisTabs<T>& tab) {
+    using S = RegRow<T, J, E>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    RegZPos<J> ps(P, bi);
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    cx<T>* lds = ldsb + S::L + ps.w * S::LP;
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    cx<T> e[E], v[E];
+    S::load_spectrum(a.A1, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, e);
+    if (a.A2) S::load_spectrum(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+    S::template transform<true>(e, lds, ps.j, lt);
+    if (a.A2) S::template transform<true>(v, lds, ps.j, lt);
+    KWH_SCHED_FENCE();
+    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+    const T* cpa = a.c0.lane(oa, ps.j);
+    const T* cpb = a.c0.lane(ob, ps.j);
+    const T* tpa = a.tau.lane(oa, ps.j);
+    const T* tpb = a.tau.lane(ob, ps.j);
+    const T* epa = a.eta.lane(oa, ps.j);
+    const T* epb = a.eta.lane(ob, ps.j);
+    const T* bpa = a.bona.lane(oa, ps.j);
+    const T* bpb = a.bona.lane(ob, ps.j);
+    const T* rpa = a.rho0.lane(oa, ps.j);
+    const T* rpb = a.rho0.lane(ob, ps.j);
+#pragma unroll
+    for (int g0 = 0; g0 < E; g0 += kGroup) {
+#pragma unroll
+        for (int m = g0; m < g0 + kGroup && m < E; ++m) {
+            const int64_t ca = oa + J * m, cb = ob + J * m;
+            const T rta = a.rhoT[ca], rtb = a.rhoT[cb];
+            const T cA = cpa[J * m], cB = cpb[J * m];
+            T tauA, etaA, tauB, etaB;
+            if (a.tau_eta_from_c) {
+                const T la = log2(cA), lb = log2(cB);
+                tauA = a.tk * exp2(a.te * la);
+                tauB = a.tk * exp2(a.te * lb);
+                etaA = a.ek * exp2(a.ee * la);
+                etaB = a.ek * exp2(a.ee * lb);
+            } else {
+                tauA = tpa[J * m];
+                tauB = tpb[J * m];
+                etaA = epa[J * m];
+                etaB = epb[J * m];
+            }
+            T ea = rta + tauA * e[m].x;
+            T eb = rtb + tauB * e[m].y;
+            if (a.A2) {
+                ea -= etaA * v[m].x;
+                eb -= etaB * v[m].y;
+            }
+            if (a.nonlinear) {
+                ea += bpa[J * m] * (rta * rta) / (T(2) * rpa[J * m]);
+                eb += bpb[J * m] * (rtb * rtb) / (T(2) * rpb[J * m]);
+            }
+            const T pA = cA * cA * ea, pB = cB * cB * eb;
+            if (va) a.p[ca] = pA;
+            if (vb) a.p[cb] = pB;
+            e[m] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
+        }
+        KWH_SCHED_FENCE();
+    }
+    S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, e);
+}
+template <typename T, int J, int E>
+__global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_EOS_WAVES)))
+    reg_z_eos(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    reg_z_eos_body<T, J, E>(bidx_here(), a, g, P, tab);
+}
+
+// ------------------------------------------------------------------------------------------
+// Strided / middle passes over a tile of NTR contiguous columns: lane (w, j), w fastest.
+// LDS: element (n1, k2) of step 1 of column w at (n1*E + k2)*NTR + w + n1 (pad one slot per E
+// rows), after the L-element twiddle table.
+// ------------------------------------------------------------------------------------------
+template <typename T, int J, int E, int NTR>
+struct RegCol {
+    static constexpr int L = J * E;
+    static constexpr size_t lds_elems() { return (size_t)L * NTR + J + L; }
+    //   step 1 write (n1 = j, k2 = r) -> (j*E*NTR + w + j) + r*NTR
+    //   step 2 read  (n1, k2 = j + J*a) -> (j*NTR + w) + n1*(E*NTR + 1) + J*a*NTR
+    template <bool INV>
+    __device__ static __forceinline__ void transform(cx<T>* v, cx<T>* lds, int j, int w, const LaneTw<T, J, E>& tw) {
+        dftN<T, E, INV>(v);
+        cx<T>* wr = lds + (j * E * NTR + w + j);
+#pragma unroll
+        for (int r = 0; r < E; ++r) wr[r * NTR] = v[r];
+        __syncthreads();
+        reg_step2<T, J, E, E * NTR + 1, NTR, INV>(v, lds + (j * NTR + w), tw);
+        __syncthreads();
+    }
+};
+
+// Largest tile width (columns per workgroup) of the column kernels: 256 lanes in FP32 (128 in
+// FP64), halved until the LDS slab and twiddle table fit 64 KiB.
+template <typename T, int J, int E>
+constexpr int reg_ntr_max() {
+    int n = (sizeof(T) == 8 ? 128 : 256) / J;
+    while (n > 1 && ((size_t)J * E * n + J + J * E) * sizeof(cx<T>) > 65536) n /= 2;
+    return n;
+}
+
+template <typename T, int J, int E, int NTR, bool INV>
+__device__ __forceinline__ void reg_strided_body(const BIdx& bi, const StridedArgs& a, const cx<T>* __restrict__ tw) {
+    using S = RegCol<T, J, E, NTR>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds = ldsb + S::L;
+    const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tw, j);
+    const int tile = bi.tile();
+    const int64_t c = (int64_t)tile * NTR + w;
+    const bool valid = c < a.ncols;
+    const int64_t cc = valid ? c : 0;
+    const int64_t base = (cc / a.inner) * (int64_t)S::L * a.inner + cc % a.inner;
+    cx<T>* data = reinterpret_cast<cx<T>*>(a.fields[bi.y]) + base + (int64_t)j * a.stride;
+    const int64_t step = (int64_t)J * a.stride;
+    cx<T> v[E];
+    {
+        const cx<T>* q = data;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            v[m] = *q;
+            q += step;
+        }
+    }
+    S::template transform<INV>(v, lds, j, w, lt);
+    if (valid) {
+        cx<T>* q = data;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            *q = v[m];
+            q += step;
+        }
+    }
+}
+template <typename T, int J, int E, int NTR, bool INV>
+__global__ void __launch_bounds__(256) reg_strided(StridedArgs a, const cx<T>* __restrict__ tw) {
+    reg_strided_body<T, J, E, NTR, INV>(bidx_here(), a, tw);
+}
+
+// Middle pass: forward along the middle axis, operators, inverse, store. Every table-using
+// operator of one input reads the same real k-space table (checked on the host), loaded once.
+template <typename T, int J, int E, int NTR>
+__device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a, const AxisTabs<T>& tab, const Geo& g) {
+    using S = RegCol<T, J, E, NTR>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* lds = ldsb + S::L;
+    const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[a.axis], j);
+    const int tile = bi.tile();
+    const int64_t c = (int64_t)tile * NTR + w;
+    const bool valid = c < a.ncols;
+    const int64_t cc = valid ? c : 0;
+    const int64_t base = (cc / a.inner) * (int64_t)S::L * a.inner + cc % a.inner + (int64_t)j * a.stride;
+    const int64_t step = (int64_t)J * a.stride;
+    const T scale = T(a.scale);
+    int i0c, i1c, i2c;
+    if (a.axis == 0) { i0c = 0; i1c = (int)(cc / g.mzh); i2c = (int)(cc % g.mzh); }
+    else { i0c = (int)(cc / g.mzh); i1c = 0; i2c = (int)(cc % g.mzh); }
+    const int64_t hstride = a.axis == 0 ? (int64_t)g.m1 * g.mzh : (int64_t)g.mzh;
+    const int64_t hbase = (a.axis == 0 ? cc : (int64_t)i0c * g.m1 * g.mzh + i2c) + (int64_t)j * hstride;
+    const int64_t hstep = (int64_t)J * hstride;
+    T kap[E];
+    const T* kap_src = nullptr;   // table currently held in kap (inputs sharing one table load it once)
+    for (int f = 0; f < a.nin; ++f) {
+        cx<T> X[E];
+        {
+            const cx<T>* q = reinterpret_cast<const cx<T>*>(a.in[f]) + base;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                X[m] = *q;
+                q += step;
+            }
+        }
+        S::template transform<false>(X, lds, j, w, lt);
+        const T* ktab = nullptr;
+        for (int o = 0; o < a.nop; ++o)
+            if (a.op[o].in == f && a.op[o].table) { ktab = reinterpret_cast<const T*>(a.op[o].table); break; }
+        if (ktab && ktab != kap_src) {
+            const T* q = ktab + hbase;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                kap[m] = *q * scale;
+                q += hstep;
+            }
+            kap_src = ktab;
+        }
+        for (int o = 0; o < a.nop; ++o) {
+            const MidOp op = a.op[o];
+            if (op.in != f) continue;
+            const bool use_kappa = op.table != nullptr;
+            const bool use_vec = op.kind != OP_TAB;
+            const cx<T>* tab1 = op.kind == OP_DDXP_KAPPA || op.kind == OP_DDXP ? tab.ddxp[op.axis]
+                              : op.kind == OP_DDXN_KAPPA ? tab.ddxn[op.axis] : tab.coll[op.axis];
+            const bool vec_on_mid = op.axis == a.axis;
+            const int fixed_idx = op.axis == 0 ? i0c : (op.axis == 1 ? i1c : i2c);
+            const cx<T> fixed_vec = use_vec && !vec_on_mid ? tab1[fixed_idx] : cx<T>{T(1), T(0)};
+            const cx<T>* tj = tab1 + j;
+            cx<T> v[E];
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                cx<T> mv = use_vec ? (vec_on_mid ? tj[J * m] : fixed_vec) : cx<T>{T(1), T(0)};
+                const T s = use_kappa ? kap[m] : scale;
+                v[m] = cmul(X[m], cx<T>{mv.x * s, mv.y * s});
+            }
+            S::template transform<true>(v, lds, j, w, lt);
+            if (valid) {
+                cx<T>* q = reinterpret_cast<cx<T>*>(op.out) + base;
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    *q = v[m];
+                    q += step;
+                }
+            }
+        }
+    }
+}
+template <typename T, int J, int E, int NTR>
+__global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_MIDDLE_WAVES)))
+    reg_middle(MidArgs a, AxisTabs<T> tab, Geo g) {
+    reg_middle_body<T, J, E, NTR>(bidx_here(), a, tab, g);
+}
+
+}  // namespace kwh

@@ -1,0 +1,92 @@
+// kwh_regops_impl.h -- builds the RegOps table of one register-resident length (L = J * E) for
+// one working type. Included only by the per-length translation units kwh_reg_<L>.hip.
+#pragma once
+#include "kwh_reg.h"
+#include "kwh_regops.h"
+
+namespace kwh {
+
+template <typename T, int J, int E>
+struct RegImpl {
+    static constexpr int NTR = reg_ntr_max<T, J, E>();
+    static_assert(NTR >= 2, "column tile too narrow");
+
+    template <bool CORR>
+    using VelB = Body<&reg_z_velocity_body<T, J, E, CORR>, &reg_z_velocity<T, J, E, CORR>,
+                      reg_waves<T, E>(KWH_VELOCITY_WAVES), false>;
+    template <bool NL, int MODE, bool CORR>
+    using DenB = Body<&reg_z_density_body<T, J, E, NL, MODE, CORR>, &reg_z_density<T, J, E, NL, MODE, CORR>,
+                      density_waves<T, J, E, NL, MODE, CORR>(), false>;
+    template <int N, bool INV>
+    using StrB = Body<&reg_strided_body<T, J, E, N, INV>, &reg_strided<T, J, E, N, INV>, 1, false>;
+    template <int N>
+    using MidB = Body<&reg_middle_body<T, J, E, N>, &reg_middle<T, J, E, N>, reg_waves<T, E>(KWH_MIDDLE_WAVES), false>;
+
+    static void z_forward_real(const Emit& em, dim3 gr, dim3 bl, size_t lds, const T* src, cx<T>* spec, T scale,
+                               const Geo& g, int P, const AxisTabs<T>& tab) {
+        em.go<Body<&reg_z_forward_real_body<T, J, E>, &reg_z_forward_real<T, J, E>, 1, false>>(gr, bl, lds, src, spec,
+                                                                                               scale, g, P, tab);
+    }
+    static void z_inverse_real(const Emit& em, dim3 gr, dim3 bl, size_t lds, const cx<T>* spec, T* dst, T scale,
+                               int accumulate, const Geo& g, int P, const AxisTabs<T>& tab) {
+        em.go<Body<&reg_z_inverse_real_body<T, J, E>, &reg_z_inverse_real<T, J, E>, 1, false>>(
+            gr, bl, lds, spec, dst, scale, accumulate, g, P, tab);
+    }
+    static void z_colloc_rows(const Emit& em, dim3 gr, dim3 bl, size_t lds, const cx<T>* spec, T* dst,
+                              const int* rows, int nlist, const Geo& g, int P, const AxisTabs<T>& tab) {
+        em.go<Body<&reg_z_colloc_rows_body<T, J, E>, &reg_z_colloc_rows<T, J, E>, 1, false>>(
+            gr, bl, lds, spec, dst, rows, nlist, g, P, tab);
+    }
+    static void z_velocity(const Emit& em, dim3 gr, dim3 bl, size_t lds, const VelArgs<T>& va, const Geo& g, int P,
+                           const AxisTabs<T>& tab, bool corr) {
+        if (corr) em.go<VelB<true>>(gr, bl, lds, va, g, P, tab);
+        else em.go<VelB<false>>(gr, bl, lds, va, g, P, tab);
+    }
+    template <bool NL>
+    static void density_nl(const Emit& em, dim3 gr, dim3 bl, size_t lds, const RhoArgs<T>& ra, const Geo& g, int P,
+                           const AxisTabs<T>& tab, int mode, bool corr) {
+        if (mode == 1) {
+            if (corr) em.go<DenB<NL, 1, true>>(gr, bl, lds, ra, g, P, tab);
+            else em.go<DenB<NL, 1, false>>(gr, bl, lds, ra, g, P, tab);
+        } else if (mode == 2) {
+            if (corr) em.go<DenB<NL, 2, true>>(gr, bl, lds, ra, g, P, tab);
+            else em.go<DenB<NL, 2, false>>(gr, bl, lds, ra, g, P, tab);
+        } else {
+            if (corr) em.go<DenB<NL, 0, true>>(gr, bl, lds, ra, g, P, tab);
+            else em.go<DenB<NL, 0, false>>(gr, bl, lds, ra, g, P, tab);
+        }
+    }
+    static void z_density(const Emit& em, dim3 gr, dim3 bl, size_t lds, const RhoArgs<T>& ra, const Geo& g, int P,
+                          const AxisTabs<T>& tab, bool nonlinear, int mode, bool corr) {
+        if (nonlinear) density_nl<true>(em, gr, bl, lds, ra, g, P, tab, mode, corr);
+        else density_nl<false>(em, gr, bl, lds, ra, g, P, tab, mode, corr);
+    }
+    static void z_eos(const Emit& em, dim3 gr, dim3 bl, size_t lds, const EosArgs<T>& ea, const Geo& g, int P,
+                      const AxisTabs<T>& tab) {
+        em.go<Body<&reg_z_eos_body<T, J, E>, &reg_z_eos<T, J, E>, reg_waves<T, E>(KWH_EOS_WAVES), false>>(
+            gr, bl, lds, ea, g, P, tab);
+    }
+    static void strided(const Emit& em, dim3 gr, dim3 bl, size_t lds, const StridedArgs& a, const cx<T>* tw, bool inv,
+                        int ntr) {
+        if (ntr == NTR) {
+            if (inv) em.go<StrB<NTR, true>>(gr, bl, lds, a, tw);
+            else em.go<StrB<NTR, false>>(gr, bl, lds, a, tw);
+        } else {
+            if (inv) em.go<StrB<NTR / 2, true>>(gr, bl, lds, a, tw);
+            else em.go<StrB<NTR / 2, false>>(gr, bl, lds, a, tw);
+        }
+    }
+    static void middle(const Emit& em, dim3 gr, dim3 bl, size_t lds, const MidArgs& a, const AxisTabs<T>& tab,
+                       const Geo& g, int ntr) {
+        if (ntr == NTR) em.go<MidB<NTR>>(gr, bl, lds, a, tab, g);
+        else em.go<MidB<NTR / 2>>(gr, bl, lds, a, tab, g);
+    }
+
+    static const RegOps<T>* ops() {
+        static const RegOps<T> o{J, E, NTR, &z_forward_real, &z_inverse_real, &z_colloc_rows, &z_velocity,
+                                 &z_density, &z_eos, &strided, &middle};
+        return &o;
+    }
+};
+
+}  // namespace kwh

@@ -21,9 +21,10 @@
 #include "../../include/kwave_hip.h"
 #include "kwh_kernels.h"
 #include "kwh_setup.h"
-#include "kwh_sq.h"
 #include "kwh_adjoint.h"
 #include "kwh_batch.h"
+#include "kwh_launch.h"
+#include "kwh_regops.h"
 
 using namespace kwh;
 
@@ -141,26 +142,6 @@ int grid_blocks(int64_t n, int threads = 256) {
 struct kwh_ctx {
     int device;
 };
-
-// One recorded launch of a step, for launches batched over several simulations: the batched
-// kernel, the simulation's own grid / workgroup / LDS and its packed arguments (a BatchPack).
-struct LaunchRec {
-    const void* fn;
-    unsigned gx, gy, block;
-    size_t lds;
-    bool flex;                          // body runs with any workgroup size
-    std::vector<unsigned char> pack;
-};
-
-template <class B, class... A>
-LaunchRec make_rec(dim3 grid, dim3 block, size_t lds, const A&... a) {
-    using P = BatchPack<A...>;
-    P pk{0, (int)grid.x, 0, 0, std::tuple<A...>(a...)};
-    LaunchRec r{(const void*)&batched_kernel<B, A...>, grid.x, grid.y, block.x, lds, B::kFlex, {}};
-    r.pack.resize(sizeof(P));
-    std::memcpy(r.pack.data(), &pk, sizeof(P));
-    return r;
-}
 
 // The launches of one step of one sim for one step key, with a hash of their kernels and
 // fixed workgroup sizes: sims whose hashes match can share batched launches.
@@ -315,7 +296,8 @@ struct SimT final : kwh_sim {
         HIPCHK(hipMemcpy(d, rep.data(), kScalarRep * sizeof(T), hipMemcpyHostToDevice));
         return SMap<T>{nullptr, v, d, 0};
     }
-    int sqR[3] = {0, 0, 0};   // R when the internal axis length is R*R with a register-resident kernel
+    // register-resident launchers per internal axis (null: generic Stockham kernels)
+    const RegOps<T>* rx[3] = {nullptr, nullptr, nullptr};
     // launch schedule chosen by the autotuner: halve the tile columns of the strided / middle
     // passes (ntr_shift), halve the row pairs per workgroup of the row kernels (p_shift), LDS
     // budget of the generic Stockham passes and the workgroup count below which they take fewer
@@ -387,6 +369,7 @@ struct SimT final : kwh_sim {
         }
         hipLaunchKernelGGL(B::kernel, grid, block, lds, stream, a...);
     }
+    Emit em() const { return Emit{stream, recording}; }
 
     // ------------------------------------------------------------------ launch geometry
     struct ZCfg { ZTile zt; int blocks, threads; size_t lds; };
@@ -439,9 +422,13 @@ struct SimT final : kwh_sim {
         std::vector<void*> v(fields);
         strided_v(axis, inv, v);
     }
-    // columns per tile for the register-resident kernels: 256 lanes (128 in float64), or the
-    // autotuner's choice (half of that)
-    int sq_ntr(int R) const { return ((sizeof(T) == 8 ? 128 : 256) / R) >> sched.ntr_shift; }
+    // columns per tile for the register-resident kernels: the widest tile whose LDS fits (256
+    // lanes in float32, 128 in float64), or the autotuner's choice (half of that)
+    int reg_ntr(const RegOps<T>* o) const { return o->ntr_max >> sched.ntr_shift; }
+    static size_t reg_col_lds(const RegOps<T>* o, int ntr) {
+        const int L = o->J * o->E;
+        return ((size_t)L * ntr + o->J + L) * sizeof(C);   // slab + pad + twiddle table
+    }
     void strided_v(int axis, bool inv, const std::vector<void*>& fields) {
         SCfg c = scfg(axis, 2);
         if (c.L <= 1) return;
@@ -455,20 +442,13 @@ struct SimT final : kwh_sim {
         a.inner = c.inner;
         a.ncols = c.ncols;
         const double bytes = 2.0 * spec_bytes() * nf;
-        const int R = sqR[axis];
-        if (R) {
-            const int ntr = sq_ntr(R);
+        if (const RegOps<T>* o = rx[axis]) {
+            const int ntr = reg_ntr(o);
             a.lg_ntr = ilog2(ntr);
             a.pitch = ntr;
             dim3 grid((unsigned)((c.ncols + ntr - 1) / ntr), nf);
-            const size_t lds = ((size_t)c.L * ntr + R + R * R) * sizeof(C);   // slab + twiddle table
             timed(inv ? K_SI : K_SF, bytes, [&] {
-#define KWH_SQS(RR, NN)                                                                                     \
-    if (inv) emit<SqStridedB<T, RR, NN, true>>(grid, dim3(ntr * R), lds, a, tab.tw[axis]); \
-    else emit<SqStridedB<T, RR, NN, false>>(grid, dim3(ntr * R), lds, a, tab.tw[axis]);
-                if (R == 16) { if (ntr == 16) { KWH_SQS(16, 16) } else { KWH_SQS(16, 8) } }
-                else { if (ntr == 32) { KWH_SQS(8, 32) } else { KWH_SQS(8, 16) } }
-#undef KWH_SQS
+                o->strided(em(), grid, dim3(ntr * o->J), reg_col_lds(o, ntr), a, tab.tw[axis], inv, ntr);
             });
             return;
         }
@@ -496,7 +476,7 @@ struct SimT final : kwh_sim {
         a.inner = c.inner;
         a.ncols = c.ncols;
         a.scale = 1.0 / (double)g.N;
-        const int R = sqR[mid_axis];
+        const RegOps<T>* o = rx[mid_axis];
         // algorithmic bytes: inputs + outputs + one read of each distinct k-space table
         std::vector<const void*> tabs_used;
         for (int i = 0; i < a.nop; ++i) {
@@ -506,7 +486,7 @@ struct SimT final : kwh_sim {
         }
         const int ntab = (int)tabs_used.size();
         const double bytes = spec_bytes() * (a.nin + a.nop) + (double)g.NH * sizeof(T) * ntab;
-        if (R) {
+        if (o) {
             for (int f = 0; f < a.nin; ++f) {   // one k-space table per input (loaded once in-kernel)
                 const void* t0 = nullptr;
                 for (auto& o : ops)
@@ -515,19 +495,12 @@ struct SimT final : kwh_sim {
                         t0 = o.table;
                     }
             }
-            const int ntr = sq_ntr(R);
+            const int ntr = reg_ntr(o);
             a.lg_ntr = ilog2(ntr);
             a.pitch = ntr;
-            const size_t lds = ((size_t)a.L * ntr + R + R * R) * sizeof(C);   // slab + twiddle table
             const unsigned blocks = (unsigned)((c.ncols + ntr - 1) / ntr);
             timed(kid, bytes, [&] {
-                if (R == 16) {
-                    if (ntr == 16) emit<SqMiddleB<T, 16, 16>>(dim3(blocks), dim3(ntr * R), lds, a, tab, g);
-                    else emit<SqMiddleB<T, 16, 8>>(dim3(blocks), dim3(ntr * R), lds, a, tab, g);
-                } else {
-                    if (ntr == 32) emit<SqMiddleB<T, 8, 32>>(dim3(blocks), dim3(ntr * R), lds, a, tab, g);
-                    else emit<SqMiddleB<T, 8, 16>>(dim3(blocks), dim3(ntr * R), lds, a, tab, g);
-                }
+                o->middle(em(), dim3(blocks), dim3(ntr * o->J), reg_col_lds(o, ntr), a, tab, g, ntr);
             });
             return;
         }
@@ -538,36 +511,32 @@ struct SimT final : kwh_sim {
         });
     }
 
-    // Launch configuration of the register-resident z kernels: P row pairs x R lanes.
-    struct SqZ { int P, blocks, threads; size_t lds; };
+    // Launch configuration of the register-resident z kernels: P row pairs x J lanes.
+    struct RegZ { int P, blocks, threads; size_t lds; };
     // Row pairs per workgroup: 256 (FP32) or 128 (FP64) threads, halved until the LDS fits 64 KiB.
-    SqZ sqz() const {
-        const int R = sqR[2];
-        SqZ z{};
-        z.P = ((sizeof(T) == 8 ? 128 : 256) / R) >> sched.p_shift;
-        const size_t per_pair = (size_t)(R * R + R + 1) * sizeof(C);
-        const size_t twb = (size_t)R * R * sizeof(C);   // LDS twiddle table
+    RegZ regz(int nrows_used = -1) const {
+        const RegOps<T>* o = rx[2];
+        const int L = o->J * o->E;
+        RegZ z{};
+        z.P = ((sizeof(T) == 8 ? 128 : 256) / o->J) >> sched.p_shift;
+        const size_t per_pair = (size_t)(L + o->E + 1) * sizeof(C);
+        const size_t twb = (size_t)L * sizeof(C);   // LDS twiddle table
         while (z.P > 1 && z.P * per_pair + twb > 65536) z.P /= 2;
-        const int npairs = (g.nrows + 1) / 2;
+        const int npairs = ((nrows_used < 0 ? g.nrows : nrows_used) + 1) / 2;
         z.blocks = (npairs + z.P - 1) / z.P;
-        z.threads = z.P * R;
+        z.threads = z.P * o->J;
         z.lds = (size_t)z.P * per_pair + twb;
         return z;
-    }
-    template <class B16, class B8, typename... Args>
-    void zlaunch(int kid, double bytes, Args... args) {
-        SqZ z = sqz();
-        timed(kid, bytes, [&] {
-            if (sqR[2] == 16) emit<B16>(dim3(z.blocks), dim3(z.threads), z.lds, args..., z.P, tab);
-            else emit<B8>(dim3(z.blocks), dim3(z.threads), z.lds, args..., z.P, tab);
-        });
     }
 
     // Full forward transform of a real field into `out` (z, then plain passes): leaves the
     // spectrum ready for a middle pass.
     void forward_to_middle(const T* src, C* out, T scale) {
-        if (sqR[2]) {
-            zlaunch<SqZForwardRealB<T, 16>, SqZForwardRealB<T, 8>>(K_ZFR, real_bytes() + spec_bytes(), src, out, scale, g);
+        if (rx[2]) {
+            const RegZ z = regz();
+            timed(K_ZFR, real_bytes() + spec_bytes(), [&] {
+                rx[2]->z_forward_real(em(), dim3(z.blocks), dim3(z.threads), z.lds, src, out, scale, g, z.P, tab);
+            });
             if (plain1) strided(1, false, {out});
             return;
         }
@@ -580,9 +549,12 @@ struct SimT final : kwh_sim {
     }
     void inverse_from_middle(C* in, T* dst, T scale, bool accumulate) {
         if (plain1) strided(1, true, {in});
-        if (sqR[2]) {
-            zlaunch<SqZInverseRealB<T, 16>, SqZInverseRealB<T, 8>>(K_ZIR, real_bytes() + spec_bytes(),
-                    (const C*)in, dst, scale, accumulate ? 1 : 0, g);
+        if (rx[2]) {
+            const RegZ z = regz();
+            timed(K_ZIR, real_bytes() + spec_bytes(), [&] {
+                rx[2]->z_inverse_real(em(), dim3(z.blocks), dim3(z.threads), z.lds, (const C*)in, dst, scale,
+                                      accumulate ? 1 : 0, g, z.P, tab);
+            });
             return;
         }
         ZCfg zc = zcfg(2);
@@ -627,7 +599,7 @@ struct SimT final : kwh_sim {
         {
             const int mm[3] = {g.m0, g.m1, g.m2};
             for (int ax = 0; ax < 3; ++ax)
-                sqR[ax] = !use_sq ? 0 : (mm[ax] == 256 ? 16 : (mm[ax] == 64 ? 8 : 0));
+                rx[ax] = use_sq && mm[ax] > 1 ? reg_ops<T>(mm[ax]) : nullptr;
         }
         const int64_t N = g.N;
 
@@ -1361,9 +1333,11 @@ struct SimT final : kwh_sim {
                 bytes += 2 * spec_bytes() + 2 * real_bytes() + mapb(dtrs[a].p) + (cu[a] ? real_bytes() : 0);
             bool anycorr = false;
             for (int a = 0; a < dim; ++a) anycorr |= cu[a] != nullptr;
-            if (sqR[2]) {
-                if (anycorr) zlaunch<SqZVelocityB<T, 16, true>, SqZVelocityB<T, 8, true>>(K_ZV, bytes, va, g);
-                else zlaunch<SqZVelocityB<T, 16, false>, SqZVelocityB<T, 8, false>>(K_ZV, bytes, va, g);
+            if (rx[2]) {
+                const RegZ z = regz();
+                timed(K_ZV, bytes, [&] {
+                    rx[2]->z_velocity(em(), dim3(z.blocks), dim3(z.threads), z.lds, va, g, z.P, tab, anycorr);
+                });
             } else {
                 ZCfg zc = zcfg(2);
                 timed(K_ZV, bytes, [&] {
@@ -1372,20 +1346,14 @@ struct SimT final : kwh_sim {
             }
         }
         // collocated contiguous-axis velocity on the sensor rows only, from R2C_z(u) (:735-745)
-        const bool zcol = sqR[2] && need_u[dim - 1];
+        const bool zcol = rx[2] && need_u[dim - 1];
         if (zcol && nzrows > 0) {
-            SqZ z = sqz();
-            const int npairs = (nzrows + 1) / 2;
-            const unsigned blocks = (unsigned)((npairs + z.P - 1) / z.P);
+            const RegZ z = regz(nzrows);
             const C* s = spec[dim - 1].as<C>();
             T* dst = ucol[dim - 1].as<T>();
             timed(K_ZCR, (double)nzrows * g.m2 * (sizeof(T) + sizeof(C) / 2), [&] {
-                if (sqR[2] == 16)
-                    emit<SqZCollocRowsB<T, 16>>(dim3(blocks), dim3(z.threads), z.lds, s, dst,
-                                       (const int*)zrows.p, nzrows, g, z.P, tab);
-                else
-                    emit<SqZCollocRowsB<T, 8>>(dim3(blocks), dim3(z.threads), z.lds, s, dst,
-                                       (const int*)zrows.p, nzrows, g, z.P, tab);
+                rx[2]->z_colloc_rows(em(), dim3(z.blocks), dim3(z.threads), z.lds, s, dst, (const int*)zrows.p,
+                                     nzrows, g, z.P, tab);
             });
         }
         auto full_colloc = [&](int a) { return need_u[a] && !(zcol && a == dim - 1); };
@@ -1410,9 +1378,12 @@ struct SimT final : kwh_sim {
             if (full_colloc(a)) {
                 C* s = spec[ucol_slot[a]].as<C>();
                 T* dst = ucol[a].as<T>();
-                if (sqR[2]) {
-                    zlaunch<SqZInverseRealB<T, 16>, SqZInverseRealB<T, 8>>(K_ZIR, real_bytes() + spec_bytes(),
-                            (const C*)s, dst, T(1), 0, g);
+                if (rx[2]) {
+                    const RegZ z = regz();
+                    timed(K_ZIR, real_bytes() + spec_bytes(), [&] {
+                        rx[2]->z_inverse_real(em(), dim3(z.blocks), dim3(z.threads), z.lds, (const C*)s, dst, T(1), 0,
+                                              g, z.P, tab);
+                    });
                     continue;
                 }
                 ZCfg zc = zcfg(2);
@@ -1451,22 +1422,13 @@ struct SimT final : kwh_sim {
             if (nonlinear) bytes += mapb(bonas.p);
             if (fft_abs) bytes += real_bytes() + spec_bytes() * (dispersion ? 2 : 1);
             else bytes += real_bytes() + mapb(c0s.p) + spec_bytes() + (stokes ? mapb(taus.p) : 0);
-            if (sqR[2]) {
-                const bool cr = cp != nullptr;
-                // MODE 0: EOS; 1: FFT absorption inputs; 2: EOS with Stokes absorption
-#define KWH_ZD(NL, MODE)                                                                                            \
-    (cr ? zlaunch<SqZDensityB<T, 16, NL, MODE, true>, SqZDensityB<T, 8, NL, MODE, true>>(K_ZD, bytes, ra, g) \
-        : zlaunch<SqZDensityB<T, 16, NL, MODE, false>, SqZDensityB<T, 8, NL, MODE, false>>(K_ZD, bytes, ra, g))
-                if (nonlinear) {
-                    if (fft_abs) KWH_ZD(true, 1);
-                    else if (stokes) KWH_ZD(true, 2);
-                    else KWH_ZD(true, 0);
-                } else {
-                    if (fft_abs) KWH_ZD(false, 1);
-                    else if (stokes) KWH_ZD(false, 2);
-                    else KWH_ZD(false, 0);
-                }
-#undef KWH_ZD
+            if (rx[2]) {
+                const RegZ z = regz();
+                const int mode = fft_abs ? 1 : (stokes ? 2 : 0);
+                timed(K_ZD, bytes, [&] {
+                    rx[2]->z_density(em(), dim3(z.blocks), dim3(z.threads), z.lds, ra, g, z.P, tab, nonlinear, mode,
+                                     cp != nullptr);
+                });
             } else {
                 ZCfg zc = zcfg(nbuf);
                 timed(K_ZD, bytes, [&] {
@@ -1509,8 +1471,11 @@ struct SimT final : kwh_sim {
             ea.outP = spec[0].as<C>();
             double bytes = spec_bytes() * (dispersion ? 3 : 2) + 2 * real_bytes() + mapb(c0s.p) + mapb(taus.p) +
                            (dispersion ? mapb(etas.p) : 0) + (nonlinear ? mapb(bonas.p) + mapb(rho0s.p) : 0);
-            if (sqR[2]) {
-                zlaunch<SqZEosB<T, 16>, SqZEosB<T, 8>>(K_ZE, bytes, ea, g);
+            if (rx[2]) {
+                const RegZ z = regz();
+                timed(K_ZE, bytes, [&] {
+                    rx[2]->z_eos(em(), dim3(z.blocks), dim3(z.threads), z.lds, ea, g, z.P, tab);
+                });
             } else {
                 ZCfg zc = zcfg(3);
                 timed(K_ZE, bytes, [&] {
@@ -1687,8 +1652,8 @@ struct SimT final : kwh_sim {
         sync();
         static std::vector<std::pair<std::string, Schedule>> cache;
         char keyb[96];
-        std::snprintf(keyb, sizeof(keyb), "%d:%d:%d:%d:%zu:%d%d%d", dim, g.m0, g.m1, g.m2, sizeof(T), sqR[0], sqR[1],
-                      sqR[2]);
+        std::snprintf(keyb, sizeof(keyb), "%d:%d:%d:%d:%zu:%d%d%d", dim, g.m0, g.m1, g.m2, sizeof(T), rx[0] ? 1 : 0,
+                      rx[1] ? 1 : 0, rx[2] ? 1 : 0);
         const std::string key(keyb);
         std::string report;
         for (auto& e : cache)
@@ -1697,13 +1662,13 @@ struct SimT final : kwh_sim {
                 report = "cached schedule for " + key + "\n";
                 return report;
             }
-        const bool sq_cols = sizeof(T) == 4 && (sqR[mid_axis] || (plain1 && sqR[1]));
+        const bool reg_cols = sizeof(T) == 4 && (rx[mid_axis] || (plain1 && rx[1]));
         bool generic = false;
         const int m[3] = {g.m0, g.m1, g.m2};
-        for (int ax = 0; ax < 3; ++ax) generic |= m[ax] > 1 && sqR[ax] == 0;
+        for (int ax = 0; ax < 3; ++ax) generic |= m[ax] > 1 && !rx[ax];
         std::vector<Schedule> cands;
-        for (int ns = 0; ns <= (sq_cols ? 1 : 0); ++ns)
-            for (int ps = 0; ps <= (sqR[2] ? 1 : 0); ++ps)
+        for (int ns = 0; ns <= (reg_cols ? 1 : 0); ++ns)
+            for (int ps = 0; ps <= (rx[2] ? 1 : 0); ++ps)
                 for (int lb = 0; lb <= (generic ? 2 : 0); ++lb) {
                     Schedule c;
                     c.ntr_shift = ns;
@@ -1793,9 +1758,12 @@ struct SimT final : kwh_sim {
     }
     // spec[0] = R2C_z(p), the derived state a step starts from
     void rebuild_P() {
-        if (sqR[2]) {
-            zlaunch<SqZForwardRealB<T, 16>, SqZForwardRealB<T, 8>>(K_ZFR, real_bytes() + spec_bytes(),
-                    (const T*)p.as<T>(), spec[0].as<C>(), T(1), g);
+        if (rx[2]) {
+            const RegZ z = regz();
+            timed(K_ZFR, real_bytes() + spec_bytes(), [&] {
+                rx[2]->z_forward_real(em(), dim3(z.blocks), dim3(z.threads), z.lds, (const T*)p.as<T>(),
+                                      spec[0].as<C>(), T(1), g, z.P, tab);
+            });
         } else {
             ZCfg zc = zcfg(2);
             hipLaunchKernelGGL((z_forward_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, p.as<T>(),
@@ -2162,11 +2130,9 @@ struct SimT final : kwh_sim {
         }
         steps = hdr[0];
         rec_steps = hdr[1];
-        // P = R2C_z(p) is derived state
-        ZCfg zc = zcfg(2);
-        hipLaunchKernelGGL((z_forward_real<T>), dim3(zc.blocks), dim3(zc.threads), zc.lds, stream, p.as<T>(),
-                           spec[0].as<C>(), T(1), g, zc.zt, tab);
-        HIPCHK(hipGetLastError());
+        // P = R2C_z(p) is derived state, rebuilt with the kernel the time step uses (bitwise the
+        // spectrum an uninterrupted run carries)
+        rebuild_P();
         HIPCHK(hipStreamSynchronize(stream));
     }
 

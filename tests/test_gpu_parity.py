@@ -532,7 +532,7 @@ def test_time_reversal(dtype):
     assert rl2 <= (1e-4 if dtype == F32 else 1e-9) and rmax <= (1e-3 if dtype == F32 else 1e-8)
 
 
-# ------------------------------------------------------------------------------------------ square-length fast path
+# ------------------------------------------------------------------------------------------ register-resident path
 @pytest.mark.parametrize("absorbing,mode,nonlinear", [
     (False, AbsorptionMode.powerLaw, False), (False, AbsorptionMode.powerLaw, True),
     (True, AbsorptionMode.stokes, False), (True, AbsorptionMode.stokes, True),

@@ -1,0 +1,144 @@
+"""GPU parity of the register-resident kernels for the axis lengths L = J*E beyond the square
+ones: 32 (4 x 8), 96 (4 x 24), 128 (8 x 16), 160 (4 x 40), 192 (8 x 24), 512 (16 x 32). Each
+length is run as the contiguous axis (row kernels), the outermost axis (middle pass) and, in 3D,
+the strided axis, against the float64 oracle on identical float32 inputs; plus agreement with
+the generic Stockham kernels on the same problem and bitwise neutrality of the launch schedule.
+
+Tolerances are the reference's parity tolerances (ParityTests.swift:54-58): float32 relL2 <= 1e-4
+and relMax <= 1e-3.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from kwavehip import (AbsorptionMode, KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, RecordField, Simulation,
+                      SimulationOptions, SourceMode, kspaceFirstOrder, makeBall, makeDisc, random_field, toneBurst)
+
+from test_gpu_parity import F32, _device, check, metrics  # noqa: F401  (_device: module fixture)
+
+pytestmark = pytest.mark.gpu
+
+
+def _medium(shape, seed, absorbing=True, nonlinear=True, mode=AbsorptionMode.powerLaw, y=1.5):
+    c = random_field(shape, seed, 4.0, 1400.0, 1600.0)
+    rho = random_field(shape, seed + 1, 4.0, 900.0, 1100.0)
+    return KWaveMedium(c, rho, alphaCoeff=0.75 if absorbing else None, alphaPower=y,
+                       alphaMode=mode if absorbing else AbsorptionMode.noAbsorption,
+                       bOnA=6.0 if nonlinear else None)
+
+
+@pytest.mark.parametrize("shape", [(96, 160), (160, 96), (128, 192), (192, 128), (512, 32), (32, 512)])
+def test_2d_register_lengths(shape):
+    """Heterogeneous absorbing nonlinear medium, p0 disc plus an additive (k-space corrected)
+    velocity line source; line sensor p / ux / uy and pFinal."""
+    nx, ny = shape
+    dx = 1e-4
+    med = _medium(shape, seed=61)
+    grid = KWaveGrid(nx, dx, ny, dx)
+    grid.setTime(90, 0.3 * dx / float(np.max(med.soundSpeed)))
+    umask = np.zeros(shape, np.float32)
+    umask[nx // 2 - 3, ny // 2 - 4: ny // 2 + 4] = 1
+    sig = (0.2 * toneBurst(1 / grid.dt, 3e6, 2)).astype(np.float32)
+    sens = np.zeros(shape, np.float32)
+    sens[nx // 2 + 6, :] = 1
+    check(f"2d_reg_{nx}x{ny}", grid, med,
+          KWaveSource(p0=makeDisc(nx, ny, radius=5), uMask=umask, ux=sig, uMode=SourceMode.additive),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.ux | RecordField.uy | RecordField.pFinal),
+          ["p", "ux", "uy", "pFinal"], pmlSize=10)
+
+
+@pytest.mark.parametrize("absorbing,mode,nonlinear", [
+    (False, AbsorptionMode.powerLaw, False), (False, AbsorptionMode.powerLaw, True),
+    (True, AbsorptionMode.stokes, False), (True, AbsorptionMode.stokes, True),
+    (True, AbsorptionMode.noDispersion, False), (True, AbsorptionMode.powerLaw, True)])
+def test_2d_register_density_variants(absorbing, mode, nonlinear):
+    """Every density-kernel variant (EOS, Stokes EOS, FFT absorption with / without dispersion;
+    linear and B/A) with an additive k-space corrected pressure source, on a 96 x 160 grid."""
+    shape = (96, 160)
+    med = _medium(shape, seed=71, absorbing=absorbing, nonlinear=nonlinear, mode=mode, y=1.4)
+    grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4)
+    grid.setTime(120, 0.3 * 1e-4 / float(np.max(med.soundSpeed)))
+    pmask = np.zeros(shape, np.float32)
+    pmask[40, 70:90] = 1
+    sig = (1e5 * toneBurst(1 / grid.dt, 2e6, 2)).astype(np.float32)
+    check(f"2d_reg_density_{absorbing}_{mode.value}_{nonlinear}", grid, med,
+          KWaveSource(p0=makeDisc(*shape, radius=6), pMask=pmask, p=sig, pMode=SourceMode.additive),
+          KWaveSensor(), ["pFinal"], pmlSize=10)
+
+
+@pytest.mark.parametrize("shape", [(32, 96, 160), (160, 32, 96), (96, 128, 32), (128, 192, 32)])
+def test_3d_register_lengths(shape):
+    """3D heterogeneous power-law medium with B/A, p0 ball plus a velocity source patch; plane
+    sensor recording p, ux and uz (the contiguous component is collocated on the sensor rows)."""
+    nx, ny, nz = shape
+    c = random_field(shape, 81, 4.0, 1400.0, 1600.0)
+    rho = random_field(shape, 82, 4.0, 900.0, 1100.0)
+    med = KWaveMedium(c, rho, alphaCoeff=0.5, alphaPower=1.1, alphaMode=AbsorptionMode.powerLaw, bOnA=5.0)
+    grid = KWaveGrid(nx, 1e-4, ny, 1e-4, nz, 1e-4)
+    grid.setTime(40, 0.3 * 1e-4 / float(c.max()))
+    umask = np.zeros(shape, np.float32)
+    umask[nx // 2 - 4, ny // 2 - 3: ny // 2 + 3, nz // 2 - 3: nz // 2 + 3] = 1
+    sig = (0.2 * toneBurst(1 / grid.dt, 3e6, 2)).astype(np.float32)
+    sens = np.zeros(shape, np.float32)
+    sens[nx // 2 + 3] = 1
+    check(f"3d_reg_{nx}x{ny}x{nz}", grid, med,
+          KWaveSource(p0=makeBall(nx, ny, nz, radius=4), uMask=umask, ux=sig),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.ux | RecordField.uz),
+          ["p", "ux", "uz"], pmlSize=6, workers=8)
+
+
+@pytest.mark.parametrize("shape", [(96, 128, 160), (160, 128, 128), (512, 512)])
+def test_register_and_generic_paths_agree(shape):
+    """The register-resident kernels and the generic Stockham kernels on the same float32 problem
+    (the scan-line grid shapes and the 512^2 grid): the two differ only in FP32 rounding."""
+    dim = len(shape)
+    args = [v for n in shape for v in (n, 1e-4)]
+    grid = KWaveGrid(*args)
+    grid.setTime(40, 2e-8)
+    p0 = makeDisc(*shape, radius=6) if dim == 2 else makeBall(*shape, radius=6)
+    med = KWaveMedium(random_field(shape, 91, 6.0, 1450.0, 1600.0), 1000.0)
+    sens = np.zeros(shape, np.float32)
+    sens[shape[0] // 2 + 8] = 1
+    run = lambda: kspaceFirstOrder(grid, med, KWaveSource(p0=p0), KWaveSensor(mask=sens, record=RecordField.p),
+                                   SimulationOptions(pmlSize=10))
+    print(f"\n[case] register vs generic {shape}", flush=True)
+    a = run()
+    os.environ["KWH_GENERIC_FFT"] = "1"
+    try:
+        b = run()
+    finally:
+        del os.environ["KWH_GENERIC_FFT"]
+    for f in ("p", "pFinal"):
+        rl2, rmax = metrics(getattr(a, f), getattr(b, f))
+        print(f"  {f} relL2={rl2:.3e} relMax={rmax:.3e}", flush=True)
+        assert rl2 < 1e-5 and rmax < 1e-4
+
+
+@pytest.mark.parametrize("shape", [(96, 160), (32, 96, 160), (160, 128, 128)])
+def test_register_lengths_schedule_is_bitwise_neutral(shape):
+    """Autotuned launch schedules change launch geometry only: results equal the default
+    schedule's bit for bit."""
+    dim = len(shape)
+    args = [v for n in shape for v in (n, 1e-4)]
+    grid = KWaveGrid(*args)
+    grid.setTime(25, 2e-8)
+    p0 = makeDisc(*shape, radius=4) if dim == 2 else makeBall(*shape, radius=4)
+    mask = np.zeros(shape, np.float32)
+    mask[(shape[0] * 3) // 4] = 1
+    med = KWaveMedium(random_field(shape, 19, 4.0, 1450.0, 1600.0), 1000.0, alphaCoeff=0.5, alphaPower=1.3,
+                      alphaMode=AbsorptionMode.powerLaw)
+    make = lambda: Simulation(grid, med, KWaveSource(p0=p0), KWaveSensor(mask=mask, record=RecordField.p),
+                              SimulationOptions(pmlSize=6))
+    print(f"\n[case] autotune {shape}", flush=True)
+    a = make()
+    a.run()
+    ref = a.outputs()
+    a.close()
+    b = make()
+    report = b.autotune()
+    print("  " + report.replace("\n", "\n  "), flush=True)
+    b.run()
+    out = b.outputs()
+    b.close()
+    assert np.array_equal(out.p, ref.p) and np.array_equal(out.pFinal, ref.pFinal)
