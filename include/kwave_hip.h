@@ -251,8 +251,8 @@ int kwh_sim_vjp(kwh_sim* sim, const void* dl_dp, int32_t checkpoint_every, void*
  *                      interleaved step by step;
  *   KWH_BATCH_FUSED    one launch per pass of the step covers every sim whose step has the same
  *                      launch sequence and launch geometry (the sims of one grid shape); the
- *                      shapes run concurrently, each on its own stream; with KWH_BATCH_GRAPH
- *                      each shape's batched step is captured once and replayed.
+ *                      shapes of one step run one after another on the batch's stream; with
+ *                      KWH_BATCH_GRAPH each shape's batched step is captured once and replayed.
  * Every mode gives results bitwise equal to running the sims one by one. Outputs are read per sim
  * (kwh_batch_read_output, or the borrowed handle from kwh_batch_sim with the kwh_sim_* readers). */
 #define KWH_BATCH_GRAPH 1

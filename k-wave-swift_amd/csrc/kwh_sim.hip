@@ -577,6 +577,8 @@ struct SimT final : kwh_sim {
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         if (const char* e = std::getenv("KWH_LDS_BUDGET")) sched.lds = (size_t)std::atol(e);
         if (const char* e = std::getenv("KWH_MIN_BLOCKS")) sched.min_blocks = std::max(1, std::atoi(e));
+        if (const char* e = std::getenv("KWH_NTR_SHIFT")) sched.ntr_shift = std::atoi(e) != 0 ? 1 : 0;
+        if (const char* e = std::getenv("KWH_P_SHIFT")) sched.p_shift = std::max(0, std::min(2, std::atoi(e)));
         scal_pool.alloc((size_t)64 * kScalarRep * sizeof(T));
         zero_map = mk_scalar(T(0));
         taus = etas = bonas = zero_map;
@@ -2204,13 +2206,13 @@ static int guarded(F&& f) {
 //    same launch sequence and launch geometry (kwh_batch.h) -- in practice the sims of one grid
 //    shape. Each sim's step is recorded once per step key; the batch concatenates the sims'
 //    argument packs per pass, uploads them with a block-owner table, and issues the step as a
-//    short list of launches on the stream of that grid shape (shapes overlap on their own
-//    streams), captured in a hipGraph and replayed with KWH_BATCH_GRAPH. Results are bitwise
+//    short list of launches on the batch's stream (the groups of one step one after another),
+//    captured in a hipGraph and replayed with KWH_BATCH_GRAPH. Results are bitwise
 //    those of the single runs: every workgroup runs the same body on the same tile.
 struct kwh_batch {
     int dev = 0;
     std::vector<kwh_sim*> sims;
-    std::vector<hipStream_t> streams;   // fused runs: one per grid shape
+    std::vector<hipStream_t> streams;   // fused runs: the batch's stream
 
     struct FusedLaunch {
         const void* fn = nullptr;
@@ -2285,32 +2287,27 @@ struct kwh_batch {
 
     void run_fused(bool use_graph) {
         const int n = (int)sims.size();
-        // one stream per grid shape (up to kMaxStreams): groups of different shapes overlap, and a
-        // sim always stays on its shape's stream, which orders its steps
-        constexpr int kMaxStreams = 8;
-        std::map<uint64_t, int> shapes;
-        std::vector<int> sidx(n);
-        for (int i = 0; i < n; ++i) {
-            auto it = shapes.find(sims[i]->shape_class());
-            if (it == shapes.end()) it = shapes.emplace(sims[i]->shape_class(), (int)shapes.size()).first;
-            sidx[i] = it->second % kMaxStreams;
-        }
-        while ((int)streams.size() < std::min((int)shapes.size(), kMaxStreams)) {
+        // One stream for the whole batch: the groups of one step (one group per grid shape and
+        // launch sequence) run one after another. Groups on streams of their own overlap on the
+        // device and measured slower than in sequence (their kernels compete for the CUs and the
+        // caches while each group's batched launches already fill the chip).
+        if (streams.empty()) {
             hipStream_t st = nullptr;
             HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
             streams.push_back(st);
         }
-        for (int i = 0; i < n; ++i) sims[i]->set_counters_on(streams[sidx[i]]);
+        hipStream_t st = streams[0];
+        for (int i = 0; i < n; ++i) sims[i]->set_counters_on(st);
         std::vector<const StepRecord*> rec(n);
         std::vector<int> ts(n);
         for (;;) {
-            // group the active sims by stream and launch sequence of their current step
-            std::vector<std::pair<std::pair<int, uint64_t>, int>> order;
+            // group the active sims by the launch sequence of their current step
+            std::vector<std::pair<uint64_t, int>> order;
             for (int i = 0; i < n; ++i) {
                 ts[i] = sims[i]->steps_done();
                 if (ts[i] >= sims[i]->total_steps()) continue;
                 rec[i] = &sims[i]->recorded_step(ts[i]);
-                order.push_back({{sidx[i], rec[i]->signature}, i});
+                order.push_back({rec[i]->signature, i});
             }
             if (order.empty()) break;
             std::stable_sort(order.begin(), order.end(),
@@ -2326,7 +2323,6 @@ struct kwh_batch {
                     members.push_back(rec[i]);
                     ++g1;
                 }
-                hipStream_t st = streams[order[g0].first.first];
                 auto it = fused.find(key);
                 if (it == fused.end()) it = fused.emplace(key, build(members)).first;
                 FusedStep& fs = *it->second;
@@ -2345,7 +2341,7 @@ struct kwh_batch {
             }
             for (auto& o : order) sims[o.second]->step_done(ts[o.second]);
         }
-        for (hipStream_t st : streams) HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipStreamSynchronize(st));
     }
 
     void run(int mode) {
