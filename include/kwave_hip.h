@@ -170,6 +170,15 @@ int kwh_sim_validate(const kwh_sim_desc* desc);
  * (setup of kspaceFirstOrder1D/2D/3D, KSpaceFirstOrder.swift:203-308, 351-491, 557-703). */
 int kwh_sim_create(kwh_ctx* ctx, const kwh_sim_desc* desc, kwh_sim** out);
 
+/* The axisymmetric solver kspaceFirstOrderAS(grid:medium:source:sensor:options:)
+ * (Sources/KWave/Solver/KSpaceFirstOrderAS.swift:17-23) on the same descriptor: a 2D grid with x
+ * axial and y radial (y index 0 on the axis), WSWA-FFT radial symmetry on a 4x radially expanded
+ * grid, a one-sided radial PML, sources p0 / p (three modes) / ux, uy (three modes), Stokes
+ * absorption, B/A; the pressure is recorded (p and its aggregates, pFinal). The handle is used
+ * with kwh_sim_run / kwh_sim_read_output like any other; step graphs, checkpoints, the adjoint and
+ * the autotuner are not available for it (KWH_EUNSUPPORTED). */
+int kwh_sim_create_axisymmetric(kwh_ctx* ctx, const kwh_sim_desc* desc, kwh_sim** out);
+
 /* Advance steps [t_begin, t_end) (the hot loop, KSpaceFirstOrder.swift:309-342, 492-546,
  * 704-751). Runs must be contiguous: t_begin equals the number of steps already run.
  * cb may be NULL. Returns after the device has finished the requested steps. */

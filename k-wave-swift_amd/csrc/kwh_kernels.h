@@ -178,7 +178,11 @@ __device__ __forceinline__ cx<T> mid_op_value(const MidOp& op, const AxisTabs<T>
             v = tab.ddxn[op.axis][idx] * (k * scale);
             break;
         }
-        case OP_COLL: v = tab.coll[op.axis][idx] * scale; break;
+        case OP_COLL: {   // with a table: the shift times that table (kappa)
+            const T k = op.table ? reinterpret_cast<const T*>(op.table)[h] : T(1);
+            v = tab.coll[op.axis][idx] * (k * scale);
+            break;
+        }
         case OP_DDXP: v = tab.ddxp[op.axis][idx] * scale; break;
         default: {
             const T k = reinterpret_cast<const T*>(op.table)[h];

@@ -22,6 +22,7 @@
 #include "kwh_kernels.h"
 #include "kwh_setup.h"
 #include "kwh_adjoint.h"
+#include "kwh_as.h"
 #include "kwh_batch.h"
 #include "kwh_launch.h"
 #include "kwh_regops.h"
@@ -109,8 +110,10 @@ std::vector<double> fft_freq(int n, double d) {
     return r;
 }
 
-// pmlProfile (Sources/KWave/PML/PML.swift:17-43).
-std::vector<double> pml_profile(int n, double dx, double dt, double c, int pml_size, double pml_alpha, bool staggered) {
+// pmlProfile (Sources/KWave/PML/PML.swift:17-43); axisym: the radial axis of the axisymmetric
+// solver has its PML at the outer edge only (:36-38).
+std::vector<double> pml_profile(int n, double dx, double dt, double c, int pml_size, double pml_alpha, bool staggered,
+                                bool axisym = false) {
     const double a = pml_alpha * (c / dx);
     const double m = pml_size;
     std::vector<double> pml(n, 1.0);
@@ -124,7 +127,7 @@ std::vector<double> pml_profile(int n, double dx, double dt, double c, int pml_s
             pl = (m + 1 - x) / m;
             pr = x / m;
         }
-        pml[k] = std::exp(-a * dt / 2.0 * std::pow(pl, 4.0));
+        if (!axisym) pml[k] = std::exp(-a * dt / 2.0 * std::pow(pl, 4.0));
         pml[n - pml_size + k] = std::exp(-a * dt / 2.0 * std::pow(pr, 4.0));
     }
     return pml;
@@ -305,6 +308,31 @@ struct SimT final : kwh_sim {
     struct Schedule { int ntr_shift = 0, p_shift = 0; size_t lds = 64 * 1024; int min_blocks = 1024; };
     Schedule sched;
     bool use_sq = true;
+
+    // axisymmetric solver (kspaceFirstOrderAS, KSpaceFirstOrderAS.swift:17-309): the fields live
+    // on the physical [nx][ny] grid (x axial, y radial); every spectral operation mirrors its
+    // input into the 4x radially expanded grid [nx][4 ny] and transforms there. The expanded
+    // geometry, tables and launchers form a second spectral space that ExScope swaps in while
+    // the transform passes run.
+    bool axisym = false;
+    struct Space { Geo g; AxisTabs<T> tab; const RegOps<T>* rx[3]; int mid_axis; bool plain1; };
+    Space ex{};
+    DBuf ex_tw[3], ex_ddxp[3], ex_ddxn[3], ex_coll[3], ex_kv[3], ex_kappa, ex_skappa, invr_buf;
+    DBuf ex_real[3], ex_spec[3];
+    void swap_space() {
+        std::swap(g, ex.g);
+        std::swap(tab, ex.tab);
+        for (int a = 0; a < 3; ++a) std::swap(rx[a], ex.rx[a]);
+        std::swap(mid_axis, ex.mid_axis);
+        std::swap(plain1, ex.plain1);
+    }
+    struct ExScope {
+        SimT* s;
+        explicit ExScope(SimT* owner) : s(owner) { s->swap_space(); }
+        ~ExScope() { s->swap_space(); }
+        ExScope(const ExScope&) = delete;
+        ExScope& operator=(const ExScope&) = delete;
+    };
 
     ~SimT() override {
         for (auto& sg : step_graphs) {
@@ -583,6 +611,7 @@ struct SimT final : kwh_sim {
         zero_map = mk_scalar(T(0));
         taus = etas = bonas = zero_map;
         dim = d.dim;
+        if (axisym) validate_axisymmetric(d);
         for (int a = 0; a < dim; ++a) { nphys[a] = d.n[a]; dphys[a] = d.d[a]; }
         dt = d.dt;
         nt = d.nt;
@@ -637,6 +666,7 @@ struct SimT final : kwh_sim {
 
         build_axis_tables_ops(d);
         build_kspace_tables(d);
+        if (axisym) build_expanded();
         build_media(d);
         build_sources(d);
         build_sensor(d);
@@ -735,8 +765,9 @@ struct SimT final : kwh_sim {
                     co[i] = C{T(cr), T(ci)};
                 }
                 KWH_REQUIRE(d.pml_size >= 0 && d.pml_size <= L, KWH_EINVAL, "pmlSize must be between 0 and the grid size");
-                auto p1 = pml_profile(L, dphys[a], dt, c_ref, d.pml_size, d.pml_alpha, false);
-                auto p2 = pml_profile(L, dphys[a], dt, c_ref, d.pml_size, d.pml_alpha, true);
+                const bool radial = axisym && ax == 2;
+                auto p1 = pml_profile(L, dphys[a], dt, c_ref, d.pml_size, d.pml_alpha, false, radial);
+                auto p2 = pml_profile(L, dphys[a], dt, c_ref, d.pml_size, d.pml_alpha, true, radial);
                 for (int i = 0; i < L; ++i) { pm[i] = T(p1[i]); pms[i] = T(p2[i]); }
             } else {
                 for (int i = 0; i < L; ++i) { dp[i] = C{0, 0}; dn[i] = C{0, 0}; co[i] = C{1, 0}; }
@@ -1534,6 +1565,13 @@ struct SimT final : kwh_sim {
     // The launches of step t, ending with the device counter advance. Host-side mirrors
     // (steps, rec_steps, P_valid) are updated by the caller.
     void step_launches(int t) {
+        if (axisym) {
+            as_step(t);
+            record_step(t);
+            emit<AdvanceB>(dim3(1), dim3(64), 0, (int*)ctr.p, (rec_p || rec_u) ? 1 : 0);
+            HIPCHK(hipGetLastError());
+            return;
+        }
         if (t == 0 && has_p0) init_step();
         else main_step(t);
         record_step(t);
@@ -1625,6 +1663,209 @@ struct SimT final : kwh_sim {
         if (ckpt_every > 0 && t % ckpt_every == 0) save_checkpoint(t / ckpt_every);
     }
 
+    // ------------------------------------------------------------------ axisymmetric solver
+    // Preconditions of kspaceFirstOrderAS (KSpaceFirstOrderAS.swift:35-38, :95-101): a 2D grid
+    // (x axial, y radial), Stokes absorption only, sources p0 / p / ux / uy; the reference returns
+    // the pressure series and pFinal (:299-308), so only pressure fields may be recorded here.
+    void validate_axisymmetric(const kwh_sim_desc& d) {
+        KWH_REQUIRE(d.dim == 2, KWH_EINVAL, "axisymmetric solver requires a 2D grid (x axial, y radial)");
+        if (d.alpha_coeff.data && d.alpha_mode != KWH_ALPHA_NONE) {
+            const double y = d.alpha_mode == KWH_ALPHA_STOKES ? 2.0 : d.alpha_power;
+            KWH_REQUIRE(std::fabs(y - 2.0) < 1e-10, KWH_EINVAL,
+                        "axisymmetric absorption requires the Stokes model (alphaPower = 2)");
+        }
+        const uint32_t pfields = KWH_REC_P | KWH_REC_P_MAX | KWH_REC_P_MIN | KWH_REC_P_RMS | KWH_REC_P_FINAL;
+        KWH_REQUIRE((d.record & ~pfields) == 0, KWH_EUNSUPPORTED,
+                    "the axisymmetric solver records the pressure only (p, pMax, pMin, pRms, pFinal)");
+        KWH_REQUIRE(!d.directivity_angle, KWH_EUNSUPPORTED, "the axisymmetric solver has no sensor directivity");
+        KWH_REQUIRE(!d.u_source[2].index || !d.u_source[2].signal, KWH_EINVAL, "uz source on a 2D grid");
+    }
+
+    // The expanded space [1][nx][4 ny]: transform plans and twiddles, the staggered derivative and
+    // shift operators of both axes (expanded ky, :63-67), kappa and the source kappa from the
+    // expanded |k| with c_ref (:55-57, :165-166), the staggered 1/r table (:85-86) and the
+    // expanded work buffers.
+    void build_expanded() {
+        const int nx = g.m1, ny = g.m2;
+        Geo G{};
+        G.dim = 2; G.off = 1; G.m0 = 1; G.m1 = nx; G.m2 = 4 * ny;
+        G.mzh = G.m2 / 2 + 1;
+        G.N = (int64_t)nx * G.m2;
+        G.NH = (int64_t)nx * G.mzh;
+        G.nrows = nx;
+        ex.g = G;
+        ex.mid_axis = 1;
+        ex.plain1 = false;
+        ex.rx[0] = nullptr;
+        ex.rx[1] = use_sq && nx > 1 ? reg_ops<T>(nx) : nullptr;
+        ex.rx[2] = use_sq ? reg_ops<T>(G.m2) : nullptr;
+        const int m[3] = {1, nx, G.m2};
+        const double dd[3] = {1.0, dphys[0], dphys[1]};
+        for (int ax = 0; ax < 3; ++ax) {
+            const int L = m[ax];
+            ex.tab.plan[ax] = L > 1 ? make_plan(L) : FftPlan{1, 0, {}};
+            std::vector<C> tw(L), dp(L), dn(L), co(L);
+            std::vector<double> kv(L, 0.0);
+            for (int i = 0; i < L; ++i) {
+                const double ang = -2.0 * M_PI * (double)i / (double)L;
+                tw[i] = C{T(std::cos(ang)), T(std::sin(ang))};
+            }
+            if (ax > 0) {
+                const auto f = fft_freq(L, dd[ax]);
+                for (int i = 0; i < L; ++i) {
+                    const double k = 2.0 * M_PI * f[i];
+                    kv[i] = k;
+                    const double th = k * dd[ax] / 2.0;
+                    double dpr = -k * std::sin(th), dpi = k * std::cos(th);
+                    double dnr = k * std::sin(th), dni = k * std::cos(th);
+                    double cr = std::cos(th), ci = -std::sin(th);
+                    if (L % 2 == 0 && i == L / 2) { dpi = 0; dni = 0; ci = 0; }   // Hermitian part
+                    dp[i] = C{T(dpr), T(dpi)};
+                    dn[i] = C{T(dnr), T(dni)};
+                    co[i] = C{T(cr), T(ci)};
+                }
+            } else {
+                dp[0] = C{0, 0};
+                dn[0] = C{0, 0};
+                co[0] = C{1, 0};
+            }
+            upload(ex_tw[ax], tw);
+            upload(ex_ddxp[ax], dp);
+            upload(ex_ddxn[ax], dn);
+            upload(ex_coll[ax], co);
+            upload(ex_kv[ax], kv);
+            ex.tab.tw[ax] = ex_tw[ax].as<C>();
+            ex.tab.ddxp[ax] = ex_ddxp[ax].as<C>();
+            ex.tab.ddxn[ax] = ex_ddxn[ax].as<C>();
+            ex.tab.coll[ax] = ex_coll[ax].as<C>();
+            ex.tab.pml[ax] = nullptr;
+            ex.tab.pmlsg[ax] = nullptr;
+        }
+        ex_kappa.alloc(G.NH * sizeof(T));
+        ex_skappa.alloc(G.NH * sizeof(T));
+        KGeo kg{};
+        kg.m0 = 1; kg.m1 = nx; kg.m2 = G.m2; kg.mzh = G.mzh; kg.NH = G.NH;
+        kg.kv0 = ex_kv[0].as<double>();
+        kg.kv1 = ex_kv[1].as<double>();
+        kg.kv2 = ex_kv[2].as<double>();
+        hipLaunchKernelGGL((kspace_tables_kernel<T>), dim3(grid_blocks(G.NH)), dim3(256), 0, stream, kg,
+                           c_ref * dt / 2.0, 0.0, 0.0, ex_kappa.as<T>(), ex_skappa.as<T>(), (T*)nullptr, (T*)nullptr);
+        HIPCHK(hipGetLastError());
+        std::vector<T> ir(ny);
+        for (int y = 0; y < ny; ++y) ir[y] = T(1.0 / ((y + 0.5) * dphys[1]));
+        upload(invr_buf, ir);
+        for (int k = 0; k < 3; ++k) {
+            ex_real[k].alloc(G.N * sizeof(T));
+            ex_spec[k].alloc(G.NH * sizeof(C));
+        }
+    }
+
+    void as_mirror(const T* w, const T* invr, T* E, int kind) {
+        const int nx = g.m1, ny = g.m2;
+        hipLaunchKernelGGL((as_mirror_kernel<T>), dim3(grid_blocks((int64_t)nx * 4 * ny)), dim3(256), 0, stream, w, invr,
+                           E, nx, ny, kind);
+        HIPCHK(hipGetLastError());
+    }
+    // ex_real[1] = dp/dx, ex_real[2] = dp/dy on the expanded grid from the WSWA mirror of pf
+    // (pressureGradients, :216-221).
+    void as_gradients(const T* pf) {
+        as_mirror(pf, nullptr, ex_real[0].as<T>(), AS_WSWA);
+        ExScope sc(this);
+        forward_to_middle(ex_real[0].as<T>(), ex_spec[0].as<C>(), T(1));
+        middle(K_MG, {ex_spec[0].p}, {MidOp{OP_DDXP_KAPPA, 1, 0, ex_spec[1].p, ex_kappa.p},
+                                      MidOp{OP_DDXP_KAPPA, 2, 0, ex_spec[2].p, ex_kappa.p}});
+        inverse_from_middle(ex_spec[1].as<C>(), ex_real[1].as<T>(), T(1), false);
+        inverse_from_middle(ex_spec[2].as<C>(), ex_real[2].as<T>(), T(1), false);
+    }
+    void as_scatter(T* d0, T* d1, SourceDev<T>& s, int t, double factor, bool add) {
+        if (t >= s.len) return;
+        hipLaunchKernelGGL((as_scatter_kernel<T>), dim3(grid_blocks(s.n)), dim3(256), 0, stream, d0, d1,
+                           s.index.template as<int64_t>(), s.signal.template as<T>(), s.base.template as<T>(), s.rows,
+                           s.len, (int)s.n, t, T(factor), add ? 1 : 0);
+        HIPCHK(hipGetLastError());
+    }
+    // E = Re ifft2(source_kappa fft2(mirror(scatter(v)))) on the expanded grid (correctSource, :170-173).
+    void as_corrected(SourceDev<T>& s, int t, int kind, double factor, T* E, C* S) {
+        fill<T>(scr_real.as<T>(), T(0), g.N);
+        as_scatter(scr_real.as<T>(), nullptr, s, t, factor, false);
+        as_mirror(scr_real.as<T>(), nullptr, E, kind);
+        ExScope sc(this);
+        forward_to_middle(E, S, T(1));
+        middle(K_MT, {(const void*)S}, {MidOp{OP_TAB, 0, 0, S, ex_skappa.p}});
+        inverse_from_middle(S, E, T(1), false);
+    }
+    void as_add_crop(T* f0, T* f1, const T* E) {
+        const int nx = g.m1, ny = g.m2;
+        hipLaunchKernelGGL((as_add_crop_kernel<T>), dim3(grid_blocks(g.N)), dim3(256), 0, stream, f0, f1, E, nx, ny);
+        HIPCHK(hipGetLastError());
+    }
+
+    // One step of the axisymmetric loop (:236-297).
+    void as_step(int t) {
+        const int nx = g.m1, ny = g.m2;
+        const int blocks = grid_blocks(g.N);
+        if (t == 0 && has_p0) {
+            // t = 0 leapfrog override with p0 (:289-296); the rest of the t = 0 step is overwritten
+            as_gradients(p0s.as<T>());
+            hipLaunchKernelGGL((as_init_kernel<T>), dim3(blocks), dim3(256), 0, stream, (const T*)p0s.as<T>(), p.as<T>(),
+                               rho[0].as<T>(), rho[1].as<T>(), u[0].as<T>(), u[1].as<T>(), (const T*)ex_real[1].as<T>(),
+                               (const T*)ex_real[2].as<T>(), c0s, dtrs[0], dtrs[1], nx, ny);
+            HIPCHK(hipGetLastError());
+            return;
+        }
+        // velocity from the pressure gradients (:237-241)
+        as_gradients(p.as<T>());
+        hipLaunchKernelGGL((as_velocity_kernel<T>), dim3(blocks), dim3(256), 0, stream, u[0].as<T>(), u[1].as<T>(),
+                           (const T*)ex_real[1].as<T>(), (const T*)ex_real[2].as<T>(), dtrs[0], dtrs[1],
+                           tab.pmlsg[1], tab.pmlsg[2], nx, ny);
+        HIPCHK(hipGetLastError());
+        // velocity sources: ux mirrors WSWA, uy HAHS (applyVelocitySource, :177-194)
+        for (int a = 0; a < 2; ++a) {
+            SourceDev<T>& s = usrc[a];
+            if (!s.present || t >= s.len) continue;
+            if (s.mode == KWH_SRC_ADDITIVE) {
+                as_corrected(s, t, a == 0 ? AS_WSWA : AS_HAHS, s.factor[a], ex_real[0].as<T>(), ex_spec[0].as<C>());
+                as_add_crop(u[a].as<T>(), nullptr, ex_real[0].as<T>());
+            } else {
+                as_scatter(u[a].as<T>(), nullptr, s, t, s.factor[a], s.mode == KWH_SRC_ADDITIVE_NO_CORRECTION);
+            }
+        }
+        // divergence: du_x/dx and du_y/dy + u_y / r on the staggered radial grid (:243-251)
+        as_mirror(u[0].as<T>(), nullptr, ex_real[0].as<T>(), AS_WSWA);
+        as_mirror(u[1].as<T>(), nullptr, ex_real[1].as<T>(), AS_HAHS);
+        as_mirror(u[1].as<T>(), invr_buf.as<T>(), ex_real[2].as<T>(), AS_HSHA);
+        {
+            ExScope sc(this);
+            for (int k = 0; k < 3; ++k) forward_to_middle(ex_real[k].as<T>(), ex_spec[k].as<C>(), T(1));
+            middle(K_MD, {ex_spec[0].p, ex_spec[1].p, ex_spec[2].p},
+                   {MidOp{OP_DDXN_KAPPA, 1, 0, ex_spec[0].p, ex_kappa.p}, MidOp{OP_DDXN_KAPPA, 2, 1, ex_spec[1].p, ex_kappa.p},
+                    MidOp{OP_COLL, 2, 2, ex_spec[2].p, ex_kappa.p}});
+            inverse_from_middle(ex_spec[0].as<C>(), ex_real[0].as<T>(), T(1), false);
+            inverse_from_middle(ex_spec[1].as<C>(), ex_real[1].as<T>(), T(1), false);
+            inverse_from_middle(ex_spec[2].as<C>(), ex_real[1].as<T>(), T(1), true);
+        }
+        // mass conservation (:253-261)
+        hipLaunchKernelGGL((as_density_kernel<T>), dim3(blocks), dim3(256), 0, stream, rho[0].as<T>(), rho[1].as<T>(),
+                           (const T*)ex_real[0].as<T>(), (const T*)ex_real[1].as<T>(), rho0s, tab.pml[1], tab.pml[2],
+                           T(dt), nonlinear ? 1 : 0, nx, ny);
+        HIPCHK(hipGetLastError());
+        // pressure source into both density splits, scale 1/(2 c^2) or 2 dt/(2 c dx) (:263-279)
+        if (psrc.present && t < psrc.len) {
+            if (psrc.mode == KWH_SRC_ADDITIVE) {
+                as_corrected(psrc, t, AS_WSWA, psrc.factor[0], ex_real[2].as<T>(), ex_spec[2].as<C>());
+                as_add_crop(rho[0].as<T>(), rho[1].as<T>(), ex_real[2].as<T>());
+            } else {
+                as_scatter(rho[0].as<T>(), rho[1].as<T>(), psrc, t, psrc.factor[0],
+                           psrc.mode == KWH_SRC_ADDITIVE_NO_CORRECTION);
+            }
+        }
+        // equation of state (:281-287)
+        hipLaunchKernelGGL((as_eos_kernel<T>), dim3(blocks), dim3(256), 0, stream, p.as<T>(), (const T*)rho[0].as<T>(),
+                           (const T*)rho[1].as<T>(), (const T*)ex_real[0].as<T>(), (const T*)ex_real[1].as<T>(), c0s, rho0s,
+                           taus, bonas, stokes ? 1 : 0, nonlinear ? 1 : 0, nx, ny);
+        HIPCHK(hipGetLastError());
+    }
+
     // ------------------------------------------------------------------ time reversal
     // Runs the loop with the p source rows reversed in time on the device (the recorded data
     // re-injected as a Dirichlet boundary), then writes max(0, factor * pFinal).
@@ -1650,6 +1891,7 @@ struct SimT final : kwh_sim {
     // per grid shape, dimension and precision for later sims in the process. State is reset.
     std::string autotune() override {
         KWH_REQUIRE(steps == 0, KWH_ESTATE, "autotune before the first step");
+        KWH_REQUIRE(!axisym, KWH_EUNSUPPORTED, "the autotuner covers kspaceFirstOrder");
         HIPCHK(hipSetDevice(dev));
         sync();
         static std::vector<std::pair<std::string, Schedule>> cache;
@@ -1728,6 +1970,7 @@ struct SimT final : kwh_sim {
     size_t ckpt_slot_bytes() const { return (size_t)(1 + 2 * dim) * g.N * sizeof(T); }
     void set_checkpointing(int every) override {
         KWH_REQUIRE(every >= 0, KWH_EINVAL, "checkpoint interval must be >= 0");
+        KWH_REQUIRE(!axisym || every == 0, KWH_EUNSUPPORTED, "the axisymmetric solver has no adjoint");
         sync();
         ckpt_every = every;
         ckpt_have.clear();
@@ -1796,6 +2039,7 @@ struct SimT final : kwh_sim {
     void vjp(const void* dldy, int every, void* grad_p0, void* grad_c0) override {
         KWH_REQUIRE(dldy && grad_p0 && grad_c0, KWH_EINVAL, "NULL argument");
         KWH_REQUIRE(every > 0, KWH_EINVAL, "checkpoint interval must be > 0");
+        KWH_REQUIRE(!axisym, KWH_EUNSUPPORTED, "the axisymmetric solver has no adjoint");
         KWH_REQUIRE(has_p0, KWH_EUNSUPPORTED, "the gradient is defined for the p0 problem");
         KWH_REQUIRE(!absorbing && !nonlinear, KWH_EUNSUPPORTED, "the adjoint covers the lossless linear loop");
         KWH_REQUIRE(!psrc.present && !usrc[0].present && !usrc[1].present && !usrc[2].present, KWH_EUNSUPPORTED,
@@ -2168,6 +2412,7 @@ struct SimT final : kwh_sim {
         return n;
     }
     void set_graph(bool on) override {
+        KWH_REQUIRE(!axisym || !on, KWH_EUNSUPPORTED, "the axisymmetric solver runs eagerly (no step graphs)");
         sync();
         use_graph = on;
     }
@@ -2499,6 +2744,25 @@ int kwh_sim_create(kwh_ctx* ctx, const kwh_sim_desc* desc, kwh_sim** out) {
             *out = s.release();
         } else {
             auto s = std::make_unique<SimT<double>>();
+            s->setup(*desc, ctx->device);
+            *out = s.release();
+        }
+    });
+}
+
+int kwh_sim_create_axisymmetric(kwh_ctx* ctx, const kwh_sim_desc* desc, kwh_sim** out) {
+    return guarded([&] {
+        KWH_REQUIRE(ctx && out, KWH_EINVAL, "ctx/out is NULL");
+        *out = nullptr;
+        validate_desc(desc);
+        if (desc->dtype == KWH_FLOAT32) {
+            auto s = std::make_unique<SimT<float>>();
+            s->axisym = true;
+            s->setup(*desc, ctx->device);
+            *out = s.release();
+        } else {
+            auto s = std::make_unique<SimT<double>>();
+            s->axisym = true;
             s->setup(*desc, ctx->device);
             *out = s.release();
         }

@@ -5,7 +5,8 @@ host-side mirror of the reference's Swift API (Sources/KWave/...) plus input hel
 """
 from .api import (AbsorptionMode, DeviceKind, DirectivityPattern, DTypePrecision, KWaveGrid, KWaveMedium,
                   KWaveSensor, KWaveSource, RecordField, Simulation, SimulationBatch, SimulationOptions,
-                  SimulationOutput, SourceMode, flatNonzeroIndices, kspaceFirstOrder, timeReversal)
+                  SimulationOutput, SourceMode, flatNonzeroIndices, kspaceFirstOrder, kspaceFirstOrderAS,
+                  timeReversal)
 from .inputs import (db2neper, gaussian, getWin, makeBall, makeCartCircle, makeCircle, makeDisc, neper2db,
                      random_field, toneBurst)
 from .transducer import DelayMaskMode, KWaveTransducer, KWaveTransducerGeometry
@@ -14,6 +15,6 @@ __all__ = [
     "AbsorptionMode", "DeviceKind", "DirectivityPattern", "DTypePrecision", "KWaveGrid", "KWaveMedium",
     "KWaveSensor", "KWaveSource", "RecordField", "Simulation", "SimulationBatch", "SimulationOptions",
     "SimulationOutput", "timeReversal", "KWaveTransducer", "KWaveTransducerGeometry", "DelayMaskMode",
-    "SourceMode", "flatNonzeroIndices", "kspaceFirstOrder", "db2neper", "gaussian", "getWin", "makeBall",
+    "SourceMode", "flatNonzeroIndices", "kspaceFirstOrder", "kspaceFirstOrderAS", "db2neper", "gaussian", "getWin", "makeBall",
     "makeCartCircle", "makeCircle", "makeDisc", "neper2db", "random_field", "toneBurst",
 ]

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkwavehip
 # Every symbol include/kwave_hip.h declares.
 EXPORTED = [
     "kwh_abi_version", "kwh_last_error", "kwh_sim_validate", "kwh_ctx_create", "kwh_ctx_destroy", "kwh_sim_create",
+    "kwh_sim_create_axisymmetric",
     "kwh_sim_run", "kwh_sim_launch", "kwh_sim_sync", "kwh_sim_output_shape", "kwh_sim_read_output",
     "kwh_sim_steps_done", "kwh_sim_state_bytes", "kwh_sim_save_state", "kwh_sim_load_state",
     "kwh_sim_set_kernel_timing", "kwh_sim_set_kernel_timing_only", "kwh_sim_kernel_stats", "kwh_sim_set_graph",
@@ -85,6 +86,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.kwh_ctx_destroy.restype = None
     lib.kwh_sim_validate.argtypes = [C.POINTER(SimDesc)]
     lib.kwh_sim_create.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(vp)]
+    lib.kwh_sim_create_axisymmetric.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(vp)]
     lib.kwh_sim_run.argtypes = [vp, C.c_int32, C.c_int32, STEP_CB, C.c_int32, vp]
     lib.kwh_sim_launch.argtypes = [vp, C.c_int32, C.c_int32]
     lib.kwh_sim_sync.argtypes = [vp]

@@ -355,7 +355,7 @@ class Simulation:
     """
 
     def __init__(self, grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor: KWaveSensor,
-                 options: SimulationOptions = None, device: int = 0):
+                 options: SimulationOptions = None, device: int = 0, axisymmetric: bool = False):
         options = options or SimulationOptions()
         if options.device != DeviceKind.hip:
             raise NotImplementedError("this library implements device .hip only (the MLX path is not part of it)")
@@ -365,7 +365,8 @@ class Simulation:
         d, self._keep, self.n_sensor = build_desc(grid, medium, source, sensor, options)
         self.record = RecordField(d.record)
         self.handle = C.c_void_p()
-        L.check(self.lib.kwh_sim_create(_context(device), C.byref(d), C.byref(self.handle)))
+        create = self.lib.kwh_sim_create_axisymmetric if axisymmetric else self.lib.kwh_sim_create
+        L.check(create(_context(device), C.byref(d), C.byref(self.handle)))
 
     @classmethod
     def _view(cls, lib, handle, grid, dtype, record, n_sensor):
@@ -617,6 +618,27 @@ def kspaceFirstOrder(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource, 
     if source.p0 is None and source.pMask is None and source.uMask is None:
         raise ValueError(f"{grid.dim}D solver requires source.p0 or a time-varying p/u source")
     sim = Simulation(grid, medium, source, sensor if sensor is not None else KWaveSensor(), options)
+    return _run_to_output(sim, options)
+
+
+def kspaceFirstOrderAS(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor: KWaveSensor,
+                       options: SimulationOptions = None) -> SimulationOutput:
+    """Axisymmetric k-space solver on the HIP device (KSpaceFirstOrderAS.swift:17-309): a 2D grid
+    with x axial and y radial (y index 0 on the axis), WSWA-FFT radial symmetry, Stokes absorption,
+    B/A, sources p0 / p / ux / uy; records the pressure (p, pFinal)."""
+    options = options or SimulationOptions()
+    if grid.dim != 2:
+        raise ValueError("axisymmetric solver requires a 2D grid (x axial, y radial)")
+    if grid.nt <= 0:
+        raise ValueError("call grid.makeTime(...) before running the solver")
+    if source.p0 is None and source.pMask is None and source.uMask is None:
+        raise ValueError("axisymmetric solver requires source.p0 or a time-varying p/u source")
+    sim = Simulation(grid, medium, source, sensor if sensor is not None else KWaveSensor(), options,
+                     axisymmetric=True)
+    return _run_to_output(sim, options)
+
+
+def _run_to_output(sim: "Simulation", options: SimulationOptions) -> SimulationOutput:
     try:
         if options.progress is None and options.fieldMonitor is None:
             sim.run()

@@ -123,8 +123,9 @@ def _bcast(v, a, dim):
 # Operators (KSpaceFirstOrder.swift, PML.swift, Recording.swift)
 # ---------------------------------------------------------------------------------------------
 
-def pml_profile(n, dx, dt, c, pml_size, pml_alpha, staggered):
-    """Split-field PML exponential profile, power 4, independent sides (PML.swift:17-43)."""
+def pml_profile(n, dx, dt, c, pml_size, pml_alpha, staggered, axisymmetric=False):
+    """Split-field PML exponential profile, power 4, independent sides (PML.swift:17-43);
+    axisymmetric: no left-side ramp (the radial axis of the axisymmetric solver, :36-38)."""
     a = pml_alpha * (c / dx)
     m = float(pml_size)
     pml = np.ones(n)
@@ -134,7 +135,8 @@ def pml_profile(n, dx, dt, c, pml_size, pml_alpha, staggered):
             pos_l, pos_r = (m + 0.5 - x) / m, (x + 0.5) / m
         else:
             pos_l, pos_r = (m + 1 - x) / m, x / m
-        pml[k] = math.exp(-a * dt / 2.0 * pos_l ** 4)
+        if not axisymmetric:
+            pml[k] = math.exp(-a * dt / 2.0 * pos_l ** 4)
         pml[n - pml_size + k] = math.exp(-a * dt / 2.0 * pos_r ** 4)
     return pml
 
@@ -666,3 +668,223 @@ def closed_form_lossless(p0, sizes, spacings, c, dt, steps):
     kmag = k_magnitude(list(sizes), list(spacings))
     P0 = np.fft.fftn(p0)
     return [np.real(np.fft.ifftn(P0 * np.cos(c * kmag * n * dt))) for n in steps]
+
+
+# ---------------------------------------------------------------------------------------------
+# Axisymmetric solver (Sources/KWave/Solver/KSpaceFirstOrderAS.swift)
+# ---------------------------------------------------------------------------------------------
+
+def mirror_wswa(w):
+    """[w, 0, -w[:, ny-1..1], -w, 0, w[:, ny-1..1]] along the radial axis (:201-204)."""
+    nx = w.shape[0]
+    tail = w[:, 1:][:, ::-1]
+    z = np.zeros((nx, 1))
+    return np.concatenate([w, z, -tail, -w, z, tail], axis=1)
+
+
+def mirror_hahs(w):
+    """[w, flip w, -w, -flip w] (:205-208)."""
+    f = w[:, ::-1]
+    return np.concatenate([w, f, -w, -f], axis=1)
+
+
+def mirror_hsha(w):
+    """[w, -flip w, -w, flip w] (:209-212)."""
+    f = w[:, ::-1]
+    return np.concatenate([w, -f, -w, f], axis=1)
+
+
+def kspace_first_order_as(grid, medium, source, sensor, options, workers=1):
+    """Axisymmetric k-space solver with WSWA-FFT radial symmetry (KSpaceFirstOrderAS.swift:17-309).
+
+    The grid is [nx, ny] with x axial (periodic) and y radial (index 0 on the axis). Every field
+    is mirrored into a 4x radially expanded grid with its own symmetry (p, ux: WSWA; uy: HAHS;
+    uy / r: HSHA) and transformed there with full complex FFTs; the radial PML is one-sided and
+    the divergence gains uy / r on the staggered radial grid r = (j + 1/2) dy. Returns
+    {"p": [nSensor, nt] or None, "pFinal": [nx, ny]}.
+    """
+    fft = _NumpyFFT(workers)
+    if grid.dim != 2:
+        raise ValueError("axisymmetric solver requires a 2D grid (x axial, y radial)")
+    nx, ny = (int(v) for v in grid.size)
+    dx, dy = (float(v) for v in grid.spacing)
+    shape = (nx, ny)
+    dt = float(grid.dt)
+    nt = int(grid.nt)
+    if nt <= 0:
+        raise ValueError("call grid.makeTime(...) before running the solver")
+    if source.p0 is None and source.pMask is None and source.uMask is None:
+        raise ValueError("axisymmetric solver requires source.p0 or a time-varying p/u source")
+
+    c0 = np.asarray(medium.soundSpeed, dtype=np.float32).astype(np.float64)
+    rho0 = np.asarray(medium.density, dtype=np.float32).astype(np.float64)
+    if c0.size == 1:
+        c0 = c0.reshape(())
+    if rho0.size == 1:
+        rho0 = rho0.reshape(())
+    if options.smoothC0 and c0.size > 1:
+        c0 = smooth(c0, fft)
+    if options.smoothRho0 and rho0.size > 1:
+        rho0 = smooth(rho0, fft)
+    c_ref = float(np.max(c0))
+
+    # expanded grid (:52-67): kappa from the expanded |k|, axial / radial staggered operators
+    ny_e = 4 * ny
+    kx = wavenumber_vector(nx, dx)
+    ky_e = wavenumber_vector(ny_e, dy).reshape(1, ny_e)
+    k_exp = np.sqrt(kx.reshape(nx, 1) ** 2 + ky_e ** 2)
+    arg = k_exp * (c_ref * dt / 2.0)
+    kappa = np.where(arg == 0, 1.0, np.sin(arg) / np.where(arg == 0, 1.0, arg))
+    ddx_pos = derivative_operator(kx, dx, +1).reshape(nx, 1)
+    ddx_neg = derivative_operator(kx, dx, -1).reshape(nx, 1)
+    ddy_k = 1j * ky_e
+    theta = ky_e * (dy / 2.0)
+    y_shift_pos = np.cos(theta) + 1j * np.sin(theta)
+    y_shift_neg = np.cos(theta) - 1j * np.sin(theta)
+
+    # PMLs (:69-81): axial two-sided, radial outer edge only
+    pml_n = int(options.pmlSize)
+    pml_x = pml_profile(nx, dx, dt, c_ref, pml_n, options.pmlAlpha, False).reshape(nx, 1)
+    pml_x_sg = pml_profile(nx, dx, dt, c_ref, pml_n, options.pmlAlpha, True).reshape(nx, 1)
+    pml_y = pml_profile(ny, dy, dt, c_ref, pml_n, options.pmlAlpha, False, True).reshape(1, ny)
+    pml_y_sg = pml_profile(ny, dy, dt, c_ref, pml_n, options.pmlAlpha, True, True).reshape(1, ny)
+    inv_y_sg = (1.0 / ((np.arange(ny) + 0.5) * dy)).reshape(1, ny)      # (:84-86)
+
+    c2 = c0 * c0
+    dt_rho0 = dt * rho0
+    dt_over_rho_x = dt / stagger_density(rho0, 0)
+    dt_over_rho_y = dt / stagger_density(rho0, 1)
+
+    absorb_tau = None                                                     # (:93-101)
+    alpha_mode = _mode(medium.alphaMode, "noAbsorption")
+    if medium.alphaCoeff is not None and alpha_mode != "noAbsorption":
+        y = 2.0 if alpha_mode == "stokes" else float(medium.alphaPower if medium.alphaPower is not None else 2.0)
+        if abs(y - 2.0) >= 1e-10:
+            raise ValueError("axisymmetric absorption requires the Stokes model (alphaPower = 2)")
+        alpha = np.asarray(medium.alphaCoeff, dtype=np.float32).astype(np.float64)
+        absorb_tau = -2.0 * alpha * db2neper(1.0, 2.0) * c0
+    bona = None                                                           # (:103-108)
+    if medium.bOnA is not None:
+        b = np.asarray(medium.bOnA, dtype=np.float32).astype(np.float64)
+        if np.max(np.abs(b)) != 0:
+            bona = b
+
+    p0_field = None
+    if source.p0 is not None:
+        p0_field = np.asarray(source.p0, dtype=np.float32).astype(np.float64)
+        if options.smoothP0:
+            p0_field = smooth(p0_field, fft)
+
+    # time-varying sources, scaled as in kspaceFirstOrder_scaleSourceTerms (:152-163)
+    p_mode = _mode(source.pMode, "additive")
+    u_mode = _mode(source.uMode, "additive")
+    p_op = make_source_op(source.pMask, source.p, p_mode, c0,
+                          (lambda c: 1.0 / (2.0 * c * c)) if p_mode == "dirichlet"
+                          else (lambda c: 2.0 * dt / (2.0 * c * dx)), shape)
+    ux_op = make_source_op(source.uMask, source.ux, u_mode, c0,
+                           (lambda c: np.ones_like(c)) if u_mode == "dirichlet" else (lambda c: 2.0 * c * dt / dx),
+                           shape)
+    uy_op = make_source_op(source.uMask, source.uy, u_mode, c0,
+                           (lambda c: np.ones_like(c)) if u_mode == "dirichlet" else (lambda c: 2.0 * c * dt / dy),
+                           shape)
+    source_kappa = np.cos(arg)
+
+    def crop(spec):
+        return np.real(fft.ifftn(spec))[:, :ny]
+
+    def correct(mat, mirror):                                             # (:170-173)
+        return crop(source_kappa * fft.fftn(mirror(mat)))
+
+    def matrix_at(op, t):
+        v = op.values_at(t)
+        if v is None:
+            return None
+        buf = np.zeros(nx * ny)
+        buf[op.indices] = v
+        return buf.reshape(shape)
+
+    def apply_velocity_source(op, fld, t, mirror):                        # (:177-194)
+        if op is None:
+            return fld
+        if u_mode == "dirichlet":
+            v = op.values_at(t)
+            if v is None:
+                return fld
+            out = fld.copy().reshape(-1)
+            out[op.indices] = v
+            return out.reshape(shape)
+        mat = matrix_at(op, t)
+        if mat is None:
+            return fld
+        if u_mode == "additiveNoCorrection":
+            return fld + mat
+        return fld + correct(mat, mirror)
+
+    def pressure_gradients(pf):                                           # (:216-221)
+        pk = kappa * fft.fftn(mirror_wswa(pf))
+        return crop(ddx_pos * pk), crop(ddy_k * y_shift_pos * pk)
+
+    p = np.zeros(shape)
+    ux = np.zeros(shape)
+    uy = np.zeros(shape)
+    rhox = np.zeros(shape)
+    rhoy = np.zeros(shape)
+    sampler = make_sampler(sensor.mask, [nx, ny], [dx, dy]) if sensor.mask is not None else None
+    p_rec = []
+    for t in range(nt):
+        dpdx, dpdy = pressure_gradients(p)
+        ux = pml_x_sg * (pml_x_sg * ux - dt_over_rho_x * dpdx)
+        ux = apply_velocity_source(ux_op, ux, t, mirror_wswa)
+        uy = pml_y_sg * (pml_y_sg * uy - dt_over_rho_y * dpdy)
+        uy = apply_velocity_source(uy_op, uy, t, mirror_hahs)
+
+        ux_k = kappa * fft.fftn(mirror_wswa(ux))                           # (:243-251)
+        duxdx = crop(ddx_neg * ux_k)
+        uy_k = ddy_k * fft.fftn(mirror_hahs(uy)) + fft.fftn(mirror_hsha(inv_y_sg * uy))
+        duydy = crop(kappa * y_shift_neg * uy_k)
+
+        if bona is not None:                                              # (:253-261)
+            dt_rho_total = dt * (2.0 * (rhox + rhoy) + rho0)
+            rhox = pml_x * (pml_x * rhox - dt_rho_total * duxdx)
+            rhoy = pml_y * (pml_y * rhoy - dt_rho_total * duydy)
+        else:
+            rhox = pml_x * (pml_x * rhox - dt_rho0 * duxdx)
+            rhoy = pml_y * (pml_y * rhoy - dt_rho0 * duydy)
+
+        if p_op is not None:                                              # (:263-279)
+            if p_mode == "dirichlet":
+                v = p_op.values_at(t)
+                if v is not None:
+                    rhox = rhox.copy().reshape(-1)
+                    rhoy = rhoy.copy().reshape(-1)
+                    rhox[p_op.indices] = v
+                    rhoy[p_op.indices] = v
+                    rhox = rhox.reshape(shape)
+                    rhoy = rhoy.reshape(shape)
+            else:
+                mat = matrix_at(p_op, t)
+                if mat is not None:
+                    if p_mode == "additive":
+                        mat = correct(mat, mirror_wswa)
+                    rhox = rhox + mat
+                    rhoy = rhoy + mat
+
+        rho_total = rhox + rhoy                                           # (:281-287)
+        eos = rho_total
+        if absorb_tau is not None:
+            eos = eos + absorb_tau * rho0 * (duxdx + duydy)
+        if bona is not None:
+            eos = eos + bona * (rho_total * rho_total) / (2.0 * rho0)
+        p = c2 * eos
+
+        if t == 0 and p0_field is not None:                               # (:289-297)
+            p = p0_field.copy()
+            rhox = p0_field / (2.0 * c2)
+            rhoy = p0_field / (2.0 * c2)
+            dpdx0, dpdy0 = pressure_gradients(p)
+            ux = (dt_over_rho_x / 2.0) * dpdx0
+            uy = (dt_over_rho_y / 2.0) * dpdy0
+
+        if sampler is not None:
+            p_rec.append(sampler.sample(p))
+    return {"p": np.stack(p_rec, axis=1) if p_rec else None, "pFinal": p}
