@@ -136,19 +136,26 @@ private func makeDescriptor(grid: KWaveGrid, medium: KWaveMedium, source: KWaveS
     return (d, nSensor)
 }
 
-private func createSim(_ d: inout kwh_sim_desc) -> OpaquePointer? {
+private func createSim(_ d: inout kwh_sim_desc, axisymmetric: Bool = false) -> OpaquePointer? {
     if sharedContext == nil { check(kwh_ctx_create(0, &sharedContext), "kwh_ctx_create") }
     var sim: OpaquePointer? = nil
-    check(kwh_sim_create(sharedContext, &d, &sim), "kwh_sim_create")
+    if axisymmetric {
+        check(kwh_sim_create_axisymmetric(sharedContext, &d, &sim), "kwh_sim_create_axisymmetric")
+    } else {
+        check(kwh_sim_create(sharedContext, &d, &sim), "kwh_sim_create")
+    }
     return sim
 }
 
+/// `axisymmetric: true` runs kspaceFirstOrderAS (KSpaceFirstOrderAS.swift:17-309) on the device:
+/// x axial, y radial; the output carries p and pFinal.
 func kspaceFirstOrderHIP(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource,
-                         sensor: KWaveSensor, options: SimulationOptions) -> SimulationOutput {
+                         sensor: KWaveSensor, options: SimulationOptions,
+                         axisymmetric: Bool = false) -> SimulationOutput {
     let arena = HostArena()
     var (d, nSensor) = makeDescriptor(grid: grid, medium: medium, source: source, sensor: sensor,
                                       options: options, arena: arena)
-    let sim = createSim(&d)
+    let sim = createSim(&d, axisymmetric: axisymmetric)
     defer { kwh_sim_destroy(sim) }
 
     if options.progress == nil && options.fieldMonitor == nil {
