@@ -619,6 +619,25 @@ __global__ void __launch_bounds__(256) z_density(RhoArgs<T> a, Geo g, ZTile zt0,
 
 // EOS with power-law absorption/dispersion (KSpaceFirstOrder.swift:836-845, :694-702):
 // p = c^2 (rhoT + tau * A1 - eta * A2 + B/A rhoT^2 / (2 rho0)); P = R2C_z(p).
+// Adjoint row kernels (reverse sweep of the lossless linear loop; the recursion is documented in
+// kwh_adjoint.h): a diagonal update fused into a contiguous-axis transform.
+//   forward mode 0: w = scale r s^spow lu;  mode 1: lr = c^2 lp + s^2 lr (stored), w = dt rho0 s lr;
+//   then spec = R2C_z(w).
+//   inverse mode 0: dst = C2R_z(spec); 1: dst += C2R_z(spec); 2: dst = s^2 dst + C2R_z(spec).
+// s is the PML profile along internal axis ia (staggered for the velocity-side terms).
+template <typename T> struct AdjRowArgs {
+    const T* lu;
+    T* lr;
+    const T* lp;
+    SMap<T> r;      // dt / rho0 staggered along the axis
+    SMap<T> c0, rho0;
+    const T* s;
+    int ia;
+    int spow;
+    T scale, dt;
+    cx<T>* spec;
+};
+
 template <typename T> struct EosArgs {
     const cx<T>* A1;
     const cx<T>* A2;       // null without dispersion

@@ -619,6 +619,94 @@ __global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_EOS_WAVES)))
     reg_z_eos_body<T, J, E>(bidx_here(), a, g, P, tab);
 }
 
+// Adjoint forward row kernel (AdjRowArgs, kwh_kernels.h): the diagonal update of the reverse sweep
+// computed on the row pair in registers, then R2C_z into a.spec.
+template <typename T, int J, int E, int MODE>
+__device__ __forceinline__ void reg_adj_fwd_body(const BIdx& bi, const AdjRowArgs<T>& a, const Geo& g, int P,
+                                                 const AxisTabs<T>& tab) {
+    using S = RegRow<T, J, E>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    RegZPos<J> ps(P, bi);
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+    const bool zax = a.ia == 2;
+    const T* sz = a.s + ps.j;
+    const T sa0 = zax ? T(1) : a.s[axis_pos(g, ra, 0, a.ia)];
+    const T sb0 = zax ? T(1) : a.s[axis_pos(g, rb, 0, a.ia)];
+    cx<T> v[E];
+    if constexpr (MODE == 0) {
+        const T* rpa = a.r.lane(oa, ps.j);
+        const T* rpb = a.r.lane(ob, ps.j);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const T qa = zax ? sz[J * m] : sa0, qb = zax ? sz[J * m] : sb0;
+            const T fa = a.spow == 2 ? qa * qa : qa, fb = a.spow == 2 ? qb * qb : qb;
+            const T xa = a.scale * rpa[J * m] * fa * a.lu[oa + J * m];
+            const T xb = a.scale * rpb[J * m] * fb * a.lu[ob + J * m];
+            v[m] = cx<T>{va ? xa : T(0), vb ? xb : T(0)};
+        }
+    } else {
+        const T* cpa = a.c0.lane(oa, ps.j);
+        const T* cpb = a.c0.lane(ob, ps.j);
+        const T* mpa = a.rho0.lane(oa, ps.j);
+        const T* mpb = a.rho0.lane(ob, ps.j);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const T qa = zax ? sz[J * m] : sa0, qb = zax ? sz[J * m] : sb0;
+            const T ca = cpa[J * m], cb = cpb[J * m];
+            const T la = ca * ca * a.lp[oa + J * m] + qa * qa * a.lr[oa + J * m];
+            const T lb = cb * cb * a.lp[ob + J * m] + qb * qb * a.lr[ob + J * m];
+            if (va) a.lr[oa + J * m] = la;
+            if (vb) a.lr[ob + J * m] = lb;
+            v[m] = cx<T>{va ? a.dt * mpa[J * m] * qa * la : T(0), vb ? a.dt * mpb[J * m] * qb * lb : T(0)};
+        }
+    }
+    S::forward_store(a.spec, ps.ra, ps.rb, g.nrows, g.mzh, ldsb + S::L + ps.w * S::LP, ps.j, lt, v);
+}
+template <typename T, int J, int E, int MODE>
+__global__ void __launch_bounds__(256) reg_adj_fwd(AdjRowArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    reg_adj_fwd_body<T, J, E, MODE>(bidx_here(), a, g, P, tab);
+}
+
+// Adjoint inverse row kernel: C2R_z(spec) stored (mode 0), accumulated (1) or added to the
+// decayed field s^2 dst (2).
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_adj_inv_body(const BIdx& bi, const cx<T>* spec, T* dst, const T* s, int ia,
+                                                 int mode, const Geo& g, int P, const AxisTabs<T>& tab) {
+    using S = RegRow<T, J, E>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    RegZPos<J> ps(P, bi);
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    cx<T> v[E];
+    S::load_spectrum(spec, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+    S::template transform<true>(v, ldsb + S::L + ps.w * S::LP, ps.j, lt);
+    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    const bool zax = ia == 2;
+    const T sa0 = mode == 2 && !zax ? s[axis_pos(g, ra, 0, ia)] : T(1);
+    const T sb0 = mode == 2 && !zax ? s[axis_pos(g, rb, 0, ia)] : T(1);
+    const T* sz = mode == 2 && zax ? s + ps.j : nullptr;
+    T* da = dst + (int64_t)ra * S::L + ps.j;
+    T* db = dst + (int64_t)rb * S::L + ps.j;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const T qa = sz ? sz[J * m] : sa0, qb = sz ? sz[J * m] : sb0;
+        if (va) da[J * m] = mode == 0 ? v[m].x : (mode == 1 ? da[J * m] + v[m].x : da[J * m] * (qa * qa) + v[m].x);
+        if (vb) db[J * m] = mode == 0 ? v[m].y : (mode == 1 ? db[J * m] + v[m].y : db[J * m] * (qb * qb) + v[m].y);
+    }
+}
+template <typename T, int J, int E>
+__global__ void __launch_bounds__(256) reg_adj_inv(const cx<T>* spec, T* dst, const T* s, int ia, int mode, Geo g,
+                                                   int P, AxisTabs<T> tab) {
+    reg_adj_inv_body<T, J, E>(bidx_here(), spec, dst, s, ia, mode, g, P, tab);
+}
+
 // ------------------------------------------------------------------------------------------
 // Strided / middle passes over a tile of NTR contiguous columns: lane (w, j), w fastest.
 // LDS: element (n1, k2) of step 1 of column w at (n1*E + k2)*NTR + w + n1 (pad one slot per E

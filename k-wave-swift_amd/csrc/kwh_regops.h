@@ -27,6 +27,11 @@ struct RegOps {
     void (*z_density)(const Emit&, dim3, dim3, size_t, const RhoArgs<T>&, const Geo&, int, const AxisTabs<T>&,
                       bool nonlinear, int mode, bool corr);
     void (*z_eos)(const Emit&, dim3, dim3, size_t, const EosArgs<T>&, const Geo&, int, const AxisTabs<T>&);
+    // adjoint row kernels (AdjRowArgs): forward mode 0 / 1, inverse mode 0 / 1 / 2
+    void (*adj_fwd)(const Emit&, dim3, dim3, size_t, const AdjRowArgs<T>&, const Geo&, int, const AxisTabs<T>&,
+                    int mode);
+    void (*adj_inv)(const Emit&, dim3, dim3, size_t, const cx<T>*, T*, const T*, int ia, int mode, const Geo&, int,
+                    const AxisTabs<T>&);
     // column kernels over tiles of ntr columns (ntr_max or ntr_max / 2), ntr * J lanes
     void (*strided)(const Emit&, dim3, dim3, size_t, const StridedArgs&, const cx<T>*, bool inv, int ntr);
     void (*middle)(const Emit&, dim3, dim3, size_t, const MidArgs&, const AxisTabs<T>&, const Geo&, int ntr);

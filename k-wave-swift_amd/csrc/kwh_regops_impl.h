@@ -66,6 +66,16 @@ struct RegImpl {
         em.go<Body<&reg_z_eos_body<T, J, E>, &reg_z_eos<T, J, E>, reg_waves<T, E>(KWH_EOS_WAVES), false>>(
             gr, bl, lds, ea, g, P, tab);
     }
+    static void adj_fwd(const Emit& em, dim3 gr, dim3 bl, size_t lds, const AdjRowArgs<T>& a, const Geo& g, int P,
+                        const AxisTabs<T>& tab, int mode) {
+        if (mode == 0) em.go<Body<&reg_adj_fwd_body<T, J, E, 0>, &reg_adj_fwd<T, J, E, 0>, 1, false>>(gr, bl, lds, a, g, P, tab);
+        else em.go<Body<&reg_adj_fwd_body<T, J, E, 1>, &reg_adj_fwd<T, J, E, 1>, 1, false>>(gr, bl, lds, a, g, P, tab);
+    }
+    static void adj_inv(const Emit& em, dim3 gr, dim3 bl, size_t lds, const cx<T>* spec, T* dst, const T* s, int ia,
+                        int mode, const Geo& g, int P, const AxisTabs<T>& tab) {
+        em.go<Body<&reg_adj_inv_body<T, J, E>, &reg_adj_inv<T, J, E>, 1, false>>(gr, bl, lds, spec, dst, s, ia, mode, g,
+                                                                                 P, tab);
+    }
     static void strided(const Emit& em, dim3 gr, dim3 bl, size_t lds, const StridedArgs& a, const cx<T>* tw, bool inv,
                         int ntr) {
         if (ntr == NTR) {
@@ -84,7 +94,7 @@ struct RegImpl {
 
     static const RegOps<T>* ops() {
         static const RegOps<T> o{J, E, NTR, &z_forward_real, &z_inverse_real, &z_colloc_rows, &z_velocity,
-                                 &z_density, &z_eos, &strided, &middle};
+                                 &z_density, &z_eos, &adj_fwd, &adj_inv, &strided, &middle};
         return &o;
     }
 };

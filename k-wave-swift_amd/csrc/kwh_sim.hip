@@ -2019,15 +2019,44 @@ struct SimT final : kwh_sim {
     }
 
     // dst (= or +=) sum_a D-_a[scale r_a s_a^spow lu_a]
+    // Adjoint row kernels (register-resident path): a diagonal update of the reverse sweep fused
+    // into the contiguous-axis transform, followed / preceded by the plain axis-1 passes.
+    void adj_forward(const AdjRowArgs<T>& aa, int mode) {
+        const RegZ z = regz();
+        timed(K_ZFR, real_bytes() * (mode == 0 ? 2 : 4) + spec_bytes(), [&] {
+            rx[2]->adj_fwd(em(), dim3(z.blocks), dim3(z.threads), z.lds, aa, g, z.P, tab, mode);
+        });
+        if (plain1) strided(1, false, {(void*)aa.spec});
+    }
+    void adj_inverse(C* sp, T* dst, const T* s, int ia, int mode) {
+        if (plain1) strided(1, true, {(void*)sp});
+        const RegZ z = regz();
+        timed(K_ZIR, real_bytes() * (mode == 0 ? 1 : 2) + spec_bytes(), [&] {
+            rx[2]->adj_inv(em(), dim3(z.blocks), dim3(z.threads), z.lds, (const C*)sp, dst, s, ia, mode, g, z.P, tab);
+        });
+    }
+
     void adj_sum_dneg(T* dst, bool accumulate, const std::vector<DBuf>& lu, int spow, T scale, DBuf& w) {
         std::vector<const void*> ins;
         std::vector<MidOp> ops;
         for (int a = 0; a < dim; ++a) {
             const int ia = a + g.off;
-            hipLaunchKernelGGL((adj_prediv_kernel<T>), dim3(grid_blocks(g.N)), dim3(256), 0, stream, w.as<T>(),
-                               lu[a].as<T>(), dtrs[a], tab.pmlsg[ia], ia, spow, scale, g);
-            HIPCHK(hipGetLastError());
-            forward_to_middle(w.as<T>(), spec[a].as<C>(), T(1));
+            if (rx[2]) {   // w = scale r_a s_a^spow lu_a fused into the forward transform
+                AdjRowArgs<T> aa{};
+                aa.lu = lu[a].as<T>();
+                aa.r = dtrs[a];
+                aa.s = tab.pmlsg[ia];
+                aa.ia = ia;
+                aa.spow = spow;
+                aa.scale = scale;
+                aa.spec = spec[a].as<C>();
+                adj_forward(aa, 0);
+            } else {
+                hipLaunchKernelGGL((adj_prediv_kernel<T>), dim3(grid_blocks(g.N)), dim3(256), 0, stream, w.as<T>(),
+                                   lu[a].as<T>(), dtrs[a], tab.pmlsg[ia], ia, spow, scale, g);
+                HIPCHK(hipGetLastError());
+                forward_to_middle(w.as<T>(), spec[a].as<C>(), T(1));
+            }
             ins.push_back(spec[a].p);
             ops.push_back(MidOp{OP_DDXN_KAPPA, ia, a, spec[a].p, kappa.p});
         }
@@ -2117,16 +2146,34 @@ struct SimT final : kwh_sim {
                 std::vector<MidOp> ops;
                 for (int a = 0; a < dim; ++a) {
                     const int ia = a + g.off;
-                    hipLaunchKernelGGL((adj_rho_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream, lr[a].as<T>(),
-                                       w.as<T>(), (const T*)lp.as<T>(), c0s, rho0s, T(dt), tab.pml[ia], ia, g);
-                    HIPCHK(hipGetLastError());
-                    forward_to_middle(w.as<T>(), spec[a].as<C>(), T(1));
+                    if (rx[2]) {   // lr update and w = m q lr fused into the forward transform
+                        AdjRowArgs<T> aa{};
+                        aa.lr = lr[a].as<T>();
+                        aa.lp = lp.as<T>();
+                        aa.c0 = c0s;
+                        aa.rho0 = rho0s;
+                        aa.s = tab.pml[ia];
+                        aa.ia = ia;
+                        aa.dt = T(dt);
+                        aa.spec = spec[a].as<C>();
+                        adj_forward(aa, 1);
+                    } else {
+                        hipLaunchKernelGGL((adj_rho_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream,
+                                           lr[a].as<T>(), w.as<T>(), (const T*)lp.as<T>(), c0s, rho0s, T(dt),
+                                           tab.pml[ia], ia, g);
+                        HIPCHK(hipGetLastError());
+                        forward_to_middle(w.as<T>(), spec[a].as<C>(), T(1));
+                    }
                     ins.push_back(spec[a].p);
                     ops.push_back(MidOp{OP_DDXP_KAPPA, ia, a, spec[a].p, kappa.p});
                 }
                 middle(K_MG, ins, ops);
                 for (int a = 0; a < dim; ++a) {
                     const int ia = a + g.off;
+                    if (rx[2]) {   // lu = s^2 lu + C2R(...) in one pass
+                        adj_inverse(spec[a].as<C>(), lu[a].as<T>(), tab.pmlsg[ia], ia, 2);
+                        continue;
+                    }
                     hipLaunchKernelGGL((adj_u_decay_kernel<T>), dim3(grid_blocks(N)), dim3(256), 0, stream,
                                        lu[a].as<T>(), tab.pmlsg[ia], ia, g);
                     HIPCHK(hipGetLastError());
