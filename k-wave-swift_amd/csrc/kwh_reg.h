@@ -31,6 +31,9 @@
 #ifndef KWH_EOS_WAVES
 #define KWH_EOS_WAVES 3
 #endif
+#ifndef KWH_DENSITY_HOLD_RHO0
+#define KWH_DENSITY_HOLD_RHO0 1
+#endif
 
 namespace kwh {
 
@@ -415,6 +418,13 @@ __device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
     const T* r0pa = a.rho0.lane(oa, ps.j);
     const T* r0pb = a.rho0.lane(ob, ps.j);
+    // rho0 of the row pair held in registers across the axes (one HBM read instead of one per axis)
+    constexpr bool HOLD = KWH_DENSITY_HOLD_RHO0 && E <= 16 && sizeof(T) == 4;
+    cx<T> r0h[HOLD ? E : 1];
+    if constexpr (HOLD) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) r0h[m] = cx<T>{r0pa[J * m], r0pb[J * m]};
+    }
     cx<T> rt[E];
     cx<T> sd[ACC ? E : 1];
     cx<T> rp[NL ? E : 1];
@@ -443,8 +453,8 @@ __device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs
         const T* cra = a.corr.lane(oa, ps.j);
         const T* crb = a.corr.lane(ob, ps.j);
         auto upd = [&](int m, T qa, T qb) {
-            const T r0a = r0pa[J * m];
-            const T r0b = r0pb[J * m];
+            const T r0a = HOLD ? r0h[m].x : r0pa[J * m];
+            const T r0b = HOLD ? r0h[m].y : r0pb[J * m];
             const T da = r0a * v[m].x, db = r0b * v[m].y;
             if constexpr (ACC) {
                 if (ax == 0) sd[m] = cx<T>{da, db};
@@ -506,8 +516,8 @@ __device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             T ea = rt[m].x, eb = rt[m].y;
-            const T r0a = r0pa[J * m];
-            const T r0b = r0pb[J * m];
+            const T r0a = HOLD ? r0h[m].x : r0pa[J * m];
+            const T r0b = HOLD ? r0h[m].y : r0pb[J * m];
             if constexpr (MODE == 2) {
                 ea += tpa[J * m] * sd[m].x;
                 eb += tpb[J * m] * sd[m].y;
