@@ -250,6 +250,77 @@ def adjoint_comparison(n_dirs=16):
             "grad_dtype": "f32", "fd_dtype": "f64"}
 
 
+# The reference's own k-space benchmark scenarios (Sources/KWaveBench/main.swift:84-222) with the
+# wall times read off its published chart (Sources/KWaveBench/k-wave-python vs swift.png, MLX on an
+# unspecified Apple-silicon GPU, +-15%), as Mcell-steps/s; None where only a speedup is legible.
+REF_PUBLISHED = {"kspace2d_64": 18.0, "kspace2d_128": 70.0, "kspace2d_256": 160.0, "kspace2d_512": 224.0,
+                 "kspace2d_1024": 283.0, "kspace2d_256_hetero": None, "kspace2d_256_absorbing": None,
+                 "kspace2d_256_source": None, "kspace3d_32": 65.0, "kspace3d_64": 125.0, "kspace3d_128": 137.0}
+
+
+def reference_scenarios(repeats=5):
+    """Each scenario as KWaveBench times it: the whole solver call (setup, the loop, read-back of
+    the output), one warmup then the median of `repeats`; dx = 1e-4 m, c = 1500 m/s,
+    rho = 1000 kg/m^3, dt from CFL 0.3, PML inside (20 in 2D, 10 in 3D), smoothed p0."""
+    from kwavehip import (AbsorptionMode, KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, RecordField,
+                          SimulationOptions, SourceMode, kspaceFirstOrder, makeBall, makeDisc, toneBurst)
+
+    def step_map(shape, base):                     # main.swift:77-83
+        m = np.full(shape, base, np.float32)
+        m[shape[0] // 2:] = np.float32(base * 1.2)
+        return m
+
+    def case2d(n, nt, kind=""):
+        grid = KWaveGrid(n, 1e-4, n, 1e-4)
+        grid.makeTime(1500.0, cfl=0.3)
+        grid.setTime(nt, grid.dt)
+        med = KWaveMedium(1500.0, 1000.0)
+        if kind == "hetero":
+            med = KWaveMedium(step_map((n, n), 1500.0), step_map((n, n), 1000.0))
+        elif kind == "absorbing":
+            med = KWaveMedium(1500.0, 1000.0, alphaCoeff=0.75, alphaPower=1.5, alphaMode=AbsorptionMode.powerLaw)
+        if kind == "source":                       # main.swift:127-163
+            pm = np.zeros((n, n), np.float32)
+            pm[20, :] = 1
+            sm = np.zeros((n, n), np.float32)
+            sm[n - 20, :] = 1
+            src = KWaveSource(pMask=pm, p=toneBurst(1 / grid.dt, 1e6, 5).astype(np.float32),
+                              pMode=SourceMode.additive)
+            return grid, med, src, KWaveSensor(mask=sm, record=RecordField.p), SimulationOptions(pmlSize=20,
+                                                                                                 smoothP0=False)
+        src = KWaveSource(p0=makeDisc(n, n, radius=n / 16))
+        return grid, med, src, KWaveSensor(record=RecordField.pFinal), SimulationOptions(pmlSize=20, smoothP0=True)
+
+    def case3d(n, nt):
+        grid = KWaveGrid(n, 1e-4, n, 1e-4, n, 1e-4)
+        grid.makeTime(1500.0, cfl=0.3)
+        grid.setTime(nt, grid.dt)
+        src = KWaveSource(p0=makeBall(n, n, n, radius=n / 8))
+        return (grid, KWaveMedium(1500.0, 1000.0), src, KWaveSensor(record=RecordField.pFinal),
+                SimulationOptions(pmlSize=10, smoothP0=True))
+
+    cases = [("kspace2d_64", case2d(64, 256)), ("kspace2d_128", case2d(128, 256)), ("kspace2d_256", case2d(256, 256)),
+             ("kspace2d_512", case2d(512, 256)), ("kspace2d_1024", case2d(1024, 256)),
+             ("kspace2d_256_hetero", case2d(256, 256, "hetero")), ("kspace2d_256_absorbing", case2d(256, 256, "absorbing")),
+             ("kspace2d_256_source", case2d(256, 256, "source")), ("kspace3d_32", case3d(32, 128)),
+             ("kspace3d_64", case3d(64, 128)), ("kspace3d_128", case3d(128, 128))]
+    out = {}
+    for name, (grid, med, src, sens, opts) in cases:
+        kspaceFirstOrder(grid, med, src, sens, opts)
+        secs = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            kspaceFirstOrder(grid, med, src, sens, opts)
+            secs.append(time.perf_counter() - t0)
+        med_s = sorted(secs)[len(secs) // 2]
+        v = grid.totalGridPoints * grid.nt / med_s / 1e6
+        ref = REF_PUBLISHED[name]
+        out[name] = {"median_s": round(med_s, 5), "Mcell_steps_per_s": round(v, 1), "reference_mlx_gpu": ref,
+                     "ratio": round(v / ref, 1) if ref else None}
+        print(f"[bench] {name}: {med_s * 1e3:.2f} ms, {v:.0f} Mcell-steps/s", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(grid, med, src, sens, opts, steps, workers):
     """Time the float64 oracle (numpy/scipy pocketfft, `workers` threads) on `steps` steps."""
     from oracle import kwave_oracle as O
@@ -263,7 +334,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)   # at least 2: step 0 is never timed
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c2")   # c2 (default), c0, or refbench (the reference's scenarios)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -276,6 +347,13 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "refbench":
+        res = reference_scenarios()
+        print(json.dumps({"metric": "Mcell-steps/s per KWaveBench scenario (whole solver call, median of 5)",
+                          "unit": "Mcell-steps/s", "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+                          "scenarios": res}), flush=True)
+        return
 
     from kwavehip import Simulation
     t_setup = time.perf_counter()
