@@ -321,12 +321,31 @@ def reference_scenarios(repeats=5):
     return out
 
 
-def cpu_baseline(grid, med, src, sens, opts, steps, workers):
-    """Time the float64 oracle (numpy/scipy pocketfft, `workers` threads) on `steps` steps."""
+def device_copy_bandwidth():
+    """Event-timed bandwidth of a plain 1 GiB -> 1 GiB device copy (kwavehip/copy_probe, a child
+    process): the practical HBM ceiling the solver's kernels are compared with."""
+    import subprocess
+    exe = os.path.join(ROOT, "k-wave-swift_amd", "kwavehip", "copy_probe")
+    if not os.path.exists(exe):
+        print("[bench] copy_probe not built: device copy bandwidth not measured", file=sys.stderr, flush=True)
+        return None
+    res = subprocess.run([exe, "20"], capture_output=True, text=True, timeout=120)
+    if res.returncode != 0:
+        print(f"[bench] copy_probe failed: {res.stderr.strip()}", file=sys.stderr, flush=True)
+        return None
+    rec = json.loads(res.stdout.strip().splitlines()[-1])
+    return {"GBps": rec["GBps"], "frac_of_8TBps": round(rec["GBps"] / 8000.0, 4), "avg_us": rec["avg_us"],
+            "bytes_per_launch": rec["bytes_per_launch"], "kernel": "copy_kernel (k-wave-swift_amd/csrc/copy_probe.hip)"}
+
+
+def cpu_baseline(grid, med, src, sens, opts, steps, workers, dtype=np.float64):
+    """Time the oracle (numpy/scipy pocketfft, `workers` threads) computing in `dtype` (float32:
+    state, operators and FFTs all single precision): 1 warmup step (the t = 0 step, which also
+    applies p0), then `steps` timed steps; the rate is taken from the median step time."""
     from oracle import kwave_oracle as O
-    out = O.kspace_first_order(grid, med, src, sens, opts, workers=workers, t_end=steps)
-    sec = out["_loop_seconds"]
-    return grid.totalGridPoints * steps / sec / 1e6, sec
+    out = O.kspace_first_order(grid, med, src, sens, opts, dtype=dtype, workers=workers, t_end=steps + 1)
+    sec = float(np.median(out["_step_seconds"][1:]))
+    return grid.totalGridPoints / sec / 1e6, sec
 
 
 def main():
@@ -443,15 +462,27 @@ def main():
         adjoint = adjoint_comparison()
         print(f"[bench] adjoint/forward {adjoint['adjoint_over_forward']}", file=sys.stderr, flush=True)
 
-    cpu = None
+    copy_bw = device_copy_bandwidth()
+    if copy_bw:
+        print(f"[bench] device copy {copy_bw['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+
+    cpu = cpu_other = None
     if not args.no_cpu_baseline:
         workers = min(os.cpu_count() or 1, 16)
-        print(f"[bench] cpu baseline: float64 oracle, {args.cpu_steps} steps, {workers} threads", file=sys.stderr,
-              flush=True)
-        v, sec = cpu_baseline(grid, med, src, sens, opts, args.cpu_steps, workers)
-        cpu = {"value": round(v, 3), "unit": "Mcell-steps/s", "cores": workers, "kind": "port",
-               "sample": f"first {args.cpu_steps} steps of the same workload, float64 numpy/scipy oracle, "
-                         f"{sec:.1f}s of loop time"}
+        legs = {}
+        for tag, npdt in (("f32", np.float32), ("f64", np.float64)):
+            print(f"[bench] cpu baseline: {tag} oracle, {args.cpu_steps} steps, {workers} threads", file=sys.stderr,
+                  flush=True)
+            v, sec = cpu_baseline(grid, med, src, sens, opts, args.cpu_steps, workers, npdt)
+            legs[tag] = {"value": round(v, 3), "unit": "Mcell-steps/s", "cores": workers, "kind": "port",
+                         "dtype": tag,
+                         "sample": f"steps 1..{args.cpu_steps} of the same workload after the t = 0 step as "
+                                   f"warmup, median step {sec:.2f} s, numpy/scipy oracle in "
+                                   f"{'float32' if tag == 'f32' else 'float64'}"}
+            print(f"[bench] cpu {tag}: {v:.2f} Mcell-steps/s", file=sys.stderr, flush=True)
+        # the headline baseline computes in the run's own precision; the other one is reported beside it
+        cpu = legs[args.dtype]
+        cpu_other = legs["f64" if args.dtype == "f32" else "f32"]
 
     line = {
         "metric": "Mcell-steps/s 3D 256^3 FP32 (err vs FP64); adjoint/forward time; batched vs plain",
@@ -459,7 +490,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": args.dtype, "data": "synthetic (seeded random media)",
         "config": dict(info, dtype=args.dtype),
-        "roofline": roof, "step_model": step_model, "kernels": kernels, "cpu_baseline": cpu, "fp64_gpu": fp64,
+        "roofline": roof, "step_model": step_model, "device_copy": copy_bw, "kernels": kernels, "cpu_baseline": cpu,
+        "cpu_baseline_other_dtype": cpu_other, "fp64_gpu": fp64,
         "batched_scan_lines": batch, "adjoint": adjoint,
         "setup_s": round(setup_s, 2),
         # whole job seen from the host boundary: upload + setup, the timed steps, read-back

@@ -583,10 +583,27 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
     colloc = [_bcast(collocation_op(kv[a], spac[a]), a, dim) for a in range(dim)] if rec_u else None
 
     cast = (lambda x: x) if R is np.float64 else (lambda x: np.asarray(x, dtype=R))
+    if R is not np.float64:
+        # single precision end to end: the operators in float32 / complex64 keep every product and
+        # every pocketfft call (complex64 in, complex64 out) at the working precision
+        C = np.result_type(R, np.complex64).type
+        kappa = kappa.astype(R)
+        ddx_pos = [d.astype(C) for d in ddx_pos]
+        ddx_neg = [d.astype(C) for d in ddx_neg]
+        pml = [x.astype(R) for x in pml]
+        pml_sg = [x.astype(R) for x in pml_sg]
+        dt_over_rho = [np.asarray(x, dtype=R) for x in dt_over_rho]
+        dt_rho0 = np.asarray(dt_rho0, dtype=R)
+        c2 = np.asarray(c2, dtype=R)
+        rho0 = np.asarray(rho0, dtype=R)          # eos_term and the nonlinear factor read it
+        if absorb is not None:
+            absorb = tuple(None if x is None else np.asarray(x, dtype=R) for x in absorb)
+        if bona is not None:
+            bona = np.asarray(bona, dtype=R)
     if state is None:
-        p = np.zeros(shape)
-        u = [np.zeros(shape) for _ in range(dim)]
-        rho = [np.zeros(shape) for _ in range(dim)]
+        p = np.zeros(shape, dtype=R)
+        u = [np.zeros(shape, dtype=R) for _ in range(dim)]
+        rho = [np.zeros(shape, dtype=R) for _ in range(dim)]
     else:
         p = state["p"].copy()
         u = [x.copy() for x in state["u"]]
@@ -595,8 +612,10 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
     t_stop = nt if t_end is None else t_end
     import time as _time
     loop_start = _time.perf_counter()
+    step_seconds = []
 
     for t in range(t_begin, t_stop):
+        step_start = _time.perf_counter()
         pk = fft.fftn(p)
         for a in range(dim):                                              # velStep
             g = np.real(fft.ifftn(ddx_pos[a] * kappa * pk))
@@ -644,6 +663,7 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
                 for a in range(dim):
                     uc = np.real(fft.ifftn(colloc[a] * fft.fftn(u[a])))
                     u_rec[a].append(sampler.sample(uc))
+        step_seconds.append(_time.perf_counter() - step_start)
 
     loop_seconds = _time.perf_counter() - loop_start
     out = finalize_recording(
@@ -656,6 +676,7 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
     if return_state:
         out["_state"] = {"p": p, "u": u, "rho": rho}
     out["_loop_seconds"] = loop_seconds
+    out["_step_seconds"] = step_seconds
     return out
 
 

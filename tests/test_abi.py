@@ -128,3 +128,13 @@ def test_python_api_errors_mirror_reference_preconditions():
     o.device = "gpu"
     with pytest.raises(NotImplementedError):
         kspaceFirstOrder(g, m, s, se, o)
+
+
+def test_c_driver_host_checks():
+    # the plain C caller (cdriver/kwh_ivp.c) links against the library and runs its host-only
+    # checks: ABI version, a valid descriptor, KWH_EINVAL with a message for nt = 0
+    import subprocess
+    exe = os.path.join(ROOT, "k-wave-swift_amd", "kwavehip", "kwh_ivp")
+    res = subprocess.run([exe, "--validate"], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0, res.stderr
+    assert '"validate": "ok"' in res.stdout and "makeTime" in res.stderr

@@ -336,3 +336,24 @@ def test_chunked_run_equals_full_run():
     b = O.kspace_first_order(grid, KWaveMedium(1500.0, 1000.0), src, KWaveSensor(), SimulationOptions(),
                              t_begin=12, state=a["_state"])
     assert np.array_equal(full["pFinal"], b["pFinal"])
+
+
+def test_float32_mode_stays_single_precision():
+    # the timing-only float32 mode: every state array is float32 and the result is the float64
+    # run to single-precision rounding (a stray float64 operator would not change the result,
+    # a dropped cast of the operators would leave the state in float64)
+    n = 32
+    grid = KWaveGrid(n, 1e-4, n, 1e-4)
+    grid.setTime(40, 2e-8)
+    med = KWaveMedium(1500.0, 1000.0, alphaCoeff=0.5, alphaPower=1.1, alphaMode=AbsorptionMode.powerLaw)
+    src = KWaveSource(p0=makeDisc(n, n, radius=4))
+    sens = KWaveSensor(mask=makeCircle(n, n, radius=10), record=RecordField.p | RecordField.ux)
+    r64 = O.kspace_first_order(grid, med, src, sens, SimulationOptions(pmlSize=6), return_state=True)
+    r32 = O.kspace_first_order(grid, med, src, sens, SimulationOptions(pmlSize=6), dtype=np.float32,
+                               return_state=True)
+    st = r32["_state"]
+    assert st["p"].dtype == np.float32
+    assert all(x.dtype == np.float32 for x in st["u"] + st["rho"])
+    for k in ("p", "ux"):
+        err = np.abs(r32[k] - r64[k]).max() / np.abs(r64[k]).max()
+        assert 1e-9 < err < 1e-5, (k, err)
