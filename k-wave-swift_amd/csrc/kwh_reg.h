@@ -31,6 +31,22 @@
 #ifndef KWH_EOS_WAVES
 #define KWH_EOS_WAVES 3
 #endif
+// Loads issued together with a kernel's first spectrum load, so their latency overlaps the
+// transform instead of following it (E <= 16 FP32): the EOS kernel's sum rho and c0 (it then runs
+// at KWH_EOS_PRELOAD_WAVES), the FFT-absorption density kernel's rho_a, the middle pass's operator
+// table. Measured at 256^3: EOS 102 -> 95 us, density 176 -> 167 us, middle passes -1..2 us.
+#ifndef KWH_EOS_PRELOAD
+#define KWH_EOS_PRELOAD 1
+#endif
+#ifndef KWH_EOS_PRELOAD_WAVES
+#define KWH_EOS_PRELOAD_WAVES 2
+#endif
+#ifndef KWH_DENSITY_PRELOAD
+#define KWH_DENSITY_PRELOAD 1
+#endif
+#ifndef KWH_MIDDLE_KAPPA_EARLY
+#define KWH_MIDDLE_KAPPA_EARLY 1
+#endif
 #ifndef KWH_DENSITY_HOLD_RHO0
 #define KWH_DENSITY_HOLD_RHO0 1
 #endif
@@ -310,6 +326,7 @@ __global__ void __launch_bounds__(256) reg_z_colloc_rows(const cx<T>* spec, T* _
 constexpr int kGroup = 8;      // density update
 constexpr int kVelGroup = 4;   // velocity update (keeps it within the three-wave register budget)
 
+
 // Velocity update, per axis: C2R_z(G_a) -> u_a = pml_sg (pml_sg u_a - dt/rho_a g) [+ sources] -> R2C_z.
 template <typename T, int J, int E, bool CORR>
 __device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArgs<T>& a, const Geo& g, int P,
@@ -328,9 +345,6 @@ __device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArg
     for (int ax = 0; ax < g.dim; ++ax) {
         const int ia = ax + g.off;
         cx<T> v[E];
-        S::load_spectrum(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-        S::template transform<true>(v, lds, ps.j, lt);
-        KWH_SCHED_FENCE();
         T* u = a.u[ax];
         const SMap<T> dtr = a.dtr[ax];
         const SMap<T> corr = a.corr[ax];
@@ -341,6 +355,9 @@ __device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArg
         const T* dtb = dtr.lane(ob, ps.j);
         const T* cra = corr.lane(oa, ps.j);
         const T* crb = corr.lane(ob, ps.j);
+        S::load_spectrum(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        S::template transform<true>(v, lds, ps.j, lt);
+        KWH_SCHED_FENCE();
         auto upd = [&](int m, T qa, T qb) {
             T xa = qa * (qa * ua[J * m] - dta[J * m] * v[m].x);
             T xb = qb * (qb * ub[J * m] - dtb[J * m] * v[m].y);
@@ -439,17 +456,24 @@ __device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs
             rp[m] = cx<T>{sa, sb};
         }
     }
+    // rho_a of the row pair loaded together with the divergence spectrum (KWH_DENSITY_PRELOAD)
+    constexpr bool PRE = KWH_DENSITY_PRELOAD && !NL && !CORR && MODE == 1 && E <= 16 && sizeof(T) == 4;
     for (int ax = 0; ax < g.dim; ++ax) {
         const int ia = ax + g.off;
         cx<T> v[E];
-        S::load_spectrum(a.D[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-        S::template transform<true>(v, lds, ps.j, lt);
-        KWH_SCHED_FENCE();
         T* rho = a.rho[ax];
-        const T* pm = tab.pml[ia];
-        const T cf = a.corr_factor[ax];
         const T* rha = rho + oa;
         const T* rhb = rho + ob;
+        S::load_spectrum(a.D[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        cx<T> rpre[PRE ? E : 1];
+        if constexpr (PRE) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) rpre[m] = cx<T>{rha[J * m], rhb[J * m]};
+        }
+        S::template transform<true>(v, lds, ps.j, lt);
+        KWH_SCHED_FENCE();
+        const T* pm = tab.pml[ia];
+        const T cf = a.corr_factor[ax];
         const T* cra = a.corr.lane(oa, ps.j);
         const T* crb = a.corr.lane(ob, ps.j);
         auto upd = [&](int m, T qa, T qb) {
@@ -465,8 +489,8 @@ __device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs
                 ma *= (T(2) * rp[m].x + r0a) / r0a;
                 mb *= (T(2) * rp[m].y + r0b) / r0b;
             }
-            T xa = qa * (qa * rha[J * m] - ma);
-            T xb = qb * (qb * rhb[J * m] - mb);
+            T xa = qa * (qa * (PRE ? rpre[m].x : rha[J * m]) - ma);
+            T xb = qb * (qb * (PRE ? rpre[m].y : rhb[J * m]) - mb);
             if constexpr (CORR) {
                 xa += cra[J * m] * cf;
                 xb += crb[J * m] * cf;
@@ -555,6 +579,10 @@ __global__ void __launch_bounds__(256, (density_waves<T, J, E, NL, MODE, CORR>()
 
 // EOS with FFT absorption: p = c^2 (rhoT + tau C2R_z(A1) - eta C2R_z(A2) [+ B/A rhoT^2 / (2 rho0)]),
 // P = R2C_z(p). Both spectra are loaded before the first transform (one memory phase).
+template <typename T, int E>
+constexpr int eos_waves() {
+    return reg_waves<T, E>(KWH_EOS_PRELOAD && E <= 16 && sizeof(T) == 4 ? KWH_EOS_PRELOAD_WAVES : KWH_EOS_WAVES);
+}
 template <typename T, int J, int E>
 __device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>& a, const Geo& g, int P,
                                                const AxisTabs<T>& tab) {
@@ -568,14 +596,23 @@ __device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>&
     const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
     cx<T> e[E], v[E];
-    S::load_spectrum(a.A1, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, e);
-    if (a.A2) S::load_spectrum(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
-    S::template transform<true>(e, lds, ps.j, lt);
-    if (a.A2) S::template transform<true>(v, lds, ps.j, lt);
-    KWH_SCHED_FENCE();
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
     const T* cpa = a.c0.lane(oa, ps.j);
     const T* cpb = a.c0.lane(ob, ps.j);
+    S::load_spectrum(a.A1, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, e);
+    if (a.A2) S::load_spectrum(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+    constexpr bool PRE = KWH_EOS_PRELOAD && E <= 16 && sizeof(T) == 4;
+    cx<T> rtp[PRE ? E : 1], cp[PRE ? E : 1];   // sum rho and c0 of the row pair, loaded with the spectra
+    if constexpr (PRE) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            rtp[m] = cx<T>{a.rhoT[oa + J * m], a.rhoT[ob + J * m]};
+            cp[m] = cx<T>{cpa[J * m], cpb[J * m]};
+        }
+    }
+    S::template transform<true>(e, lds, ps.j, lt);
+    if (a.A2) S::template transform<true>(v, lds, ps.j, lt);
+    KWH_SCHED_FENCE();
     const T* tpa = a.tau.lane(oa, ps.j);
     const T* tpb = a.tau.lane(ob, ps.j);
     const T* epa = a.eta.lane(oa, ps.j);
@@ -589,8 +626,8 @@ __device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>&
 #pragma unroll
         for (int m = g0; m < g0 + kGroup && m < E; ++m) {
             const int64_t ca = oa + J * m, cb = ob + J * m;
-            const T rta = a.rhoT[ca], rtb = a.rhoT[cb];
-            const T cA = cpa[J * m], cB = cpb[J * m];
+            const T rta = PRE ? rtp[m].x : a.rhoT[ca], rtb = PRE ? rtp[m].y : a.rhoT[cb];
+            const T cA = PRE ? cp[m].x : cpa[J * m], cB = PRE ? cp[m].y : cpb[J * m];
             T tauA, etaA, tauB, etaB;
             if (a.tau_eta_from_c) {
                 const T la = log2(cA), lb = log2(cB);
@@ -624,7 +661,7 @@ __device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>&
     S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, e);
 }
 template <typename T, int J, int E>
-__global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_EOS_WAVES)))
+__global__ void __launch_bounds__(256, (eos_waves<T, E>()))
     reg_z_eos(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
     reg_z_eos_body<T, J, E>(bidx_here(), a, g, P, tab);
 }
@@ -825,19 +862,25 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
                 q += step;
             }
         }
-        S::template transform<false>(X, lds, j, w, lt);
         const T* ktab = nullptr;
         for (int o = 0; o < a.nop; ++o)
             if (a.op[o].in == f && a.op[o].table) { ktab = reinterpret_cast<const T*>(a.op[o].table); break; }
-        if (ktab && ktab != kap_src) {
-            const T* q = ktab + hbase;
+        // the operator table (kappa or |k|^p) is loaded with the input when KWH_MIDDLE_KAPPA_EARLY
+        // is set, else after the forward transform
+        auto load_table = [&] {
+            if (ktab && ktab != kap_src) {
+                const T* q = ktab + hbase;
 #pragma unroll
-            for (int m = 0; m < E; ++m) {
-                kap[m] = *q * scale;
-                q += hstep;
+                for (int m = 0; m < E; ++m) {
+                    kap[m] = *q * scale;
+                    q += hstep;
+                }
+                kap_src = ktab;
             }
-            kap_src = ktab;
-        }
+        };
+        if (KWH_MIDDLE_KAPPA_EARLY) load_table();
+        S::template transform<false>(X, lds, j, w, lt);
+        if (!KWH_MIDDLE_KAPPA_EARLY) load_table();
         for (int o = 0; o < a.nop; ++o) {
             const MidOp op = a.op[o];
             if (op.in != f) continue;

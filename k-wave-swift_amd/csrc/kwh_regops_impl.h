@@ -63,7 +63,7 @@ struct RegImpl {
     }
     static void z_eos(const Emit& em, dim3 gr, dim3 bl, size_t lds, const EosArgs<T>& ea, const Geo& g, int P,
                       const AxisTabs<T>& tab) {
-        em.go<Body<&reg_z_eos_body<T, J, E>, &reg_z_eos<T, J, E>, reg_waves<T, E>(KWH_EOS_WAVES), false>>(
+        em.go<Body<&reg_z_eos_body<T, J, E>, &reg_z_eos<T, J, E>, eos_waves<T, E>(), false>>(
             gr, bl, lds, ea, g, P, tab);
     }
     static void adj_fwd(const Emit& em, dim3 gr, dim3 bl, size_t lds, const AdjRowArgs<T>& a, const Geo& g, int P,

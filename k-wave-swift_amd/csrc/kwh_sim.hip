@@ -239,6 +239,11 @@ struct SimT final : kwh_sim {
     DBuf p, u[3], rho[3], rhoT, ucol[3], p0s;
     DBuf spec[6];          // spectra work buffers
     DBuf scr_real, scr_spec, corr_u[3], corr_p;
+    // k-space-corrected sources: the middle-axis slabs [slab_lo, slab_lo + slab_n) that hold every
+    // source point (slab_n = 0: not used). The correction's forward passes transform only these
+    // slabs, into src_spec, whose other slabs stay zero from setup on.
+    DBuf src_spec;
+    int slab_lo = 0, slab_n = 0;
     bool has_p0 = false;
     bool P_valid = false;  // spec[0] holds R2C_z(p) for the next step
 
@@ -1093,6 +1098,24 @@ struct SimT final : kwh_sim {
             scr_real.alloc(g.N * sizeof(T));
             scr_spec.alloc(g.NH * sizeof(C));
         }
+        if ((any_add || any_u_add) && !axisym) {
+            const int mid_len = mid_axis == 0 ? g.m0 : g.m1;
+            const int64_t plane = g.N / mid_len;   // cells per middle-axis index (C-order outermost)
+            int64_t lo = mid_len, hi = -1;
+            auto span = [&](const kwh_source_field& sf) {
+                lo = std::min(lo, sf.index[0] / plane);
+                hi = std::max(hi, sf.index[sf.n_points - 1] / plane);   // indices are ascending
+            };
+            if (any_add) span(d.p_source);
+            for (int a = 0; a < dim; ++a)
+                if (usrc[a].present && usrc[a].mode == KWH_SRC_ADDITIVE) span(d.u_source[a]);
+            if (mid_len > 1 && hi >= lo && 2 * (hi - lo + 1) <= mid_len) {
+                slab_lo = (int)lo;
+                slab_n = (int)(hi - lo + 1);
+                src_spec.alloc(g.NH * sizeof(C));
+                fill<C>(src_spec.as<C>(), C{0, 0}, g.NH);
+            }
+        }
         if (any_add) corr_p.alloc(g.N * sizeof(T));
         for (int a = 0; a < dim; ++a)
             if (usrc[a].present && usrc[a].mode == KWH_SRC_ADDITIVE) corr_u[a].alloc(g.N * sizeof(T));
@@ -1278,16 +1301,47 @@ struct SimT final : kwh_sim {
     }
 
     // ------------------------------------------------------------------ the time step
+    // Geometry restricted to n slabs of the middle axis (the same rows and columns as the full grid
+    // inside them): the contiguous- and strided-axis passes of those slabs, on pointers offset to
+    // the first slab.
+    struct SlabScope {
+        SimT* s;
+        Geo saved;
+        SlabScope(SimT* owner, int n) : s(owner), saved(owner->g) {
+            Geo& g = s->g;
+            if (s->mid_axis == 0) g.m0 = n;
+            else g.m1 = n;
+            g.nrows = g.m0 * g.m1;
+            g.N = (int64_t)g.nrows * g.m2;
+            g.NH = (int64_t)g.nrows * g.mzh;
+        }
+        ~SlabScope() { s->g = saved; }
+        SlabScope(const SlabScope&) = delete;
+        SlabScope& operator=(const SlabScope&) = delete;
+    };
     void compute_correction(SourceDev<T>& s, int t, T* dst, double factor) {
         // Re ifft(source_kappa * fft(scatter(v))) (SourceOp.apply :167-171)
-        fill<T>(scr_real.as<T>(), T(0), g.N);
+        const int64_t plane = slab_n ? g.N / (mid_axis == 0 ? g.m0 : g.m1) : 0;
+        T* sr = scr_real.as<T>() + (int64_t)slab_lo * plane;
+        fill<T>(sr, T(0), slab_n ? (int64_t)slab_n * plane : g.N);
         timed(K_SC, 0, [&] {
             emit<ScatterB<T>>(dim3(grid_blocks(s.n)), dim3(256), 0, scr_real.as<T>(),
                                s.index.template as<int64_t>(), s.signal.template as<T>(), s.base.template as<T>(), s.rows, s.len, (int)s.n,
                                (const int*)ctr.p, T(factor));
         });
-        forward_to_middle(scr_real.as<T>(), scr_spec.as<C>(), T(1));
-        middle(K_MT, {scr_spec.p}, {MidOp{OP_TAB, 0, 0, scr_spec.p, skappa.p}});
+        if (slab_n) {
+            // only the source slabs are transformed along the contiguous and strided axes; the
+            // middle pass reads all of src_spec (zero elsewhere) and writes the dense scr_spec
+            C* ss = src_spec.as<C>() + (int64_t)slab_lo * (plane / g.m2) * g.mzh;
+            {
+                SlabScope sc(this, slab_n);
+                forward_to_middle(sr, ss, T(1));
+            }
+            middle(K_MT, {src_spec.p}, {MidOp{OP_TAB, 0, 0, scr_spec.p, skappa.p}});
+        } else {
+            forward_to_middle(scr_real.as<T>(), scr_spec.as<C>(), T(1));
+            middle(K_MT, {scr_spec.p}, {MidOp{OP_TAB, 0, 0, scr_spec.p, skappa.p}});
+        }
         inverse_from_middle(scr_spec.as<C>(), dst, T(1), false);
     }
 

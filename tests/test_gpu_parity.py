@@ -351,6 +351,56 @@ def test_3d_velocity_source_and_nonlinear():
           KWaveSensor(mask=sens), ["p", "pFinal"], pmlSize=6, smoothP0=False)
 
 
+def test_3d_corrected_sources_in_few_slabs():
+    """k-space-corrected additive p and ux sources in different x planes (12 and 20) of a 64x32x32
+    grid: the correction transforms only the x slabs 12..20 along y and z; against the oracle."""
+    nx, n = 64, 32
+    grid = KWaveGrid(nx, 1e-4, n, 1e-4, n, 1e-4)
+    grid.setTime(60, 2e-8)   # 0.3 cells per step: both wavefronts pass the sensor plane
+    pm = np.zeros((nx, n, n), np.float32)
+    pm[12, 14:18, 14:18] = 1
+    um = np.zeros((nx, n, n), np.float32)
+    um[20, 10:22, 10:22] = 1
+    sig = toneBurst(1 / grid.dt, 2e6, 2).astype(np.float32)
+    src = KWaveSource(pMask=pm, p=(1e5 * sig).astype(np.float32), pMode=SourceMode.additive,
+                      uMask=um, ux=(0.1 * sig).astype(np.float32), uMode=SourceMode.additive)
+    sens = np.zeros((nx, n, n), np.float32)
+    sens[26, :, 16] = 1
+    check("3d_slab_sources", grid, KWaveMedium(1500.0, 1000.0), src, KWaveSensor(mask=sens), ["p", "pFinal"],
+          pmlSize=8, smoothP0=False)
+
+
+def test_corrected_source_slab_path_matches_full_path():
+    """The same additive ux source once within one x row (the correction transforms only that row)
+    and once with one more point, driven by a zero signal, 40 rows away (the source then spans more
+    than half the x axis and the correction transforms the whole grid). In float64 both agree with
+    each other and with the oracle to rounding. They are not bitwise equal: the full pass also
+    stores the spectrum of the source row's partner in the two-rows-per-transform packing, whose
+    exact value 0 comes out as rounding residue."""
+    n = 64
+    grid = KWaveGrid(n, 1e-4, n, 1e-4)
+    grid.setTime(50, 2e-8)
+    base = toneBurst(1 / grid.dt, 1e6, 2).astype(np.float32)
+    near = np.zeros((n, n), np.float32)
+    near[20, 28:36] = 1
+    rows = np.stack([np.roll(np.pad(base, (0, 8)), d) for d in range(8)]).astype(np.float32)
+    far = near.copy()
+    far[60, 5] = 1
+    rows_far = np.concatenate([rows, np.zeros((1, rows.shape[1]), np.float32)])
+    sens = KWaveSensor(mask=np.ones((n, n), np.float32), record=RecordField.p | RecordField.pFinal)
+    med = KWaveMedium(1500.0, 1000.0)
+    src_near = KWaveSource(uMask=near, ux=rows, uMode=SourceMode.additive)
+    print("\n[case] corrected source: slab path vs full path, 2D 64^2 float64", flush=True)
+    a = kspaceFirstOrder(grid, med, src_near, sens, SimulationOptions(pmlSize=8, smoothP0=False, dtype=F64))
+    b = kspaceFirstOrder(grid, med, KWaveSource(uMask=far, ux=rows_far, uMode=SourceMode.additive), sens,
+                         SimulationOptions(pmlSize=8, smoothP0=False, dtype=F64))
+    ref = O.kspace_first_order(grid, med, src_near, sens, SimulationOptions(pmlSize=8, smoothP0=False))
+    for name, x, y in (("slab vs full", a.p, b.p), ("slab vs oracle", a.p, ref["p"]), ("full vs oracle", b.p, ref["p"])):
+        rl2, rmax = metrics(x, y)
+        print(f"  {name}: relL2={rl2:.3e} relMax={rmax:.3e}", flush=True)
+        assert rl2 <= 1e-12 and rmax <= 1e-11, (name, rl2, rmax)
+
+
 def test_lossless_closed_form_3d():
     """pmlAlpha = 0, homogeneous: the device loop against IFFT(P0 cos(c|k|t)) directly."""
     n, dx, c = 32, 1e-4, 1500.0
