@@ -324,7 +324,10 @@ __global__ void __launch_bounds__(256) reg_z_colloc_rows(const cx<T>* spec, T* _
 // earlier ones (bounds live registers; other waves hide the latency).
 #define KWH_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
 constexpr int kGroup = 8;      // density update
-constexpr int kVelGroup = 4;   // velocity update (keeps it within the three-wave register budget)
+#ifndef KWH_VEL_GROUP
+#define KWH_VEL_GROUP 4
+#endif
+constexpr int kVelGroup = KWH_VEL_GROUP;   // velocity update (keeps it within the four-wave register budget)
 
 
 // Velocity update, per axis: C2R_z(G_a) -> u_a = pml_sg (pml_sg u_a - dt/rho_a g) [+ sources] -> R2C_z.
