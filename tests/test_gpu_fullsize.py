@@ -1,0 +1,63 @@
+"""Full-size runs checked through a size-independent property: for a homogeneous, lossless,
+PML-free medium the k-space corrected loop is exact, so the final pressure's spectrum is
+P0(k) cos(c |k| (nt-1) dt) at every wavevector (KSpaceFirstOrder.swift:557-761 with the t = 0
+override :724-733). The spectrum is sampled at a few wavevectors by separable sums over the
+three axes, so no full-size FFT runs on the host.
+
+256^3 runs the register-resident kernels of the benchmark; 1024^3 (1.07 G cells, int64 offsets,
+the generic Stockham kernels at length 1024 on every axis) is the largest cube that fits the
+device with this physics."""
+import numpy as np
+import pytest
+
+from oracle import kwave_oracle as O
+from kwavehip import KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, SimulationOptions, kspaceFirstOrder
+
+from test_gpu_parity import _device  # noqa: F401  (module fixture: the library must load, a device must exist)
+
+pytestmark = pytest.mark.gpu
+
+
+def _ball(n, r):
+    i = np.arange(n, dtype=np.int64) - n // 2
+    d2 = (i * i)[:, None] + (i * i)[None, :]
+    out = np.zeros((n, n, n), np.float32)
+    for x in range(n):
+        out[x] = ((d2 + i[x] * i[x]) <= r * r).astype(np.float32)
+    return out
+
+
+def _spectra(p, idxs):
+    """P(k) = sum_x p[x] exp(-2 pi i k.x / n) at the integer wavevectors idxs [(kx, ky, kz), ...],
+    one x plane at a time (float32 products on the plane, float64 accumulation over x)."""
+    n = p.shape
+    e = [np.exp(-2j * np.pi * np.outer(np.arange(n[a]), [k[a] for k in idxs]) / n[a]) for a in range(3)]
+    ez_re, ez_im = e[2].real.astype(np.float32), e[2].imag.astype(np.float32)
+    acc = np.zeros(len(idxs), np.complex128)
+    for x in range(n[0]):
+        pz = (p[x] @ ez_re).astype(np.float64) + 1j * (p[x] @ ez_im).astype(np.float64)   # [ny, nk]
+        acc += e[0][x] * np.einsum("yk,yk->k", pz, e[1])
+    return acc
+
+
+@pytest.mark.parametrize("n,nt,r", [(256, 40, 12), (1024, 6, 40)])
+def test_fullsize_lossless_closed_form(n, nt, r):
+    dx, c = 1e-4, 1500.0
+    grid = KWaveGrid(n, dx, n, dx, n, dx)
+    grid.setTime(nt, 0.3 * dx / c)
+    p0 = _ball(n, r)
+    print(f"\n[case] full-size closed form {n}^3 FP32, nt={nt}", flush=True)
+    out = kspaceFirstOrder(grid, KWaveMedium(c, 1000.0), KWaveSource(p0=p0), KWaveSensor(),
+                           SimulationOptions(pmlAlpha=0.0, smoothP0=False))
+    pf = out.pFinal
+    assert pf.shape == (n, n, n) and np.all(np.isfinite(pf))
+    kv = O.fft_freq(n, dx) * 2.0 * np.pi
+    scale = float(p0.sum())   # |P0(0)|, the largest spectral magnitude of a non-negative p0
+    idxs = [(0, 0, 0), (1, 0, 0), (0, 3, 5), (7, -4, 2), (n // 8, n // 16, -n // 32), (n // 4, 5, n // 3)]
+    kmag = np.array([np.sqrt(sum(kv[i % n] ** 2 for i in idx)) for idx in idxs])
+    want = _spectra(p0, idxs) * np.cos(c * kmag * (nt - 1) * grid.dt)
+    got = _spectra(pf, idxs)
+    err = np.abs(got - want) / scale
+    for idx, w, e in zip(idxs, want, err):
+        print(f"  k={idx}: |P|={abs(w):.4e} err/|P0(0)|={e:.2e}", flush=True)
+    assert err.max() <= 1e-5, err.max()
