@@ -317,6 +317,17 @@ template <typename T>
 __device__ void z_forward(cx<T>* spec, const Geo& g, const ZTile& zt, const FftPlan& pl, const cx<T>* tw, cx<T>* src,
                           cx<T>* other, int tid, int nthr) {
     __syncthreads();
+    // Odd row count: the last pair has no b row. Its slot holds what the kernel left there (the
+    // rounding residue of an inverse, at the scale of the field before the update), and the
+    // packed transform's rounding is relative to max(|a|, |b|): clear it so that row a keeps its
+    // own precision. The condition is uniform over the workgroup.
+    if (g.nrows & 1) {
+        const int lastq = (g.nrows - 1) / 2;
+        if (lastq >= zt.q0 && lastq < zt.q0 + zt.ntr) {
+            for (int n = tid; n < g.m2; n += nthr) src[n * zt.pitch + (lastq - zt.q0)].y = T(0);
+            __syncthreads();
+        }
+    }
     cx<T>* res = src;
     if (pl.nstages > 0) {
         // ping-pong between src and other; result lands in the buffer the last stage writes
@@ -412,13 +423,7 @@ __device__ __forceinline__ void z_forward_real_body(const BIdx& bi, const T* __r
     cx<T>* b0 = reinterpret_cast<cx<T>*>(smem);
     cx<T>* b1 = b0 + (size_t)g.m2 * zt.pitch;
     KWH_CELL_LOOP(zt, g, tid, nthr, { comp(b0[n * zt.pitch + w], part) = src[cell] * scale; })
-    // odd row count: the missing b row must be zero
-    if (g.nrows & 1) {
-        const int lastq = (g.nrows - 1) / 2;
-        if (lastq >= zt.q0 && lastq < zt.q0 + zt.ntr)
-            for (int n = tid; n < g.m2; n += nthr) b0[n * zt.pitch + (lastq - zt.q0)].y = T(0);
-    }
-    z_forward<T>(spec, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);
+    z_forward<T>(spec, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);   // clears the missing b row
 }
 template <typename T>
 __global__ void __launch_bounds__(256) z_forward_real(const T* __restrict__ src, cx<T>* spec, T scale, Geo g,
@@ -751,13 +756,7 @@ __device__ __forceinline__ void z_init_body(const BIdx& bi, const InitArgs<T>& a
         a.p[cell] = v;
         comp(b0[n * zt.pitch + w], part) = v;
     })
-    if (g.nrows & 1) {
-        __syncthreads();
-        const int lastq = (g.nrows - 1) / 2;
-        if (lastq >= zt.q0 && lastq < zt.q0 + zt.ntr)
-            for (int n = tid; n < g.m2; n += nthr) b0[n * zt.pitch + (lastq - zt.q0)].y = T(0);
-    }
-    z_forward<T>(a.outP, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);
+    z_forward<T>(a.outP, g, zt, tab.plan[2], tab.tw[2], b0, b1, tid, nthr);   // clears the missing b row
 }
 template <typename T>
 __global__ void __launch_bounds__(256) z_init(InitArgs<T> a, Geo g, ZTile zt0, AxisTabs<T> tab) {

@@ -2356,6 +2356,28 @@ struct SimT final : kwh_sim {
         auto series = [&] { *ndim = 2; shape[0] = nsens; shape[1] = rec_steps; };
         auto agg = [&] { *ndim = 1; shape[0] = nsens; };
         const int uc = (field >= KWH_OUT_UX && field <= KWH_OUT_IZ_AVG) ? ((field - KWH_OUT_UX) % 3) : 0;
+        if (nsens == 0 && field != KWH_OUT_P_FINAL && !(field >= KWH_OUT_UX_FINAL && field <= KWH_OUT_UZ_FINAL)) {
+            // a sensor mask without points: every recorded sensor field is empty ([0, nt] series,
+            // [0] aggregates), as the reference's gather over no indices
+            uint32_t bit = 0;
+            bool ser = false;
+            switch (field) {
+                case KWH_OUT_P: bit = KWH_REC_P; ser = true; break;
+                case KWH_OUT_P_MAX: bit = KWH_REC_P_MAX; break;
+                case KWH_OUT_P_MIN: bit = KWH_REC_P_MIN; break;
+                case KWH_OUT_P_RMS: bit = KWH_REC_P_RMS; break;
+                case KWH_OUT_UX: case KWH_OUT_UY: case KWH_OUT_UZ: bit = (uint32_t)KWH_REC_UX << uc; ser = true; break;
+                case KWH_OUT_UX_MAX: case KWH_OUT_UY_MAX: case KWH_OUT_UZ_MAX: bit = KWH_REC_U_MAX; break;
+                case KWH_OUT_UX_RMS: case KWH_OUT_UY_RMS: case KWH_OUT_UZ_RMS: bit = KWH_REC_U_RMS; break;
+                case KWH_OUT_IX_AVG: case KWH_OUT_IY_AVG: case KWH_OUT_IZ_AVG: bit = KWH_REC_I_AVG; break;
+                default: throw KwhError(KWH_EINVAL, "unknown output field");
+            }
+            KWH_REQUIRE((record & bit) && uc < dim, KWH_EINVAL, "field not recorded");
+            *ndim = ser ? 2 : 1;
+            shape[0] = 0;
+            if (ser) shape[1] = steps;   // no step is recorded without points; the series has nt columns
+            return;
+        }
         switch (field) {
             case KWH_OUT_P_FINAL: grid(); return;
             case KWH_OUT_P: KWH_REQUIRE(store_p && (record & KWH_REC_P), KWH_EINVAL, "p not recorded"); series(); return;
@@ -2389,6 +2411,7 @@ struct SimT final : kwh_sim {
         int64_t count = 1;
         for (int i = 0; i < nd; ++i) count *= shape[i];
         KWH_REQUIRE(bytes >= (size_t)count * sizeof(T), KWH_EINVAL, "destination buffer too small");
+        if (count == 0) return;   // empty sensor fields
         HIPCHK(hipSetDevice(dev));
         sync();
         auto d2h = [&](const void* src, size_t b) {
