@@ -190,7 +190,9 @@ int kwh_sim_launch(kwh_sim* sim, int32_t t_begin, int32_t t_end);
 int kwh_sim_sync(kwh_sim* sim);
 
 /* Shape of an output after the run (finalizeRecording, Recording.swift:129-168).
- * Returns KWH_EINVAL if the field was not recorded. */
+ * Returns KWH_EINVAL if the field was not recorded. With a sensor mask that has no points, the
+ * recorded sensor fields are empty: series [0, steps run], aggregates [0] (read them with any
+ * buffer; nothing is copied). */
 int kwh_sim_output_shape(kwh_sim* sim, int32_t field, int64_t shape[3], int32_t* ndim);
 
 /* Copy an output (kwh_output) into dst (element type = sim dtype). */
