@@ -19,18 +19,24 @@ Reference files (paths relative to the reference checkout):
   Sources/KWave/Sensor/Directivity.swift        2D directivity filter
   Sources/KWave/Material/Conversion.swift       db2neper
 
-How each part is pinned (tests/test_oracle.py):
+How each part is pinned (tests/test_oracle.py, tests/test_oracle_physics_pins.py):
   fft_freq / wavenumbers     numpy.fft.fftfreq, GridTests.swift:8-16
   make_time                  GridTests.swift:18-24, ParityTests.swift:63 (nt == 302)
   derivative/colloc ops      plane-wave identity (exact spectral derivative of sin)
   whole loop (lossless)      closed form p^(k,t) = p^0(k) cos(c|k|t) with pml_alpha = 0
   pml_profile                closed form exp(-a dt/2 x^4) at the documented positions
-  smoothing                  window identities (DC gain 1, max restore), even-size symmetry
+  smoothing                  numpy.blackman, exact integer radii (3-4-5, 2-3-6), numpy.interp on a
+                             stretched axis, two-mode amplitude ratio; DC gain 1, max restore
+  heterogeneous coupling     impedance step: reflection (Z2-Z1)/(Z2+Z1), transmission 2Z2/(Z2+Z1)
+                             for c and rho contrasts, along each axis; transmitted speed c2
   absorption / dispersion    mode-wise damping rate vs alpha0 * omega^y * c; alpha -> 0 limit
-  B/A                        B/A = 0 limit; quadratic term checked on a 1-step closed form
+  B/A                        Fubini solution of a progressive wave (second harmonic growth with
+                             beta = 1 + B/2A, B/A -> 0, 2, 6); quadratic scaling; linear limit
   sources                    dirichlet sets values exactly; additive-no-correction linearity
   recording                  pMax >= pRms >= 0, iAvg of an in-phase plane wave
   directivity                isotropic limit (size -> 0) equals plain sampling
+Planted mistakes (a Hann window, an n for n-1, c for c^2, dt*rho for dt/rho, a dropped or halved
+nonlinear factor) are monkeypatched into the module-level functions below and shown to fail.
 """
 from __future__ import annotations
 
@@ -160,6 +166,29 @@ def stagger_density(rho0, axis):
     m = np.moveaxis(rho0, axis, 0)
     st = np.concatenate([0.5 * (m[:-1] + m[1:]), m[-1:]], axis=0)
     return np.moveaxis(st, 0, axis)
+
+
+def dt_over_staggered_density(dt, rho0, axis):
+    """dt / rho0 on the +d/2 grid of `axis`, the velocity update's coefficient
+    (KSpaceFirstOrder.swift:612-614; 2D :401-402, 1D :245)."""
+    return dt / stagger_density(rho0, axis)
+
+
+def eos_sound_speed_squared(c0):
+    """c0^2, the linear equation-of-state coefficient p = c0^2 rho (KSpaceFirstOrder.swift:610; 2D :399, 1D :243)."""
+    return c0 * c0
+
+
+def nonlinear_mass_factor(rho_total_prev, rho0):
+    """(2 sum(rho) + rho0) / rho0 scaling the mass-conservation term, from the previous step's
+    split densities (Nonlinearity.factor, KSpaceFirstOrder.swift:894)."""
+    return (2.0 * rho_total_prev + rho0) / rho0
+
+
+def nonlinear_eos_term(bona, rho_total, rho0):
+    """B/A * rho^2 / (2 rho0), the quadratic equation-of-state term (Nonlinearity.term,
+    KSpaceFirstOrder.swift:897)."""
+    return bona * (rho_total * rho_total) / (2.0 * rho0)
 
 
 def db2neper(alpha, y=1.0):
@@ -510,9 +539,9 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
     pml_sg = [_bcast(pml_profile(sizes[a], spac[a], dt, c_ref, pml_n, options.pmlAlpha, True), a, dim)
               for a in range(dim)]
 
-    c2 = c0 * c0
+    c2 = eos_sound_speed_squared(c0)
     dt_rho0 = dt * rho0
-    dt_over_rho = [dt / stagger_density(rho0, a) for a in range(dim)]
+    dt_over_rho = [dt_over_staggered_density(dt, rho0, a) for a in range(dim)]
 
     n = float(dim)
     p_mode = _mode(source.pMode, "additive")
@@ -624,7 +653,7 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
             if u_ops[a] is not None:
                 u[a] = cast(u_ops[a].apply(u[a], t, source_kappa, fft))
         rho_prev = sum(rho)                                               # rhoStep
-        nl = (2.0 * rho_prev + rho0) / rho0 if bona is not None else None
+        nl = nonlinear_mass_factor(rho_prev, rho0) if bona is not None else None
         du = []
         for a in range(dim):
             d = cast(np.real(fft.ifftn(ddx_neg[a] * kappa * fft.fftn(u[a]))))
@@ -639,7 +668,7 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
         if absorb is not None:
             eos = eos + eos_term(sum(du), rho_t)
         if bona is not None:
-            eos = eos + bona * (rho_t * rho_t) / (2.0 * rho0)
+            eos = eos + nonlinear_eos_term(bona, rho_t, rho0)
         p = cast(c2 * eos)
         if t == 0 and p0f is not None:                                    # :724-733
             p = cast(p0f.copy())
@@ -771,10 +800,10 @@ def kspace_first_order_as(grid, medium, source, sensor, options, workers=1):
     pml_y_sg = pml_profile(ny, dy, dt, c_ref, pml_n, options.pmlAlpha, True, True).reshape(1, ny)
     inv_y_sg = (1.0 / ((np.arange(ny) + 0.5) * dy)).reshape(1, ny)      # (:84-86)
 
-    c2 = c0 * c0
+    c2 = eos_sound_speed_squared(c0)
     dt_rho0 = dt * rho0
-    dt_over_rho_x = dt / stagger_density(rho0, 0)
-    dt_over_rho_y = dt / stagger_density(rho0, 1)
+    dt_over_rho_x = dt_over_staggered_density(dt, rho0, 0)
+    dt_over_rho_y = dt_over_staggered_density(dt, rho0, 1)
 
     absorb_tau = None                                                     # (:93-101)
     alpha_mode = _mode(medium.alphaMode, "noAbsorption")
@@ -865,7 +894,7 @@ def kspace_first_order_as(grid, medium, source, sensor, options, workers=1):
         duydy = crop(kappa * y_shift_neg * uy_k)
 
         if bona is not None:                                              # (:253-261)
-            dt_rho_total = dt * (2.0 * (rhox + rhoy) + rho0)
+            dt_rho_total = dt * rho0 * nonlinear_mass_factor(rhox + rhoy, rho0)
             rhox = pml_x * (pml_x * rhox - dt_rho_total * duxdx)
             rhoy = pml_y * (pml_y * rhoy - dt_rho_total * duydy)
         else:
@@ -895,7 +924,7 @@ def kspace_first_order_as(grid, medium, source, sensor, options, workers=1):
         if absorb_tau is not None:
             eos = eos + absorb_tau * rho0 * (duxdx + duydy)
         if bona is not None:
-            eos = eos + bona * (rho_total * rho_total) / (2.0 * rho0)
+            eos = eos + nonlinear_eos_term(bona, rho_total, rho0)
         p = c2 * eos
 
         if t == 0 and p0_field is not None:                               # (:289-297)
