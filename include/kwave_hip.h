@@ -38,7 +38,7 @@ typedef enum {
     KWH_EINVAL = -1,        /* invalid argument / descriptor (reference: precondition traps) */
     KWH_ENOMEM = -2,        /* device allocation failed */
     KWH_EDEVICE = -3,       /* HIP runtime error or no device */
-    KWH_ENONFINITE = -4,    /* non-finite pressure detected by the optional check */
+    KWH_ENONFINITE = -4,    /* non-finite pressure (kwh_sim_set_nonfinite_check) */
     KWH_EUNSUPPORTED = -5,  /* configuration outside what this library implements */
     KWH_ESTATE = -6         /* call out of order (e.g. reading an output before running) */
 } kwh_status;
@@ -213,6 +213,16 @@ int kwh_sim_set_kernel_timing(kwh_sim* sim, int enable);
  * two events per launch of that kernel, none elsewhere. Resets the statistics. */
 int kwh_sim_set_kernel_timing_only(kwh_sim* sim, const char* kernel_name);
 int kwh_sim_kernel_stats(kwh_sim* sim, kwh_kernel_stat* stats, int32_t capacity, int32_t* count);
+
+/* Optional blow-up detection (no counterpart in the reference, whose solver only traps on its
+ * preconditions, KSpaceFirstOrder.swift:564-576): with every > 0, kwh_sim_run scans the pressure
+ * field on the device after every `every`-th step and after the last step of each call, and
+ * returns KWH_ENONFINITE as soon as a scan finds a NaN or an infinity (typically a time step above
+ * the CFL stability limit). Without a callback the run stops at the end of the first chunk of
+ * `every` steps whose field is not finite; with one, after the callback of the flagged step. The
+ * steps run so far stay done (kwh_sim_steps_done) and the outputs stay readable. 0 (the default)
+ * turns the check off; the scan reads the pressure once (N values) per check. */
+int kwh_sim_set_nonfinite_check(kwh_sim* sim, int32_t every);
 
 /* Replay the time step from captured hipGraphs (1 = on): one graph per set of active
  * k-space-corrected sources, captured on first use; results are bitwise those of eager runs. */

@@ -852,6 +852,14 @@ template <typename T> struct GatherSet {
     GatherArgs<T> f[4];
 };
 
+// Running max / min that keep a NaN once one has been seen (the semantics of a max / min over the
+// whole series, as numpy's), so a blow-up shows in pMax / pMin / uMax instead of leaving them at
+// their last finite values.
+template <typename T>
+__device__ __forceinline__ T nan_max(T m, T v) { return (v > m || v != v) ? v : m; }
+template <typename T>
+__device__ __forceinline__ T nan_min(T m, T v) { return (v < m || v != v) ? v : m; }
+
 template <typename T>
 __device__ __forceinline__ void gather_kernel_body(const BIdx& bi, const GatherSet<T>& set) {
     const GatherArgs<T>& a = set.f[bi.y];
@@ -866,8 +874,8 @@ __device__ __forceinline__ void gather_kernel_body(const BIdx& bi, const GatherS
             v = a.field[a.idx[s]];
         }
         if (a.series) a.series[(int64_t)slot * a.n + s] = v;
-        if (a.vmax) a.vmax[s] = first ? v : fmax(a.vmax[s], v);
-        if (a.vmin) a.vmin[s] = first ? v : fmin(a.vmin[s], v);
+        if (a.vmax) a.vmax[s] = first ? v : nan_max(a.vmax[s], v);
+        if (a.vmin) a.vmin[s] = first ? v : nan_min(a.vmin[s], v);
         if (a.sum2) a.sum2[s] = (first ? 0.0 : a.sum2[s]) + (double)v * (double)v;
     }
 }
@@ -886,8 +894,8 @@ __device__ __forceinline__ void gather_pos_kernel_body(const BIdx& bi, const T* 
         const T v = field[idx[s]];
         const int64_t q = pos[s];
         if (series) series[(int64_t)t * nsens + q] = v;
-        if (vmax) vmax[q] = first ? v : fmax(vmax[q], v);
-        if (vmin) vmin[q] = first ? v : fmin(vmin[q], v);
+        if (vmax) vmax[q] = first ? v : nan_max(vmax[q], v);
+        if (vmin) vmin[q] = first ? v : nan_min(vmin[q], v);
         if (sum2) sum2[q] = (first ? 0.0 : sum2[q]) + (double)v * (double)v;
     }
 }
@@ -895,6 +903,16 @@ template <typename T>
 __global__ void gather_pos_kernel(const T* field, const int64_t* idx, const int64_t* pos, int64_t n, int64_t nsens,
                                   T* series, T* vmax, T* vmin, double* sum2, const int* ctr) {
     gather_pos_kernel_body<T>(bidx_here(), field, idx, pos, n, nsens, series, vmax, vmin, sum2, ctr);
+}
+
+// Non-finite check of a field after step `step`: flag[0] = min(flag[0], step) if any value is a
+// NaN or an infinity (flag starts at INT_MAX).
+template <typename T>
+__global__ void nonfinite_kernel(const T* x, int64_t n, int step, int* flag) {
+    bool bad = false;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        bad |= !isfinite(x[i]);
+    if (bad) atomicMin(flag, step);
 }
 
 // pRms / uRms = sqrt(sum2 / nt).
@@ -917,8 +935,8 @@ __global__ void element_traces_kernel(const T* series, int64_t n, int nt, const 
         double acc = 0.0;
         for (int q = rowptr[e]; q < rowptr[e + 1]; ++q) {
             const int k = rows[q];
-            const int ts = t + delay[k];
-            if (ts < nt) acc += (double)series[(int64_t)ts * n + k];
+            const int64_t ts = (int64_t)t + delay[k];
+            if (ts < nt) acc += (double)series[ts * n + k];
         }
         out[i] = T(acc * scale);
     }

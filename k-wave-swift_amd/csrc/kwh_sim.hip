@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -173,6 +174,7 @@ struct kwh_sim {
     virtual void element_traces(int field, const int32_t* element, const int32_t* delay, int nel, double scale,
                                 void* dst, size_t bytes) = 0;
     virtual void set_checkpointing(int every) = 0;
+    virtual void set_nonfinite_check(int every) = 0;
     virtual std::string autotune() = 0;
     virtual void time_reversal_result(double factor, void* dst) = 0;
     virtual void vjp(const void* dldy, int every, void* grad_p0, void* grad_c0) = 0;
@@ -291,6 +293,10 @@ struct SimT final : kwh_sim {
     double stat_bytes[kNames] = {};
     // graph replay of the time step
     bool use_graph = false;
+    // optional non-finite check of p every nonfinite_every steps (0 = off); nf_flag holds the
+    // smallest step index after which p held a NaN or an infinity (INT_MAX: none)
+    int nonfinite_every = 0;
+    DBuf nf_flag;
 
     // one-element device buffers behind scalar SMaps (branch-free at() in the kernels)
     DBuf scal_pool;
@@ -2327,11 +2333,55 @@ struct SimT final : kwh_sim {
         HIPCHK(hipStreamSynchronize(stream));
         if (timing) collect_timing();
     }
+    void set_nonfinite_check(int every) override {
+        KWH_REQUIRE(every >= 0, KWH_EINVAL, "check interval must be >= 0");
+        nonfinite_every = every;
+        if (every > 0 && !nf_flag.p) nf_flag.alloc(sizeof(int));
+    }
+    // Scan p after step t (enqueued; the flag is read by nonfinite_seen after a sync).
+    void nonfinite_scan(int t) {
+        hipLaunchKernelGGL((nonfinite_kernel<T>), dim3(grid_blocks(g.N)), dim3(256), 0, stream, p.as<T>(), g.N, t,
+                           nf_flag.as<int>());
+        HIPCHK(hipGetLastError());
+    }
+    // After a sync: throw KWH_ENONFINITE if a scan flagged a step.
+    void nonfinite_raise() {
+        int first = INT_MAX;
+        HIPCHK(hipMemcpy(&first, nf_flag.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (first != INT_MAX) {
+            char msg[160];
+            std::snprintf(msg, sizeof(msg),
+                          "non-finite pressure after step %d (checked every %d steps); the run stopped after step %d",
+                          first, nonfinite_every, steps - 1);
+            throw KwhError(KWH_ENONFINITE, msg);
+        }
+    }
     void run(int t0, int t1, kwh_step_cb cb, int interval, void* user) override {
         check_range(t0, t1);
+        const int ck = nonfinite_every;
+        if (ck > 0) {
+            HIPCHK(hipSetDevice(dev));
+            hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, stream, nf_flag.as<int>(), INT_MAX, (int64_t)1);
+            HIPCHK(hipGetLastError());
+        }
+        // step t is checked when (t + 1) % ck == 0 and after the last step of the call
+        auto check_due = [&](int t) { return ck > 0 && (((t + 1) % ck) == 0 || t + 1 == t1); };
         if (!cb) {
-            launch(t0, t1);
-            sync();
+            if (ck <= 0) {
+                launch(t0, t1);
+                sync();
+                return;
+            }
+            // chunks of ck steps, each followed by a scan and a read of the flag: the run stops at
+            // the first chunk whose end state is not finite
+            for (int t = t0; t < t1;) {
+                const int te = std::min(t1, (t / ck + 1) * ck);
+                launch(t, te);
+                nonfinite_scan(te - 1);
+                sync();
+                nonfinite_raise();
+                t = te;
+            }
             return;
         }
         const int iv = std::max(interval, 1);
@@ -2343,8 +2393,11 @@ struct SimT final : kwh_sim {
                 host.resize(g.N);
                 HIPCHK(hipMemcpyAsync(host.data(), p.p, g.N * sizeof(T), hipMemcpyDeviceToHost, stream));
             }
+            const bool scan = check_due(t);
+            if (scan) nonfinite_scan(t);
             sync();
             cb(t, nt, due ? host.data() : nullptr, user);
+            if (scan) nonfinite_raise();
         }
     }
 
@@ -2468,17 +2521,42 @@ struct SimT final : kwh_sim {
         for (int a = 0; a < dim; ++a) { add(ser_u[a]); add(umax[a]); add(usum2[a]); }
         return v;
     }
-    size_t state_bytes() override {
-        size_t b = 2 * sizeof(int32_t);
-        for (auto& pr : state_parts()) b += pr.second;
-        return b;
+    // A state blob opens with a header that names the sim it belongs to (precision, dimension,
+    // grid, steps of the run, sensor points, recorded fields, payload size) and the counters.
+    // load_state rejects a blob whose header does not match this sim, or whose counters this sim
+    // could not have reached, before anything is copied to the device.
+    struct StateHeader {
+        uint32_t magic;
+        int32_t elem_bytes, dim, nt;
+        int32_t n[3];
+        uint32_t record;
+        int64_t nsens;
+        uint64_t payload;
+        int32_t steps, rec_steps;
+    };
+    static constexpr uint32_t kStateMagic = 0x4b574853u;   // "KWHS"
+    StateHeader state_header() {
+        StateHeader h{};
+        h.magic = kStateMagic;
+        h.elem_bytes = (int32_t)sizeof(T);
+        h.dim = dim;
+        h.nt = nt;
+        for (int a = 0; a < 3; ++a) h.n[a] = nphys[a];
+        h.record = record;
+        h.nsens = nsens;
+        for (auto& pr : state_parts()) h.payload += pr.second;
+        h.steps = steps;
+        h.rec_steps = rec_steps;
+        return h;
     }
+    size_t state_bytes() override { return sizeof(StateHeader) + state_header().payload; }
     void save_state(void* dst, size_t bytes) override {
+        KWH_REQUIRE(dst != nullptr, KWH_EINVAL, "state buffer is NULL");
         KWH_REQUIRE(bytes >= state_bytes(), KWH_EINVAL, "state buffer too small");
         sync();
         char* o = reinterpret_cast<char*>(dst);
-        int32_t hdr[2] = {steps, rec_steps};
-        std::memcpy(o, hdr, sizeof(hdr));
+        const StateHeader hdr = state_header();
+        std::memcpy(o, &hdr, sizeof(hdr));
         o += sizeof(hdr);
         for (auto& pr : state_parts()) {
             HIPCHK(hipMemcpyAsync(o, pr.first, pr.second, hipMemcpyDeviceToHost, stream));
@@ -2487,19 +2565,28 @@ struct SimT final : kwh_sim {
         HIPCHK(hipStreamSynchronize(stream));
     }
     void load_state(const void* src, size_t bytes) override {
-        KWH_REQUIRE(bytes >= state_bytes(), KWH_EINVAL, "state buffer too small");
+        KWH_REQUIRE(src != nullptr, KWH_EINVAL, "state buffer is NULL");
+        KWH_REQUIRE(bytes >= sizeof(StateHeader), KWH_EINVAL, "state buffer too small");
+        StateHeader hdr;
+        std::memcpy(&hdr, src, sizeof(hdr));
+        const StateHeader mine = state_header();
+        KWH_REQUIRE(hdr.magic == kStateMagic, KWH_EINVAL, "not a kwavehip state blob");
+        KWH_REQUIRE(hdr.elem_bytes == mine.elem_bytes && hdr.dim == mine.dim && hdr.nt == mine.nt &&
+                        hdr.n[0] == mine.n[0] && hdr.n[1] == mine.n[1] && hdr.n[2] == mine.n[2] &&
+                        hdr.record == mine.record && hdr.nsens == mine.nsens && hdr.payload == mine.payload,
+                    KWH_EINVAL, "state blob belongs to a different simulation (precision, grid, nt, sensor or record)");
+        KWH_REQUIRE(bytes >= sizeof(StateHeader) + mine.payload, KWH_EINVAL, "state buffer too small");
+        KWH_REQUIRE(hdr.steps >= 0 && hdr.steps <= nt, KWH_EINVAL, "state step counter out of range");
+        KWH_REQUIRE(hdr.rec_steps == ((rec_p || rec_u) ? hdr.steps : 0), KWH_EINVAL,
+                    "state record counter inconsistent with the step counter");
         sync();
-        const char* o = reinterpret_cast<const char*>(src);
-        int32_t hdr[2];
-        std::memcpy(hdr, o, sizeof(hdr));
-        KWH_REQUIRE(hdr[0] >= 0 && hdr[0] <= nt, KWH_EINVAL, "state step counter out of range");
-        o += sizeof(hdr);
+        const char* o = reinterpret_cast<const char*>(src) + sizeof(hdr);
         for (auto& pr : state_parts()) {
             HIPCHK(hipMemcpyAsync(pr.first, o, pr.second, hipMemcpyHostToDevice, stream));
             o += pr.second;
         }
-        steps = hdr[0];
-        rec_steps = hdr[1];
+        steps = hdr.steps;
+        rec_steps = hdr.rec_steps;
         // P = R2C_z(p) is derived state, rebuilt with the kernel the time step uses (bitwise the
         // spectrum an uninterrupted run carries)
         rebuild_P();
@@ -3041,6 +3128,13 @@ int kwh_sim_set_checkpointing(kwh_sim* sim, int32_t every) {
     return guarded([&] {
         KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
         sim->set_checkpointing(every);
+    });
+}
+
+int kwh_sim_set_nonfinite_check(kwh_sim* sim, int32_t every) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        sim->set_nonfinite_check(every);
     });
 }
 

@@ -10,6 +10,7 @@ from .api import (AbsorptionMode, DeviceKind, DirectivityPattern, DTypePrecision
 from .inputs import (db2neper, gaussian, getWin, makeBall, makeCartCircle, makeCircle, makeDisc, neper2db,
                      random_field, toneBurst)
 from .transducer import DelayMaskMode, KWaveTransducer, KWaveTransducerGeometry
+from ._lib import KWaveHIPError, KWaveNonFiniteError
 
 __all__ = [
     "AbsorptionMode", "DeviceKind", "DirectivityPattern", "DTypePrecision", "KWaveGrid", "KWaveMedium",
@@ -17,4 +18,5 @@ __all__ = [
     "SimulationOutput", "timeReversal", "KWaveTransducer", "KWaveTransducerGeometry", "DelayMaskMode",
     "SourceMode", "flatNonzeroIndices", "kspaceFirstOrder", "kspaceFirstOrderAS", "db2neper", "gaussian", "getWin", "makeBall",
     "makeCartCircle", "makeCircle", "makeDisc", "neper2db", "random_field", "toneBurst",
+    "KWaveHIPError", "KWaveNonFiniteError",
 ]

@@ -29,7 +29,7 @@ EXPORTED = [
     "kwh_sim_set_kernel_timing", "kwh_sim_set_kernel_timing_only", "kwh_sim_kernel_stats", "kwh_sim_set_graph",
     "kwh_sim_element_traces", "kwh_sim_autotune", "kwh_sim_set_checkpointing", "kwh_sim_vjp", "kwh_sim_destroy",
     "kwh_batch_create", "kwh_batch_run", "kwh_batch_sim", "kwh_batch_read_output", "kwh_batch_destroy",
-    "kwh_time_reversal",
+    "kwh_time_reversal", "kwh_sim_set_nonfinite_check",
 ]
 
 
@@ -104,6 +104,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                                            C.c_double, vp, C.c_size_t]
     lib.kwh_sim_autotune.argtypes = [vp, C.c_char_p, C.c_size_t]
     lib.kwh_sim_set_checkpointing.argtypes = [vp, C.c_int32]
+    lib.kwh_sim_set_nonfinite_check.argtypes = [vp, C.c_int32]
     lib.kwh_sim_vjp.argtypes = [vp, vp, C.c_int32, vp, vp]
     lib.kwh_sim_destroy.argtypes = [vp]
     lib.kwh_sim_destroy.restype = None
@@ -126,6 +127,10 @@ class KWaveHIPError(RuntimeError):
         self.code = code
 
 
+class KWaveNonFiniteError(KWaveHIPError, FloatingPointError):
+    """KWH_ENONFINITE: the optional blow-up check found a NaN or an infinity in the pressure."""
+
+
 def check(code: int) -> None:
     """Map a kwh_status to a Python exception (the reference traps on invalid input)."""
     if code == OK:
@@ -137,4 +142,6 @@ def check(code: int) -> None:
         raise NotImplementedError(msg)
     if code == ENOMEM:
         raise MemoryError(msg)
+    if code == ENONFINITE:
+        raise KWaveNonFiniteError(code, msg)
     raise KWaveHIPError(code, msg)

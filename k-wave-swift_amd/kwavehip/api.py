@@ -213,6 +213,9 @@ class SimulationOptions:
     progress: Optional[Callable[[int, int], None]] = None
     fieldMonitor: Optional[Callable[[int, np.ndarray], None]] = None
     fieldMonitorInterval: int = 1
+    # no counterpart in the reference: scan p for NaN / inf every this many steps (0 = off); the
+    # run stops with kwavehip.KWaveNonFiniteError (KWH_ENONFINITE) at the first flagged check
+    nonfiniteCheckEvery: int = 0
 
 
 @dataclass
@@ -367,6 +370,8 @@ class Simulation:
         self.handle = C.c_void_p()
         create = self.lib.kwh_sim_create_axisymmetric if axisymmetric else self.lib.kwh_sim_create
         L.check(create(_context(device), C.byref(d), C.byref(self.handle)))
+        if getattr(options, "nonfiniteCheckEvery", 0):
+            self.set_nonfinite_check(options.nonfiniteCheckEvery)
 
     @classmethod
     def _view(cls, lib, handle, grid, dtype, record, n_sensor):
@@ -459,6 +464,11 @@ class Simulation:
         L.check(self.lib.kwh_sim_kernel_stats(self.handle, arr, cnt.value, C.byref(cnt)))
         return [{"name": arr[i].name.decode(), "launches": arr[i].launches, "total_ms": arr[i].total_ms,
                  "bytes_per_launch": arr[i].bytes_per_launch} for i in range(cnt.value)]
+
+    def set_nonfinite_check(self, every: int):
+        """Scan p for NaN / inf after every `every`-th step of later runs (0 = off); a run that finds
+        one stops and raises KWaveNonFiniteError (KWH_ENONFINITE)."""
+        L.check(self.lib.kwh_sim_set_nonfinite_check(self.handle, int(every)))
 
     def set_graph(self, on: bool):
         """Replay the time step from captured hipGraphs (bitwise equal to eager launches)."""
