@@ -198,6 +198,17 @@ int kwh_sim_output_shape(kwh_sim* sim, int32_t field, int64_t shape[3], int32_t*
 /* Copy an output (kwh_output) into dst (element type = sim dtype). */
 int kwh_sim_read_output(kwh_sim* sim, int32_t field, void* dst, size_t dst_bytes);
 
+/* kwh_sim_read_output without device allocations. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory between set-up and teardown (serving loops that read
+ * results after every run, harnesses that forbid hipMalloc). Same arguments, same results (bit
+ * for bit) as kwh_sim_read_output; the staging that call allocates per read (sensor-major series
+ * rows, pRms / uRms, iAvg) lives instead in a static 8 MiB device array per GPU, resolved once per
+ * device by kwh_ctx_create, and the outputs are produced in chunks of sensors that fit it. Limit:
+ * one sensor's recorded series (steps run x element size) must fit the 8 MiB, else
+ * KWH_EUNSUPPORTED. Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (sims on several host threads) race and must be serialised by the caller. */
+int kwh_sim_read_output_noalloc(kwh_sim* sim, int32_t field, void* dst, size_t dst_bytes);
+
 /* Number of steps completed so far. */
 int kwh_sim_steps_done(kwh_sim* sim, int32_t* steps);
 

@@ -969,7 +969,7 @@ __global__ void scaled_positive_kernel(const T* p, T* out, T factor, int64_t n) 
 // I = (1/nt) sum_t p[t] * Re(ifft(e^{i w/2} fft(u)))[t] = (1/nt^2) Re sum_k conj(P_k) S_k U_k.
 // One thread per sensor point, direct DFT in float64 (cost n * nt^2).
 template <typename T>
-__device__ void intensity_point(const T* p, const T* u, T* out, int64_t n, int nt, int64_t s) {
+__device__ double intensity_point(const T* p, const T* u, int64_t n, int nt, int64_t s) {
     const double two_pi = 6.283185307179586476925286766559;
     double acc = 0.0;
     const int half = nt / 2;
@@ -992,13 +992,32 @@ __device__ void intensity_point(const T* p, const T* u, T* out, int64_t n, int n
         // Re(conj(P) * SU)
         acc += pr * sur + pi * sui;
     }
-    out[s] = T(acc / ((double)nt * (double)nt));
+    return acc / ((double)nt * (double)nt);
 }
 
 template <typename T>
 __global__ void intensity_kernel(const T* p, const T* u, T* out, int64_t n, int nt) {
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x)
-        intensity_point(p, u, out, n, nt, s);
+        out[s] = T(intensity_point(p, u, n, nt, s));
+}
+
+// The same for sensors [s0, s0 + cnt) into out[0, cnt).
+template <typename T>
+__global__ void intensity_range_kernel(const T* p, const T* u, T* out, int64_t n, int nt, int64_t s0, int64_t cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = T(intensity_point(p, u, n, nt, s0 + i));
+}
+
+// Recorded series, time-major [nt][n], to sensor-major rows [cnt][nt] for sensors [s0, s0 + cnt)
+// (the [nSensor, nt] layout of SimulationOutput).
+template <typename T>
+__global__ void series_rows_kernel(const T* series, int64_t n, int nt, int64_t s0, int64_t cnt, T* out) {
+    const int64_t total = cnt * nt;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / nt;
+        const int t = (int)(i - r * nt);
+        out[i] = series[(int64_t)t * n + s0 + r];
+    }
 }
 
 }  // namespace kwh

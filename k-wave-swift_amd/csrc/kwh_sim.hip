@@ -15,6 +15,7 @@
 #include <list>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -141,6 +142,18 @@ int grid_blocks(int64_t n, int threads = 256) {
     return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 65536);
 }
 
+// Device scratch of the allocation-free entry point kwh_sim_read_output_noalloc: output staging
+// (series rows, pRms, iAvg) in chunks of sensors that fit it. One copy per device, held until the
+// process ends; every call on a device shares it and keeps the last call's contents (each call
+// writes every scratch byte it reads back, so nothing needs clearing), and calls that overlap on
+// one device race on it. In host code `scratch` only names the symbol: kernels and copies get
+// the per-device address resolved by resolve_scratch.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
 
 struct kwh_ctx {
@@ -163,6 +176,7 @@ struct kwh_sim {
     virtual void run(int t0, int t1, kwh_step_cb cb, int interval, void* user) = 0;
     virtual void output_shape(int field, int64_t shape[3], int* ndim) = 0;
     virtual void read_output(int field, void* dst, size_t bytes) = 0;
+    virtual void read_output_noalloc(int field, void* dst, size_t bytes, void* scratch_ptr, size_t scratch_bytes) = 0;
     virtual int steps_done() const = 0;
     virtual size_t state_bytes() = 0;
     virtual void save_state(void* dst, size_t bytes) = 0;
@@ -2511,6 +2525,76 @@ struct SimT final : kwh_sim {
         }
     }
 
+    // read_output without device allocations: every staging buffer lives in the caller-provided
+    // device scratch, and the series and per-sensor outputs are produced in chunks of sensors that
+    // fit it. Same kernels and copies as read_output, so the results are identical.
+    void read_output_noalloc(int field, void* dst, size_t bytes, void* scr, size_t scr_bytes) override {
+        int64_t shape[3];
+        int nd = 0;
+        output_shape(field, shape, &nd);
+        int64_t count = 1;
+        for (int i = 0; i < nd; ++i) count *= shape[i];
+        KWH_REQUIRE(bytes >= (size_t)count * sizeof(T), KWH_EINVAL, "destination buffer too small");
+        if (count == 0) return;
+        const bool series = field == KWH_OUT_P || (field >= KWH_OUT_UX && field <= KWH_OUT_UZ);
+        KWH_REQUIRE(!series || (size_t)rec_steps * sizeof(T) <= scr_bytes, KWH_EUNSUPPORTED,
+                    "recorded series longer than the allocation-free scratch holds (8 MiB per sensor row)");
+        HIPCHK(hipSetDevice(dev));
+        sync();
+        char* out = reinterpret_cast<char*>(dst);
+        T* st = reinterpret_cast<T*>(scr);
+        auto d2h = [&](void* to, const void* from, size_t b) {
+            HIPCHK(hipMemcpyAsync(to, from, b, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        };
+        // sensors per chunk when each needs `per` elements of staging
+        auto chunks = [&](int64_t per, auto&& body) {
+            const int64_t cap = std::max<int64_t>((int64_t)(scr_bytes / (sizeof(T) * (size_t)per)), 1);
+            for (int64_t s0 = 0; s0 < nsens; s0 += cap) {
+                const int64_t cnt = std::min(cap, nsens - s0);
+                body(s0, cnt);
+                HIPCHK(hipGetLastError());
+                d2h(out + (size_t)s0 * per * sizeof(T), st, (size_t)cnt * per * sizeof(T));
+            }
+        };
+        auto series_rows = [&](const DBuf& ser) {
+            chunks(rec_steps, [&](int64_t s0, int64_t cnt) {
+                hipLaunchKernelGGL((series_rows_kernel<T>), dim3(grid_blocks(cnt * rec_steps)), dim3(256), 0, stream,
+                                   ser.as<T>(), nsens, rec_steps, s0, cnt, st);
+            });
+        };
+        auto rms = [&](const DBuf& s2) {
+            chunks(1, [&](int64_t s0, int64_t cnt) {
+                hipLaunchKernelGGL((rms_kernel<T>), dim3(grid_blocks(cnt)), dim3(256), 0, stream, s2.as<double>() + s0, st,
+                                   cnt, 1.0 / std::max(rec_steps, 1));
+            });
+        };
+        switch (field) {
+            case KWH_OUT_P: series_rows(ser_p); return;
+            case KWH_OUT_UX: case KWH_OUT_UY: case KWH_OUT_UZ: series_rows(ser_u[field - KWH_OUT_UX]); return;
+            case KWH_OUT_P_RMS: rms(psum2); return;
+            case KWH_OUT_UX_RMS: case KWH_OUT_UY_RMS: case KWH_OUT_UZ_RMS: rms(usum2[field - KWH_OUT_UX_RMS]); return;
+            case KWH_OUT_IX_AVG: case KWH_OUT_IY_AVG: case KWH_OUT_IZ_AVG: {
+                const int c = field - KWH_OUT_IX_AVG;
+                chunks(1, [&](int64_t s0, int64_t cnt) {
+                    hipLaunchKernelGGL((intensity_range_kernel<T>), dim3(grid_blocks(cnt, 64)), dim3(64), 0, stream,
+                                       ser_p.as<T>(), ser_u[c].as<T>(), st, nsens, rec_steps, s0, cnt);
+                });
+                return;
+            }
+            case KWH_OUT_P_FINAL: d2h(dst, p.p, g.N * sizeof(T)); return;
+            case KWH_OUT_P_MAX: d2h(dst, pmax.p, nsens * sizeof(T)); return;
+            case KWH_OUT_P_MIN: d2h(dst, pmin.p, nsens * sizeof(T)); return;
+            case KWH_OUT_UX_MAX: case KWH_OUT_UY_MAX: case KWH_OUT_UZ_MAX:
+                d2h(dst, umax[field - KWH_OUT_UX_MAX].p, nsens * sizeof(T));
+                return;
+            case KWH_OUT_UX_FINAL: case KWH_OUT_UY_FINAL: case KWH_OUT_UZ_FINAL:
+                d2h(dst, u[field - KWH_OUT_UX_FINAL].p, g.N * sizeof(T));
+                return;
+            default: throw KwhError(KWH_EINVAL, "unknown output field");
+        }
+    }
+
     // State = p, u, rho, sensor accumulators and series, counters.
     std::vector<std::pair<void*, size_t>> state_parts() {
         std::vector<std::pair<void*, size_t>> v;
@@ -2926,6 +3010,18 @@ int kwh_abi_version(void) { return KWH_ABI_VERSION; }
 
 const char* kwh_last_error(void) { return g_last_error.c_str(); }
 
+// Once per device, with the device current: the address of that device's copy of `scratch`.
+static void resolve_scratch(int device) {
+    KWH_REQUIRE(device < kMaxDevices, KWH_EUNSUPPORTED, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    KWH_REQUIRE(e == hipSuccess && ptr != nullptr, KWH_EDEVICE,
+                std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+}
+
 int kwh_ctx_create(int device_ordinal, kwh_ctx** out) {
     return guarded([&] {
         KWH_REQUIRE(out != nullptr, KWH_EINVAL, "out is NULL");
@@ -2934,6 +3030,7 @@ int kwh_ctx_create(int device_ordinal, kwh_ctx** out) {
         KWH_REQUIRE(e == hipSuccess && n > 0, KWH_EDEVICE, "no HIP device available");
         KWH_REQUIRE(device_ordinal >= 0 && device_ordinal < n, KWH_EINVAL, "device ordinal out of range");
         HIPCHK(hipSetDevice(device_ordinal));
+        resolve_scratch(device_ordinal);
         *out = new kwh_ctx{device_ordinal};
     });
 }
@@ -3014,6 +3111,16 @@ int kwh_sim_read_output(kwh_sim* sim, int32_t field, void* dst, size_t dst_bytes
     return guarded([&] {
         KWH_REQUIRE(sim && dst, KWH_EINVAL, "NULL argument");
         sim->read_output(field, dst, dst_bytes);
+    });
+}
+
+int kwh_sim_read_output_noalloc(kwh_sim* sim, int32_t field, void* dst, size_t dst_bytes) {
+    return guarded([&] {
+        KWH_REQUIRE(sim && dst, KWH_EINVAL, "NULL argument");
+        const int device = sim->device();
+        void* ptr = (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+        KWH_REQUIRE(ptr != nullptr, KWH_ESTATE, "scratch address of this device not resolved");
+        sim->read_output_noalloc(field, dst, dst_bytes, ptr, kScratchBytes);
     });
 }
 

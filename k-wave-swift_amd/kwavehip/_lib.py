@@ -29,7 +29,7 @@ EXPORTED = [
     "kwh_sim_set_kernel_timing", "kwh_sim_set_kernel_timing_only", "kwh_sim_kernel_stats", "kwh_sim_set_graph",
     "kwh_sim_element_traces", "kwh_sim_autotune", "kwh_sim_set_checkpointing", "kwh_sim_vjp", "kwh_sim_destroy",
     "kwh_batch_create", "kwh_batch_run", "kwh_batch_sim", "kwh_batch_read_output", "kwh_batch_destroy",
-    "kwh_time_reversal", "kwh_sim_set_nonfinite_check",
+    "kwh_time_reversal", "kwh_sim_set_nonfinite_check", "kwh_sim_read_output_noalloc",
 ]
 
 
@@ -92,6 +92,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.kwh_sim_sync.argtypes = [vp]
     lib.kwh_sim_output_shape.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     lib.kwh_sim_read_output.argtypes = [vp, C.c_int32, vp, C.c_size_t]
+    lib.kwh_sim_read_output_noalloc.argtypes = [vp, C.c_int32, vp, C.c_size_t]
     lib.kwh_sim_steps_done.argtypes = [vp, C.POINTER(C.c_int32)]
     lib.kwh_sim_state_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
     lib.kwh_sim_save_state.argtypes = [vp, vp, C.c_size_t]
