@@ -35,7 +35,8 @@ from . import kwave_oracle as O
 class LosslessOperators:
     """Operators of the lossless loop for one grid / medium / sensor (c_ref from c_ref_from)."""
 
-    def __init__(self, grid, rho0, sensor_mask, options, c_ref):
+    def __init__(self, grid, rho0, sensor_mask, options, c_ref, workers=1):
+        self.fft = O._NumpyFFT(workers)
         self.dim = grid.dim
         self.sizes = list(grid.size)
         self.spac = list(grid.spacing)
@@ -61,7 +62,7 @@ class LosslessOperators:
         self.sampler = O.make_sampler(sensor_mask, sizes, spac)
 
     def D(self, op, f):
-        return np.real(np.fft.ifftn(op * np.fft.fftn(f)))
+        return np.real(self.fft.ifftn(op * self.fft.fftn(f)))
 
     def sample(self, p):
         return self.sampler.sample(p)

@@ -472,6 +472,93 @@ def finalize_recording(record, p, us, p_final, u_finals):
     return out
 
 
+class _Recorder:
+    """The recording of one sensor (RecordPlan / SensorSampler, Recording.swift:8-97, and
+    finalizeRecording, :129-168). When the record needs no series (no p / u* / iAvg) the samples
+    are folded into running max / min / sum of squares instead of being stored, which is the same
+    max / min / rms over the recorded steps without the [nSensor, nt] array."""
+
+    def __init__(self, sensor, sizes, spac, dim, kv):
+        self.dim = dim
+        self.rec = rec = int(sensor.record) if sensor is not None else P
+        self.rec_p = bool(rec & (P | P_MAX | P_MIN | P_RMS | I_AVG))
+        self.rec_u = bool(rec & (UX | UY | UZ | U_MAX | U_RMS | I_AVG))
+        self.sampler = (make_sampler(sensor.mask, sizes, spac)
+                        if (sensor is not None and sensor.mask is not None) else None)
+        self.direct = directivity_filter(sensor, sizes, spac) if sensor is not None else None
+        self.colloc = [_bcast(collocation_op(kv[a], spac[a]), a, dim) for a in range(dim)] if self.rec_u else None
+        self.keep = bool(rec & (P | UX | UY | UZ | I_AVG))
+        self.p_rec, self.u_rec = [], [[] for _ in range(dim)]
+        self.p_agg, self.u_agg = None, [None] * dim
+
+    @staticmethod
+    def _fold(agg, v):
+        if agg is None:
+            return [v.copy(), v.copy(), v * v, 1]
+        np.maximum(agg[0], v, out=agg[0])          # NaN propagates, as in max over the series
+        np.minimum(agg[1], v, out=agg[1])
+        agg[2] += v * v
+        agg[3] += 1
+        return agg
+
+    def record(self, p, u, fft):
+        if self.sampler is None:
+            return
+        if self.rec_p:
+            if self.direct is not None:
+                entries, nsens = self.direct
+                pkd = fft.fftn(p)
+                v = np.zeros(nsens)
+                for wgt, rows, gather in entries:
+                    v[rows] = np.real(fft.ifftn(pkd * wgt)).reshape(-1)[gather]
+            else:
+                v = self.sampler.sample(p)
+            if self.keep:
+                self.p_rec.append(v)
+            else:
+                self.p_agg = self._fold(self.p_agg, v)
+        if self.rec_u:
+            for a in range(self.dim):
+                uc = np.real(fft.ifftn(self.colloc[a] * fft.fftn(u[a])))
+                v = self.sampler.sample(uc)
+                if self.keep:
+                    self.u_rec[a].append(v)
+                else:
+                    self.u_agg[a] = self._fold(self.u_agg[a], v)
+
+    def finish(self, p, u):
+        rec = self.rec
+        if self.keep or (self.p_agg is None and all(a is None for a in self.u_agg)):
+            return finalize_recording(
+                rec,
+                np.stack(self.p_rec, axis=1) if self.p_rec else None,
+                [np.stack(self.u_rec[a], axis=1) if self.u_rec[a] else None for a in range(self.dim)],
+                p,
+                [x for x in u] if rec & U_FINAL else None,
+            )
+        out = {"pFinal": p}
+        if self.p_agg is not None:
+            mx, mn, s2, n = self.p_agg
+            if rec & P_MAX:
+                out["pMax"] = mx
+            if rec & P_MIN:
+                out["pMin"] = mn
+            if rec & P_RMS:
+                out["pRms"] = np.sqrt(s2 / n)
+        for a, agg in enumerate(self.u_agg):
+            if agg is None:
+                continue
+            nm = "xyz"[a]
+            mx, _, s2, n = agg
+            if rec & U_MAX:
+                out[f"u{nm}Max"] = mx
+            if rec & U_RMS:
+                out[f"u{nm}Rms"] = np.sqrt(s2 / n)
+            if rec & U_FINAL:
+                out[f"u{nm}Final"] = u[a]
+        return out
+
+
 # ---------------------------------------------------------------------------------------------
 # Solver (KSpaceFirstOrder.swift:203-761)
 # ---------------------------------------------------------------------------------------------
@@ -484,14 +571,15 @@ def _mode(x, default):
 
 
 def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, workers=1,
-                       t_begin=0, t_end=None, state=None, return_state=False):
+                       t_begin=0, t_end=None, state=None, return_state=False, progress=None):
     """Run the k-space first-order loop and return a dict of recorded outputs.
 
     `grid` exposes dim, size (list), spacing (list), dt, nt. `medium` exposes soundSpeed,
     density, alphaCoeff, alphaPower, alphaMode, bOnA. `source` exposes p0, pMask, p, pMode,
     uMask, ux, uy, uz, uMode. `sensor` exposes mask, record (RecordField bits),
-    directivityAngle, directivitySize, directivityPattern. `options` exposes pmlSize, pmlAlpha,
-    smoothP0, smoothC0, smoothRho0.
+    directivityAngle, directivitySize, directivityPattern; a list of sensors records each of them
+    in the same run (a list of dicts is returned). `options` exposes pmlSize, pmlAlpha, smoothP0,
+    smoothC0, smoothRho0. `progress(t)` is called after every step.
 
     Follows kspaceFirstOrder1D/2D/3D (KSpaceFirstOrder.swift:203-349, 351-555, 557-761); the
     2D no-source fused step (:475-483) is the same arithmetic and is not separated here.
@@ -604,12 +692,8 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
             return ab
         return ab - eta * np.real(fft.ifftn(nab2 * fft.fftn(rho_t)))
 
-    rec = int(sensor.record) if sensor is not None else P
-    rec_p = bool(rec & (P | P_MAX | P_MIN | P_RMS | I_AVG))
-    rec_u = bool(rec & (UX | UY | UZ | U_MAX | U_RMS | I_AVG))
-    sampler = make_sampler(sensor.mask, sizes, spac) if (sensor is not None and sensor.mask is not None) else None
-    direct = directivity_filter(sensor, sizes, spac) if sensor is not None else None
-    colloc = [_bcast(collocation_op(kv[a], spac[a]), a, dim) for a in range(dim)] if rec_u else None
+    several = isinstance(sensor, (list, tuple))
+    recorders = [_Recorder(sn, sizes, spac, dim, kv) for sn in (sensor if several else [sensor])]
 
     cast = (lambda x: x) if R is np.float64 else (lambda x: np.asarray(x, dtype=R))
     if R is not np.float64:
@@ -637,7 +721,6 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
         p = state["p"].copy()
         u = [x.copy() for x in state["u"]]
         rho = [x.copy() for x in state["rho"]]
-    p_rec, u_rec = [], [[] for _ in range(dim)]
     t_stop = nt if t_end is None else t_end
     import time as _time
     loop_start = _time.perf_counter()
@@ -677,36 +760,20 @@ def kspace_first_order(grid, medium, source, sensor, options, dtype=np.float64, 
             pk0 = fft.fftn(p)
             for a in range(dim):
                 u[a] = cast((dt_over_rho[a] / 2.0) * np.real(fft.ifftn(ddx_pos[a] * kappa * pk0)))
-        if sampler is not None:
-            if rec_p:
-                if direct is not None:
-                    entries, nsens = direct
-                    pkd = fft.fftn(p)
-                    s = np.zeros(nsens)
-                    for wgt, rows, gather in entries:
-                        s[rows] = np.real(fft.ifftn(pkd * wgt)).reshape(-1)[gather]
-                    p_rec.append(s)
-                else:
-                    p_rec.append(sampler.sample(p))
-            if rec_u:
-                for a in range(dim):
-                    uc = np.real(fft.ifftn(colloc[a] * fft.fftn(u[a])))
-                    u_rec[a].append(sampler.sample(uc))
+        for r in recorders:
+            r.record(p, u, fft)
         step_seconds.append(_time.perf_counter() - step_start)
+        if progress is not None:
+            progress(t)
 
     loop_seconds = _time.perf_counter() - loop_start
-    out = finalize_recording(
-        rec,
-        np.stack(p_rec, axis=1) if p_rec else None,
-        [np.stack(u_rec[a], axis=1) if u_rec[a] else None for a in range(dim)],
-        p,
-        [x for x in u] if rec & U_FINAL else None,
-    )
-    if return_state:
-        out["_state"] = {"p": p, "u": u, "rho": rho}
-    out["_loop_seconds"] = loop_seconds
-    out["_step_seconds"] = step_seconds
-    return out
+    outs = [r.finish(p, u) for r in recorders]
+    for out in outs:
+        if return_state:
+            out["_state"] = {"p": p, "u": u, "rho": rho}
+        out["_loop_seconds"] = loop_seconds
+        out["_step_seconds"] = step_seconds
+    return outs if several else outs[0]
 
 
 def closed_form_lossless(p0, sizes, spacings, c, dt, steps):

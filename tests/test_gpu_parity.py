@@ -659,22 +659,6 @@ def test_generic_and_register_paths_agree():
     assert rl2 < 1e-12
 
 
-def test_3d_256_headline_workload_short():
-    """The 256^3 workload itself (GRF media, power law y = 1.1, p0 ball, plane p + uz), 6 steps
-    against the float64 oracle."""
-    n = 256
-    c = random_field((n, n, n), 3, 20.0, 1400.0, 1600.0)
-    rho = random_field((n, n, n), 4, 20.0, 900.0, 1100.0)
-    med = KWaveMedium(c, rho, alphaCoeff=0.5, alphaPower=1.1, alphaMode=AbsorptionMode.powerLaw)
-    grid = KWaveGrid(n, 1e-4, n, 1e-4, n, 1e-4)
-    grid.setTime(6, 0.3 * 1e-4 / float(c.max()))
-    sens = np.zeros((n, n, n), np.float32)
-    sens[136, :, :] = 1    # close enough to the ball for signal within 6 steps
-    check("3d_256_short", grid, med, KWaveSource(p0=makeBall(n, n, n, radius=8)),
-          KWaveSensor(mask=sens, record=RecordField.p | RecordField.uz), ["p", "uz", "pFinal"], pmlSize=10,
-          workers=16)
-
-
 def test_3d_256_fp32_vs_fp64_100_steps():
     """FP32 against FP64 on the device at 256^3 over 100 steps: relative L2 of the plane p / uz
     series and of pFinal normalised by max_t |p| (target <= 1e-4)."""

@@ -357,3 +357,30 @@ def test_float32_mode_stays_single_precision():
     for k in ("p", "ux"):
         err = np.abs(r32[k] - r64[k]).max() / np.abs(r64[k]).max()
         assert 1e-9 < err < 1e-5, (k, err)
+
+
+def test_streamed_aggregates_and_several_sensors_equal_separate_runs():
+    """Aggregates folded step by step (no series stored) equal those of the stored series, and one
+    run recording several sensors equals one run per sensor."""
+    n = 40
+    grid = KWaveGrid(n, 1e-4, n, 1e-4)
+    grid.setTime(50, 2e-8)
+    med = KWaveMedium(1500.0, 1000.0, alphaCoeff=0.5, alphaPower=1.5, alphaMode=AbsorptionMode.powerLaw)
+    src = KWaveSource(p0=makeDisc(n, n, radius=4))
+    full = np.ones((n, n), np.float32)
+    line = np.zeros((n, n), np.float32)
+    line[6, 4:30] = 1
+    agg = RecordField.pMax | RecordField.pMin | RecordField.pRms | RecordField.uMax | RecordField.uRms
+    s_agg = KWaveSensor(mask=full, record=agg)
+    s_ser = KWaveSensor(mask=full, record=agg | RecordField.p | RecordField.ux | RecordField.uy)
+    s_line = KWaveSensor(mask=line, record=RecordField.p)
+    both = O.kspace_first_order(grid, med, src, [s_agg, s_line], SimulationOptions(pmlSize=8))
+    ser = O.kspace_first_order(grid, med, src, s_ser, SimulationOptions(pmlSize=8))
+    one = O.kspace_first_order(grid, med, src, s_line, SimulationOptions(pmlSize=8))
+    assert "p" not in both[0]
+    for k in ("pMax", "pMin", "uxMax", "uyMax"):
+        np.testing.assert_array_equal(both[0][k], ser[k])
+    for k in ("pRms", "uxRms", "uyRms"):
+        np.testing.assert_allclose(both[0][k], ser[k], rtol=1e-13)
+    np.testing.assert_array_equal(both[1]["p"], one["p"])
+    np.testing.assert_array_equal(both[1]["pFinal"], one["pFinal"])
