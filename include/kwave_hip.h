@@ -297,6 +297,51 @@ void kwh_batch_destroy(kwh_batch* batch);
 
 void kwh_sim_destroy(kwh_sim* sim);
 
+/* ---- k-Wave HDF5 input / output -------------------------------------------------------------
+ * The k-Wave C++ file format the reference writes and reads through CHDF5 (KWaveCpp.writeInput2D,
+ * readPFinal2D: Sources/KWave/IO/KWaveInput.swift:12-99; HDF5File: Sources/KWave/IO/HDF5.swift:
+ * 26-190), so that `kwave-cli run --input sim.h5 --output results.h5` (REQUIREMENTS.md 8.3) runs
+ * on the device. Host-only calls: libhdf5 (>= 1.10) is loaded with dlopen on first use
+ * (KWH_HDF5_LIBRARY, the usual sonames, /opt/conda/lib); without it every call returns
+ * KWH_EUNSUPPORTED. Layout: uint64 [1,1,1] integer scalars and flags, float32 [1,1,1] real
+ * scalars, float32 [Nz,Ny,Nx] fields (x fastest), uint64 [1,1,Ns] sensor_mask_index (1-based,
+ * column-major), root attributes file_type / major_version "1" / minor_version "2". */
+typedef struct kwh_h5_input kwh_h5_input;
+
+/* KWH_OK when libhdf5 can be loaded, else KWH_EUNSUPPORTED with the reason. */
+int kwh_h5_available(void);
+
+/* Write a k-Wave input file for a p0 problem (writeInput2D, KWaveInput.swift:12-99, extended to
+ * 3D grids, sound-speed / density maps and B/A): grid, Nt, dt, one PML size and alpha on every
+ * axis, c0 / c_ref = max(c0) / rho0 and its staggered copies (scalars or fields), BonA, the p0
+ * field as given (the smoothing flags are not stored: the file holds the final p0) and the binary
+ * sensor mask. Time-varying sources, absorption, Cartesian sensors and directivity are
+ * KWH_EUNSUPPORTED. */
+int kwh_h5_write_input(const char* path, const kwh_sim_desc* desc);
+
+/* Read a k-Wave input file (2D or 3D, p0 source, index sensor mask, lossless, optional B/A) into
+ * a handle whose descriptor (kwh_h5_input_desc) points into the handle's buffers: float32, no
+ * smoothing, record = pFinal; copy it and set record / dtype before kwh_sim_create. The sensor
+ * points are in ascending C-order like every kwh_sim_desc; the handle remembers the file's
+ * order for kwh_h5_write_output. Files with other sources or flags set: KWH_EUNSUPPORTED. */
+int kwh_h5_read_input(const char* path, kwh_h5_input** out);
+const kwh_sim_desc* kwh_h5_input_desc(kwh_h5_input* in);
+void kwh_h5_input_destroy(kwh_h5_input* in);
+
+/* Write a k-Wave output file from a sim that has run: p_final as [Nz,Ny,Nx] (readPFinal2D,
+ * KWaveInput.swift:101-111), and the recorded p / p_max / p_min / p_rms as [1,Nt,Ns] / [1,1,Ns]
+ * with the sensor points in the input file's order when `in` is given (else ascending C-order).
+ * Values in the sim's precision (float32 sims write float32). */
+int kwh_h5_write_output(const char* path, kwh_sim* sim, const kwh_h5_input* in);
+
+/* Read any numeric dataset as float64 (HDF5File.readFloatDataset, HDF5.swift:50-71): dims[ndim]
+ * as stored; dst == NULL queries the shape only. */
+int kwh_h5_read_dataset(const char* path, const char* name, int64_t dims[8], int32_t* ndim, double* dst,
+                        size_t dst_count);
+
+/* Read a root string attribute (HDF5File.readRootStringAttribute, HDF5.swift:149-161). */
+int kwh_h5_read_root_attribute(const char* path, const char* name, char* dst, size_t dst_bytes);
+
 #ifdef __cplusplus
 }
 #endif

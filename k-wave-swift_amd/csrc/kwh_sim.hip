@@ -200,6 +200,7 @@ struct kwh_sim {
     virtual int device() const = 0;
     virtual const StepRecord& recorded_step(int t) = 0;   // step t's launches, recorded once per step key
     virtual uint64_t shape_class() const = 0;            // grid shape and precision
+    virtual int elem_bytes() const = 0;                  // 4 (float32) or 8 (float64)
 };
 
 namespace {
@@ -2416,6 +2417,7 @@ struct SimT final : kwh_sim {
     }
 
     int steps_done() const override { return steps; }
+    int elem_bytes() const override { return (int)sizeof(T); }
 
     void output_shape(int field, int64_t shape[3], int* ndim) override {
         shape[0] = shape[1] = shape[2] = 1;
@@ -3009,6 +3011,14 @@ extern "C" {
 int kwh_abi_version(void) { return KWH_ABI_VERSION; }
 
 const char* kwh_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+// Shared with the k-Wave HDF5 I/O translation unit (kwh_h5.cpp); not part of the exported ABI.
+__attribute__((visibility("hidden"))) void kwh_set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
+__attribute__((visibility("hidden"))) int kwh_sim_elem_bytes(const kwh_sim* sim) { return sim ? sim->elem_bytes() : 0; }
+
+extern "C" {
 
 // Once per device, with the device current: the address of that device's copy of `scratch`.
 static void resolve_scratch(int device) {

@@ -30,6 +30,8 @@ EXPORTED = [
     "kwh_sim_element_traces", "kwh_sim_autotune", "kwh_sim_set_checkpointing", "kwh_sim_vjp", "kwh_sim_destroy",
     "kwh_batch_create", "kwh_batch_run", "kwh_batch_sim", "kwh_batch_read_output", "kwh_batch_destroy",
     "kwh_time_reversal", "kwh_sim_set_nonfinite_check", "kwh_sim_read_output_noalloc",
+    "kwh_h5_available", "kwh_h5_write_input", "kwh_h5_read_input", "kwh_h5_input_desc", "kwh_h5_input_destroy",
+    "kwh_h5_write_output", "kwh_h5_read_dataset", "kwh_h5_read_root_attribute",
 ]
 
 
@@ -116,6 +118,17 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.kwh_batch_read_output.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_size_t]
     lib.kwh_batch_destroy.argtypes = [vp]
     lib.kwh_batch_destroy.restype = None
+    lib.kwh_h5_available.restype = C.c_int
+    lib.kwh_h5_write_input.argtypes = [C.c_char_p, C.POINTER(SimDesc)]
+    lib.kwh_h5_read_input.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.kwh_h5_input_desc.argtypes = [vp]
+    lib.kwh_h5_input_desc.restype = C.POINTER(SimDesc)
+    lib.kwh_h5_input_destroy.argtypes = [vp]
+    lib.kwh_h5_input_destroy.restype = None
+    lib.kwh_h5_write_output.argtypes = [C.c_char_p, vp, vp]
+    lib.kwh_h5_read_dataset.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_double), C.c_size_t]
+    lib.kwh_h5_read_root_attribute.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]
     if lib.kwh_abi_version() != ABI_VERSION:
         raise ImportError("libkwavehip ABI version mismatch")
     _lib = lib

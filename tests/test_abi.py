@@ -17,7 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "kwave_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(kwh_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*|const kwh_sim_desc\*)\s+(kwh_\w+)\s*\(", text,
+                                 flags=re.M)))
 
 
 def test_library_loads_and_exports_every_declared_symbol():

@@ -1,7 +1,7 @@
 """Full-size runs checked through a size-independent property: for a homogeneous, lossless,
 PML-free medium the k-space corrected loop is exact, so the final pressure's spectrum is
 P0(k) cos(c |k| (nt-1) dt) at every wavevector (KSpaceFirstOrder.swift:557-761 with the t = 0
-override :724-733). The spectrum is sampled at a few wavevectors by separable sums over the
+override :724-733). The spectrum is sampled at 64-128 wavevectors by separable sums over the
 three axes, so no full-size FFT runs on the host.
 
 256^3 runs the register-resident kernels of the benchmark; 1024^3 (1.07 G cells, int64 offsets,
@@ -53,11 +53,18 @@ def test_fullsize_lossless_closed_form(n, nt, r):
     assert pf.shape == (n, n, n) and np.all(np.isfinite(pf))
     kv = O.fft_freq(n, dx) * 2.0 * np.pi
     scale = float(p0.sum())   # |P0(0)|, the largest spectral magnitude of a non-negative p0
-    idxs = [(0, 0, 0), (1, 0, 0), (0, 3, 5), (7, -4, 2), (n // 8, n // 16, -n // 32), (n // 4, 5, n // 3)]
+    # a few structured wavevectors (DC, axes, the Nyquist planes) and seeded random ones over the
+    # whole band: 128 at 256^3, 64 at 1024^3
+    idxs = [(0, 0, 0), (1, 0, 0), (0, 3, 5), (7, -4, 2), (n // 8, n // 16, -n // 32), (n // 4, 5, n // 3),
+            (n // 2, 0, 0), (0, n // 2, 1), (3, 2, n // 2), (n // 2, n // 2, n // 2)]
+    rng = np.random.default_rng(n)
+    total = 128 if n <= 256 else 64
+    idxs += [tuple(int(v) for v in rng.integers(-n // 2, n // 2, 3)) for _ in range(total - len(idxs))]
     kmag = np.array([np.sqrt(sum(kv[i % n] ** 2 for i in idx)) for idx in idxs])
     want = _spectra(p0, idxs) * np.cos(c * kmag * (nt - 1) * grid.dt)
     got = _spectra(pf, idxs)
     err = np.abs(got - want) / scale
-    for idx, w, e in zip(idxs, want, err):
-        print(f"  k={idx}: |P|={abs(w):.4e} err/|P0(0)|={e:.2e}", flush=True)
+    worst = int(np.argmax(err))
+    print(f"  {len(idxs)} wavevectors: max err/|P0(0)| = {err.max():.2e} at k={idxs[worst]} "
+          f"(|P| there {abs(want[worst]):.3e}), median {float(np.median(err)):.2e}", flush=True)
     assert err.max() <= 1e-5, err.max()
