@@ -138,9 +138,11 @@ typedef struct kwh_ctx kwh_ctx;
 typedef struct kwh_sim kwh_sim;
 typedef struct kwh_batch kwh_batch;
 
-/* Called between step chunks. t is the last completed step index, p_host the pressure field
- * copied to host when the callback is due (every cb_interval steps), else NULL. Replaces the
- * reference's progress / fieldMonitor closures (KSpaceFirstOrder.swift:30-37). */
+/* Called once per step, in order, between step chunks (up to 64 steps per device sync; a chunk
+ * ends at every step whose pressure is delivered). t is the step index, p_host the pressure field
+ * after step t (sim dtype, grid-shaped) when t % cb_interval == 0, else NULL; cb_interval <= 0
+ * never delivers the field (progress only). Replaces the reference's progress / fieldMonitor
+ * closures (KSpaceFirstOrder.swift:30-37, called at :337-340). p_host is valid during the call. */
 typedef void (*kwh_step_cb)(int32_t t, int32_t nt, const void* p_host, void* user);
 
 /* Per-kernel timing record filled by kwh_sim_kernel_stats. */

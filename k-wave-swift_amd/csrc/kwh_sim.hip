@@ -2399,20 +2399,33 @@ struct SimT final : kwh_sim {
             }
             return;
         }
+        // Steps run in chunks of up to kCallbackChunk with one sync each; every step still gets its
+        // callback, in order, after its chunk. A chunk ends at the next step whose pressure the
+        // callback receives (t % interval == 0; interval <= 0: none) and at the check interval.
+        constexpr int kCallbackChunk = 64;
+        const bool want_p = interval > 0;
         const int iv = std::max(interval, 1);
         std::vector<T> host;
-        for (int t = t0; t < t1; ++t) {
-            launch(t, t + 1);
-            const bool due = (t % iv) == 0;
+        for (int t = t0; t < t1;) {
+            int te = std::min(t1, t + kCallbackChunk);
+            if (want_p) {
+                const int next_due = ((t + iv - 1) / iv) * iv;
+                if (next_due < te) te = next_due + 1;
+            }
+            if (ck > 0) te = std::min(te, (t / ck + 1) * ck);
+            launch(t, te);
+            const int last = te - 1;
+            const bool due = want_p && (last % iv) == 0;
             if (due) {
                 host.resize(g.N);
                 HIPCHK(hipMemcpyAsync(host.data(), p.p, g.N * sizeof(T), hipMemcpyDeviceToHost, stream));
             }
-            const bool scan = check_due(t);
-            if (scan) nonfinite_scan(t);
+            const bool scan = check_due(last);
+            if (scan) nonfinite_scan(last);
             sync();
-            cb(t, nt, due ? host.data() : nullptr, user);
+            for (int s = t; s <= last; ++s) cb(s, nt, (s == last && due) ? host.data() : nullptr, user);
             if (scan) nonfinite_raise();
+            t = te;
         }
     }
 

@@ -400,7 +400,8 @@ class Simulation:
         return s.value
 
     def run(self, t_begin: int = None, t_end: int = None, callback=None, interval: int = 1):
-        """Advance steps [t_begin, t_end); callback(t, nt, p_or_None) between steps."""
+        """Advance steps [t_begin, t_end); callback(t, nt, p_or_None) once per step, with the pressure
+        after step t when t % interval == 0 (interval <= 0: never)."""
         t0 = self.steps_done if t_begin is None else t_begin
         t1 = self.grid.nt if t_end is None else t_end
         if callback is None:
@@ -417,7 +418,7 @@ class Simulation:
             callback(t, nt, arr)
 
         cb = L.STEP_CB(_cb)
-        L.check(self.lib.kwh_sim_run(self.handle, t0, t1, cb, max(int(interval), 1), None))
+        L.check(self.lib.kwh_sim_run(self.handle, t0, t1, cb, int(interval), None))
 
     def launch(self, t_begin: int, t_end: int):
         L.check(self.lib.kwh_sim_launch(self.handle, t_begin, t_end))
@@ -661,7 +662,7 @@ def _run_to_output(sim: "Simulation", options: SimulationOptions) -> SimulationO
                 if options.progress is not None:
                     options.progress(t, nt)
 
-            sim.run(callback=cb, interval=iv if options.fieldMonitor is not None else 1 << 30)
+            sim.run(callback=cb, interval=iv if options.fieldMonitor is not None else 0)
         return sim.outputs()
     finally:
         sim.close()

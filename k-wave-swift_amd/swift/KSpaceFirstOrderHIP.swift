@@ -76,7 +76,56 @@ private func sourceField(mask: MLXArray?, signal: MLXArray?, mode: SourceMode, _
                             rows: Int32(rows), len: Int32(sig.count / max(rows, 1)), mode: m, delay: nil, scale: nil)
 }
 
-private var sharedContext: OpaquePointer? = nil
+/// The device context of the `.hip` path: opened on first use, kept for the process. A class with
+/// a lock instead of a global `var`, which Swift 6 language mode rejects as unprotected shared
+/// mutable state; the lock makes concurrent first uses from several threads safe.
+private final class HIPContext: @unchecked Sendable {
+    static let shared = HIPContext()
+    private let lock = NSLock()
+    private var ctx: OpaquePointer? = nil
+
+    func get() -> OpaquePointer? {
+        lock.lock()
+        defer { lock.unlock() }
+        if ctx == nil { check(kwh_ctx_create(0, &ctx), "kwh_ctx_create") }
+        return ctx
+    }
+}
+
+/// What the C step callback needs, passed through its `user` pointer (Unmanaged, retained by the
+/// caller for the duration of kwh_sim_run).
+private final class StepCallbacks {
+    let progress: ((Int, Int) -> Void)?
+    let monitor: ((Int, MLXArray) -> Void)?
+    let shape: [Int]
+    let count: Int
+    let float64: Bool
+
+    init(options: SimulationOptions, grid: KWaveGrid) {
+        progress = options.progress
+        monitor = options.fieldMonitor
+        shape = grid.size
+        count = grid.totalGridPoints
+        float64 = options.dtype == .float64
+    }
+
+    /// One call per step t (in order); pHost is the pressure after step t in the sim's precision
+    /// when the field is due, else nil.
+    func step(_ t: Int, _ nt: Int, _ pHost: UnsafeRawPointer?) {
+        if let monitor, let pHost {
+            let p: MLXArray
+            if float64 {
+                let b = UnsafeBufferPointer(start: pHost.assumingMemoryBound(to: Double.self), count: count)
+                p = MLXArray(converting: Array(b)).reshaped(shape)
+            } else {
+                let b = UnsafeBufferPointer(start: pHost.assumingMemoryBound(to: Float.self), count: count)
+                p = MLXArray(Array(b)).reshaped(shape)
+            }
+            monitor(t, p)
+        }
+        progress?(t, nt)
+    }
+}
 
 /// Flattens the API values into a descriptor whose pointers live in `arena`.
 private func makeDescriptor(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor: KWaveSensor,
@@ -137,12 +186,12 @@ private func makeDescriptor(grid: KWaveGrid, medium: KWaveMedium, source: KWaveS
 }
 
 private func createSim(_ d: inout kwh_sim_desc, axisymmetric: Bool = false) -> OpaquePointer? {
-    if sharedContext == nil { check(kwh_ctx_create(0, &sharedContext), "kwh_ctx_create") }
+    let ctx = HIPContext.shared.get()
     var sim: OpaquePointer? = nil
     if axisymmetric {
-        check(kwh_sim_create_axisymmetric(sharedContext, &d, &sim), "kwh_sim_create_axisymmetric")
+        check(kwh_sim_create_axisymmetric(ctx, &d, &sim), "kwh_sim_create_axisymmetric")
     } else {
-        check(kwh_sim_create(sharedContext, &d, &sim), "kwh_sim_create")
+        check(kwh_sim_create(ctx, &d, &sim), "kwh_sim_create")
     }
     return sim
 }
@@ -161,20 +210,17 @@ func kspaceFirstOrderHIP(grid: KWaveGrid, medium: KWaveMedium, source: KWaveSour
     if options.progress == nil && options.fieldMonitor == nil {
         check(kwh_sim_run(sim, 0, Int32(grid.nt), nil, 0, nil), "kwh_sim_run")
     } else {
-        // Step by step so progress(t, nt) and fieldMonitor(t, p) fire as in the reference (:337-340).
-        let interval = max(options.fieldMonitorInterval, 1)
-        var host = [Float](repeating: 0, count: grid.totalGridPoints)
-        for t in 0..<grid.nt {
-            check(kwh_sim_run(sim, Int32(t), Int32(t + 1), nil, 0, nil), "kwh_sim_run")
-            if let monitor = options.fieldMonitor, t % interval == 0 {
-                host.withUnsafeMutableBytes { b in
-                    check(kwh_sim_read_output(sim, Int32(KWH_OUT_P_FINAL.rawValue), b.baseAddress, b.count),
-                          "kwh_sim_read_output")
-                }
-                monitor(t, MLXArray(host).reshaped(grid.size))
-            }
-            options.progress?(t, grid.nt)
-        }
+        // progress(t, nt) every step and fieldMonitor(t, p) every fieldMonitorInterval steps, as in
+        // the reference (:337-340), delivered by the library's step callback: steps run in chunks
+        // between device syncs, and the field is copied to the host only on the steps it is due.
+        let callbacks = StepCallbacks(options: options, grid: grid)
+        let user = Unmanaged.passRetained(callbacks).toOpaque()
+        defer { Unmanaged<StepCallbacks>.fromOpaque(user).release() }
+        let interval: Int32 = options.fieldMonitor == nil ? 0 : Int32(max(options.fieldMonitorInterval, 1))
+        check(kwh_sim_run(sim, 0, Int32(grid.nt), { t, nt, pHost, user in
+            guard let user else { return }
+            Unmanaged<StepCallbacks>.fromOpaque(user).takeUnretainedValue().step(Int(t), Int(nt), pHost)
+        }, interval, user), "kwh_sim_run")
     }
 
     func read(_ which: kwh_output) -> MLXArray? {

@@ -457,6 +457,29 @@ def test_progress_and_field_monitor_cadence():
     assert prog == list(range(16))
 
 
+def test_callback_fields_equal_the_state_after_their_step():
+    """kwh_step_cb: one call per step in order (steps run in chunks between syncs), the pressure
+    after step t delivered when t % interval == 0, none with interval 0 (progress only)."""
+    n = 32
+    grid = KWaveGrid(n, 1e-4, n, 1e-4)
+    grid.setTime(150, 2e-8)
+    args = (grid, KWaveMedium(1500.0, 1000.0), KWaveSource(p0=makeDisc(n, n, radius=3)), KWaveSensor())
+    seen = {}
+    sim = Simulation(*args)
+    sim.run(callback=lambda t, nt, p: seen.__setitem__(t, None if p is None else p.copy()), interval=7)
+    assert list(seen) == list(range(150))
+    assert [t for t, v in seen.items() if v is not None] == list(range(0, 150, 7))
+    for t in (0, 7, 98, 147):
+        ref = Simulation(*args)
+        ref.run(0, t + 1)
+        assert np.array_equal(seen[t], ref.output(L.OUT_P_FINAL)), t
+    prog = []
+    quiet = Simulation(*args)
+    quiet.run(callback=lambda t, nt, p: prog.append((t, nt, p is None)), interval=0)
+    assert prog == [(t, 150, True) for t in range(150)]
+    assert np.array_equal(quiet.output(L.OUT_P_FINAL), sim.output(L.OUT_P_FINAL))
+
+
 def test_deterministic_replay():
     n = 48
     grid = KWaveGrid(n, 1e-4, n, 1e-4)
