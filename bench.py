@@ -31,19 +31,10 @@ def workload(name, dtype):
                           RecordField, SimulationOptions, makeBall, makeCircle, makeDisc, random_field)
     dt_ = DTypePrecision.float64 if dtype == "f64" else DTypePrecision.float32
     if name == "c2":
-        n, dx = 256, 1e-4
-        c = random_field((n, n, n), 3, 20.0, 1400.0, 1600.0)
-        rho = random_field((n, n, n), 4, 20.0, 900.0, 1100.0)
-        med = KWaveMedium(c, rho, alphaCoeff=0.5, alphaPower=1.1, alphaMode=AbsorptionMode.powerLaw)
-        grid = KWaveGrid(n, dx, n, dx, n, dx)
-        grid.setTime(1000, 0.3 * dx / float(c.max()))
-        src = KWaveSource(p0=makeBall(n, n, n, radius=8))
-        mask = np.zeros((n, n, n), np.float32)
-        mask[168, :, :] = 1
-        sens = KWaveSensor(mask=mask, record=RecordField.p | RecordField.uz)
-        opts = SimulationOptions(pmlSize=10, smoothP0=True, dtype=dt_)
+        from kwavehip import workloads
+        grid, med, src, sens, opts = workloads.throughput_3d(n=256, steps=1000, dtype=dt_)
         info = {"workload": "3D 256^3 heterogeneous power-law (y=1.1) p0-ball, plane sensor p+uz",
-                "grid": [n, n, n], "pml": 10, "nt": 1000}
+                "grid": [256, 256, 256], "pml": 10, "nt": 1000}
         # byte model per step: s*N*(2*d*T + S), T = 15 transforms, S = 27 other full-grid arrays
         model = {"T": 15, "S": 27, "d": 3}
     elif name == "c0":
@@ -110,36 +101,12 @@ def precision_check(grid, med, src, sens, opts):
 
 
 def scan_line_cases(n_runs=48, nt=800, seed=7):
-    """Per-scan-line transmits: 48 independent 3D FP32 runs, sizes drawn from {96^3, 128x96x96,
-    160x128x128}, speckle media c = 1540 (1 + U[-0.02, 0.02]) m/s, rho = 1000 (1 + U[-0.02, 0.02]),
-    a 32-element linear array (width 2, length 24, kerf 0) at x = 10 driven with a 2 MHz 2-cycle
-    burst, steering U[-20, 20] deg, focus U[15, 40] mm, elevation focus 20 mm, nt = 800, receive on
-    the active voxels (p), combined per element."""
-    from kwavehip import (KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, KWaveTransducer,
-                          KWaveTransducerGeometry, RecordField, SimulationOptions, SourceMode, toneBurst)
-    sizes = [(96, 96, 96), (128, 96, 96), (160, 128, 128)]
-    pick = np.random.default_rng(seed).integers(0, 3, n_runs)
-    dx = 1e-4
-    cases = []
-    for r in range(n_runs):
-        shape = sizes[pick[r]]
-        rng = np.random.default_rng(100 + r)
-        c = (1540.0 * (1 + rng.uniform(-0.02, 0.02, shape))).astype(np.float32)
-        rho = (1000.0 * (1 + rng.uniform(-0.02, 0.02, shape))).astype(np.float32)
-        grid = KWaveGrid(shape[0], dx, shape[1], dx, shape[2], dx)
-        grid.setTime(nt, 0.3 * dx / float(c.max()))
-        beam = np.random.default_rng(200 + r)
-        steer, focus = beam.uniform(-20.0, 20.0), beam.uniform(15e-3, 40e-3)
-        geom = KWaveTransducerGeometry(numberElements=32, elementWidth=2, elementLength=24, elementSpacing=0,
-                                       position=(10, (shape[1] - 64) // 2, (shape[2] - 24) // 2), dy=dx, dz=dx)
-        burst = toneBurst(1 / grid.dt, 2e6, 2).astype(np.float32)
-        tr = KWaveTransducer(geom, shape, grid.dt, 1540.0, focusDistance=focus, elevationFocusDistance=20e-3,
-                             steeringAngle=steer, inputSignal=burst)
-        mask, row, delays, scales = tr.velocity_source()
-        src = KWaveSource(uMask=mask, ux=row, uDelay=delays, uScale=scales, uMode=SourceMode.additive)
-        sens = KWaveSensor(mask=mask, record=RecordField.p)
-        cases.append(((grid, KWaveMedium(c, rho), src, sens, SimulationOptions(pmlSize=10, smoothP0=False)), tr))
-    return cases
+    """Per-scan-line transmits (kwavehip.workloads.scan_line_batch): 48 independent 3D FP32 runs,
+    sizes drawn from {96^3, 128x96x96, 160x128x128}, speckle media, a 32-element linear array at
+    x = 10 with per-run steering and focus, nt = 800, receive on the active voxels (p), combined
+    per element."""
+    from kwavehip import workloads
+    return workloads.scan_line_batch(n_runs, nt, seed)
 
 
 def batch_comparison(n_runs=48, nt=800):
@@ -178,25 +145,10 @@ def batch_comparison(n_runs=48, nt=800):
 
 
 def adjoint_case():
-    """2D sound-speed / photoacoustic inversion step: 512^2, dx = 1e-4 m, PML 20, c from a seeded
-    Gaussian random field in [1450, 1650] m/s (seed 11), rho = 1000, p0 = 20 random discs
-    (seed 12, radius 3..12 cells, amplitude U[0.2, 1]), smoothP0 off, 256-point Cartesian circle of
-    radius 22 mm, nt = 1500, L = 1/2 ||p - p_obs||^2 with p_obs from c + 1% random field (seed 13)."""
-    from kwavehip import KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, RecordField, SimulationOptions, makeCartCircle, makeDisc, random_field
-    n, dx = 512, 1e-4
-    grid = KWaveGrid(n, dx, n, dx)
-    c = random_field((n, n), 11, 20.0, 1450.0, 1650.0)
-    grid.setTime(1500, 0.3 * dx / float(c.max()))
-    rng = np.random.default_rng(12)
-    p0 = np.zeros((n, n), np.float32)
-    for _ in range(20):
-        cx, cy = rng.integers(120, n - 120, 2)
-        p0 += float(rng.uniform(0.2, 1.0)) * makeDisc(n, n, cx=int(cx), cy=int(cy), radius=int(rng.integers(3, 13)))
-    pert = random_field((n, n), 13, 20.0, -1.0, 1.0)
-    c_obs = (c.astype(np.float64) * (1.0 + 0.01 * pert)).astype(np.float32)
-    sens = KWaveSensor(mask=makeCartCircle(22e-3, 256), record=RecordField.p)
-    opts = SimulationOptions(pmlSize=20, smoothP0=False)
-    return grid, c, c_obs, p0, sens, opts
+    """2D sound-speed / photoacoustic inversion (kwavehip.workloads.inversion_2d): 512^2, 1500
+    steps, 256-point Cartesian circle, L = 1/2 ||p - p_obs||^2 with p_obs from a 1% perturbation."""
+    from kwavehip import workloads
+    return workloads.inversion_2d(n=512, steps=1500)
 
 
 def adjoint_comparison(n_dirs=16):
@@ -323,7 +275,8 @@ def reference_scenarios(repeats=5):
 
 def device_copy_bandwidth():
     """Event-timed bandwidth of a plain 1 GiB -> 1 GiB device copy (kwavehip/copy_probe, a child
-    process): the practical HBM ceiling the solver's kernels are compared with."""
+    process): a reference point beside the solver's kernels, not a ceiling (the column passes of
+    the solver sustain more); its PMC counts calibrate FETCH_SIZE / WRITE_SIZE to bytes."""
     import subprocess
     exe = os.path.join(ROOT, "k-wave-swift_amd", "kwavehip", "copy_probe")
     if not os.path.exists(exe):
@@ -335,7 +288,8 @@ def device_copy_bandwidth():
         return None
     rec = json.loads(res.stdout.strip().splitlines()[-1])
     return {"GBps": rec["GBps"], "frac_of_8TBps": round(rec["GBps"] / 8000.0, 4), "avg_us": rec["avg_us"],
-            "bytes_per_launch": rec["bytes_per_launch"], "kernel": "copy_kernel (k-wave-swift_amd/csrc/copy_probe.hip)"}
+            "bytes_per_launch": rec["bytes_per_launch"], "kernel": "copy_kernel (k-wave-swift_amd/csrc/copy_probe.hip)",
+            "note": "plain grid-stride copy: a reference point, not the HBM ceiling"}
 
 
 def cpu_baseline(grid, med, src, sens, opts, steps, workers, dtype=np.float64):
@@ -440,7 +394,26 @@ def main():
                 "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_us": round(avg_s * 1e6, 2),
                 "launches_timed": dom["launches"]}
     step_model = {"bytes_per_step": model_bytes, "GBps": round(model_bytes / (ms_step * 1e-3) / 1e9, 1),
-                  "frac_of_8TBps": round(model_bytes / (ms_step * 1e-3) / HBM_PEAK, 4)}
+                  "frac_of_8TBps": round(model_bytes / (ms_step * 1e-3) / HBM_PEAK, 4),
+                  "note": "algorithmic byte model of the step (transform passes + full-grid arrays), not measured "
+                          "traffic; see step_measured for the DRAM bytes the counters saw"}
+    # DRAM bytes per step from the committed PMC summary (every counted kernel x its launches per
+    # step in the warmup steps) at the measured step time
+    step_measured = None
+    if warm_stats:
+        per_step, missing = 0, []
+        for st in warm_stats:
+            tr = measured_traffic(args.workload, args.dtype, st["name"])
+            if tr is None:
+                missing.append(st["name"])
+                continue
+            per_step += tr["bytes"] * st["launches"] / max(args.warmup - 1, 1)
+        if per_step:
+            step_measured = {"bytes_per_step": int(per_step),
+                             "GBps": round(per_step / (ms_step * 1e-3) / 1e9, 1),
+                             "frac_of_8TBps": round(per_step / (ms_step * 1e-3) / HBM_PEAK, 4),
+                             "source": "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch)",
+                             "kernels_without_counts": missing}
     sim.close()
 
     del outputs
@@ -468,7 +441,10 @@ def main():
 
     cpu = cpu_other = None
     if not args.no_cpu_baseline:
-        workers = min(os.cpu_count() or 1, 16)
+        # every core of the process's share: os.cpu_count(), capped at 16, the CPU share of one GPU on
+        # the GPU boxes (their os.cpu_count() reports the whole node)
+        ncpu = os.cpu_count() or 1
+        workers = min(ncpu, 16)
         legs = {}
         for tag, npdt in (("f32", np.float32), ("f64", np.float64)):
             print(f"[bench] cpu baseline: {tag} oracle, {args.cpu_steps} steps, {workers} threads", file=sys.stderr,
@@ -476,6 +452,8 @@ def main():
             v, sec = cpu_baseline(grid, med, src, sens, opts, args.cpu_steps, workers, npdt)
             legs[tag] = {"value": round(v, 3), "unit": "Mcell-steps/s", "cores": workers, "kind": "port",
                          "dtype": tag,
+                         "cores_note": f"{workers} threads = min(os.cpu_count() = {ncpu}, 16: the CPU share of "
+                                       f"one GPU on the GPU box)",
                          "sample": f"steps 1..{args.cpu_steps} of the same workload after the t = 0 step as "
                                    f"warmup, median step {sec:.2f} s, numpy/scipy oracle in "
                                    f"{'float32' if tag == 'f32' else 'float64'}"}
@@ -490,7 +468,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": args.dtype, "data": "synthetic (seeded random media)",
         "config": dict(info, dtype=args.dtype),
-        "roofline": roof, "step_model": step_model, "device_copy": copy_bw, "kernels": kernels, "cpu_baseline": cpu,
+        "roofline": roof, "step_model": step_model, "step_measured": step_measured, "device_copy": copy_bw, "kernels": kernels, "cpu_baseline": cpu,
         "cpu_baseline_other_dtype": cpu_other, "fp64_gpu": fp64,
         "batched_scan_lines": batch, "adjoint": adjoint,
         "setup_s": round(setup_s, 2),

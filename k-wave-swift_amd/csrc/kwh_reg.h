@@ -50,6 +50,15 @@
 #ifndef KWH_DENSITY_HOLD_RHO0
 #define KWH_DENSITY_HOLD_RHO0 1
 #endif
+// Element loops versioned on run-time switches that are uniform over a launch (operator kind of a
+// middle pass; the EOS kernel's tau / eta source, dispersion and B/A), so no element carries a
+// branch.
+#ifndef KWH_MIDDLE_VERSIONED
+#define KWH_MIDDLE_VERSIONED 1
+#endif
+#ifndef KWH_EOS_VERSIONED
+#define KWH_EOS_VERSIONED 1
+#endif
 
 namespace kwh {
 
@@ -121,11 +130,13 @@ struct RegRow {
     // Load the Hermitian-extended Z = A + iB of rows (ra, rb) in the lane-natural pattern.
     // Elements n = j + J m < L/2 read k = n from base (j); n >= L/2 read k = L - n from base
     // (L - j) with negative compile-time offsets, so every load is base + immediate.
+    template <bool FULL = false>
     __device__ static __forceinline__ void load_spectrum(const cx<T>* __restrict__ spec, int ra, int rb, int nrows,
                                                          int mzh, int j, cx<T>* v) {
-        const T ma = ra < nrows ? T(1) : T(0), mb = rb < nrows ? T(1) : T(0);
-        const cx<T>* sa = spec + (int64_t)(ra < nrows ? ra : 0) * mzh;
-        const cx<T>* sb = spec + (int64_t)(rb < nrows ? rb : 0) * mzh;
+        const bool va = FULL || ra < nrows, vb = FULL || rb < nrows;
+        const T ma = va ? T(1) : T(0), mb = vb ? T(1) : T(0);
+        const cx<T>* sa = spec + (int64_t)(va ? ra : 0) * mzh;
+        const cx<T>* sb = spec + (int64_t)(vb ? rb : 0) * mzh;
         const cx<T>* la = sa + j;
         const cx<T>* lb = sb + j;
         const cx<T>* ha = sa + (L - j);
@@ -149,6 +160,7 @@ struct RegRow {
     }
 
     // Forward of v (a in .x, b in .y), split into half spectra, stored to rows (ra, rb).
+    template <bool FULL = false>
     __device__ static __forceinline__ void forward_store(cx<T>* spec, int ra, int rb, int nrows, int mzh, cx<T>* lds,
                                                          int j, const LaneTw<T, J, E>& tw, cx<T>* v) {
         transform<false>(v, lds, j, tw);
@@ -156,7 +168,7 @@ struct RegRow {
 #pragma unroll
         for (int m = 0; m < E; ++m) wr[(J + 1) * m] = v[m];
         __syncthreads();
-        const bool va = ra < nrows, vb = rb < nrows;
+        const bool va = FULL || ra < nrows, vb = FULL || rb < nrows;
         // k = j + J m: slot j + (J+1) m; mirror L - k: slot (L - j + E - [j>0]) - (J+1) m (k > 0)
         const cx<T>* pk = lds + j;
         const cx<T>* pm = lds + (L - j + E - (j > 0 ? 1 : 0));
@@ -189,6 +201,16 @@ struct RegZPos {
         rb = ra + 1;
     }
 };
+
+// Every row of the workgroup's tile (P row pairs) is inside the grid: the row kernels are then
+// instantiated with FULL = true, so their per-element row predicates fold away and no store carries
+// an exec-mask branch. Uniform over the workgroup.
+__device__ __forceinline__ bool reg_tile_full(const BIdx& bi, int P, int nrows) {
+    return ((int64_t)bi.tile() + 1) * P * 2 <= nrows;
+}
+#ifndef KWH_ROW_FULL_TILES
+#define KWH_ROW_FULL_TILES 1
+#endif
 
 // Sparse sources on register-resident rows: go through LDS (natural order) when the tile has any.
 template <typename T, int J, int E>
@@ -331,8 +353,8 @@ constexpr int kVelGroup = KWH_VEL_GROUP;   // velocity update (keeps it within t
 
 
 // Velocity update, per axis: C2R_z(G_a) -> u_a = pml_sg (pml_sg u_a - dt/rho_a g) [+ sources] -> R2C_z.
-template <typename T, int J, int E, bool CORR>
-__device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArgs<T>& a, const Geo& g, int P,
+template <typename T, int J, int E, bool CORR, bool FULL>
+__device__ __forceinline__ void reg_z_velocity_impl(const BIdx& bi, const VelArgs<T>& a, const Geo& g, int P,
                                                     const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -342,7 +364,7 @@ __device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArg
     LaneTw<T, J, E> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
-    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
     for (int ax = 0; ax < g.dim; ++ax) {
@@ -358,7 +380,7 @@ __device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArg
         const T* dtb = dtr.lane(ob, ps.j);
         const T* cra = corr.lane(oa, ps.j);
         const T* crb = corr.lane(ob, ps.j);
-        S::load_spectrum(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        S::template load_spectrum<FULL>(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
         S::template transform<true>(v, lds, ps.j, lt);
         KWH_SCHED_FENCE();
         auto upd = [&](int m, T qa, T qb) {
@@ -391,6 +413,7 @@ __device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArg
                 KWH_SCHED_FENCE();
             }
         }
+
         if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{(int)(bi.tile() * P), P, 0, 0}))
             reg_apply_sparse<T, J, E>(a.sp[ax], g, 2 * bi.tile() * P, 2 * P, lds_all, ps.w, ps.j, a.ctr[0], v);
         {
@@ -402,9 +425,17 @@ __device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArg
                 if (vb) wb[J * m] = v[m].y;
             }
         }
-        S::forward_store(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+        S::template forward_store<FULL>(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
     }
 }
+template <typename T, int J, int E, bool CORR>
+__device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArgs<T>& a, const Geo& g, int P, const AxisTabs<T>& tab) {
+    if (KWH_ROW_FULL_TILES && reg_tile_full(bi, P, g.nrows))
+        reg_z_velocity_impl<T, J, E, CORR, true>(bi, a, g, P, tab);
+    else
+        reg_z_velocity_impl<T, J, E, CORR, false>(bi, a, g, P, tab);
+}
+
 template <typename T, int J, int E, bool CORR>
 __global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_VELOCITY_WAVES)))
     reg_z_velocity(VelArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
@@ -420,8 +451,8 @@ template <typename T, int J, int E, bool NL, int MODE, bool CORR>
 constexpr int density_waves() {
     return (NL || CORR) ? 1 : reg_waves<T, E>(MODE == 0 ? KWH_DENSITY_WAVES : 2);
 }
-template <typename T, int J, int E, bool NL, int MODE, bool CORR>
-__device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P,
+template <typename T, int J, int E, bool NL, int MODE, bool CORR, bool FULL>
+__device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P,
                                                    const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
     constexpr bool ACC = MODE != 0;   // accumulate rho0 * sum_a du_a
@@ -432,7 +463,7 @@ __device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs
     LaneTw<T, J, E> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
-    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
     const int row0 = 2 * bi.tile() * P;
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
@@ -467,7 +498,7 @@ __device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs
         T* rho = a.rho[ax];
         const T* rha = rho + oa;
         const T* rhb = rho + ob;
-        S::load_spectrum(a.D[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        S::template load_spectrum<FULL>(a.D[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
         cx<T> rpre[PRE ? E : 1];
         if constexpr (PRE) {
 #pragma unroll
@@ -560,20 +591,28 @@ __device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs
             if (vb) a.p[ob + J * m] = pB;
             v[m] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
         }
-        S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+        S::template forward_store<FULL>(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
     } else {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cx<T>{va ? sd[m].x : T(0), vb ? sd[m].y : T(0)};
-        S::forward_store(a.F1, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+        S::template forward_store<FULL>(a.F1, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             if (va) a.rhoT[oa + J * m] = rt[m].x;
             if (vb) a.rhoT[ob + J * m] = rt[m].y;
             rt[m] = cx<T>{va ? rt[m].x : T(0), vb ? rt[m].y : T(0)};
         }
-        if (a.dispersion) S::forward_store(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, rt);
+        if (a.dispersion) S::template forward_store<FULL>(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, rt);
     }
 }
+template <typename T, int J, int E, bool NL, int MODE, bool CORR>
+__device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P, const AxisTabs<T>& tab) {
+    if (KWH_ROW_FULL_TILES && reg_tile_full(bi, P, g.nrows))
+        reg_z_density_impl<T, J, E, NL, MODE, CORR, true>(bi, a, g, P, tab);
+    else
+        reg_z_density_impl<T, J, E, NL, MODE, CORR, false>(bi, a, g, P, tab);
+}
+
 template <typename T, int J, int E, bool NL, int MODE, bool CORR>
 __global__ void __launch_bounds__(256, (density_waves<T, J, E, NL, MODE, CORR>()))
     reg_z_density(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
@@ -586,8 +625,8 @@ template <typename T, int E>
 constexpr int eos_waves() {
     return reg_waves<T, E>(KWH_EOS_PRELOAD && E <= 16 && sizeof(T) == 4 ? KWH_EOS_PRELOAD_WAVES : KWH_EOS_WAVES);
 }
-template <typename T, int J, int E>
-__device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>& a, const Geo& g, int P,
+template <typename T, int J, int E, bool FULL>
+__device__ __forceinline__ void reg_z_eos_impl(const BIdx& bi, const EosArgs<T>& a, const Geo& g, int P,
                                                const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -596,14 +635,14 @@ __device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>&
     LaneTw<T, J, E> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = ldsb + S::L + ps.w * S::LP;
-    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
     cx<T> e[E], v[E];
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
     const T* cpa = a.c0.lane(oa, ps.j);
     const T* cpb = a.c0.lane(ob, ps.j);
-    S::load_spectrum(a.A1, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, e);
-    if (a.A2) S::load_spectrum(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+    S::template load_spectrum<FULL>(a.A1, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, e);
+    if (a.A2) S::template load_spectrum<FULL>(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
     constexpr bool PRE = KWH_EOS_PRELOAD && E <= 16 && sizeof(T) == 4;
     cx<T> rtp[PRE ? E : 1], cp[PRE ? E : 1];   // sum rho and c0 of the row pair, loaded with the spectra
     if constexpr (PRE) {
@@ -624,45 +663,115 @@ __device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>&
     const T* bpb = a.bona.lane(ob, ps.j);
     const T* rpa = a.rho0.lane(oa, ps.j);
     const T* rpb = a.rho0.lane(ob, ps.j);
+    // The element loop, instantiated per combination of the three run-time switches (uniform over
+    // the launch) so that its body carries no per-element branch.
+    auto elements = [&](auto from_c, auto has_a2, auto nl) {
+        constexpr bool FROMC = decltype(from_c)::value, A2 = decltype(has_a2)::value, NLIN = decltype(nl)::value;
 #pragma unroll
-    for (int g0 = 0; g0 < E; g0 += kGroup) {
+        for (int g0 = 0; g0 < E; g0 += kGroup) {
 #pragma unroll
-        for (int m = g0; m < g0 + kGroup && m < E; ++m) {
-            const int64_t ca = oa + J * m, cb = ob + J * m;
-            const T rta = PRE ? rtp[m].x : a.rhoT[ca], rtb = PRE ? rtp[m].y : a.rhoT[cb];
-            const T cA = PRE ? cp[m].x : cpa[J * m], cB = PRE ? cp[m].y : cpb[J * m];
-            T tauA, etaA, tauB, etaB;
-            if (a.tau_eta_from_c) {
-                const T la = log2(cA), lb = log2(cB);
-                tauA = a.tk * exp2(a.te * la);
-                tauB = a.tk * exp2(a.te * lb);
-                etaA = a.ek * exp2(a.ee * la);
-                etaB = a.ek * exp2(a.ee * lb);
-            } else {
-                tauA = tpa[J * m];
-                tauB = tpb[J * m];
-                etaA = epa[J * m];
-                etaB = epb[J * m];
+            for (int m = g0; m < g0 + kGroup && m < E; ++m) {
+                const int64_t ca = oa + J * m, cb = ob + J * m;
+                const T rta = PRE ? rtp[m].x : a.rhoT[ca], rtb = PRE ? rtp[m].y : a.rhoT[cb];
+                const T cA = PRE ? cp[m].x : cpa[J * m], cB = PRE ? cp[m].y : cpb[J * m];
+                T tauA, etaA = T(0), tauB, etaB = T(0);
+                if constexpr (FROMC) {
+                    const T la = log2(cA), lb = log2(cB);
+                    tauA = a.tk * exp2(a.te * la);
+                    tauB = a.tk * exp2(a.te * lb);
+                    if constexpr (A2) {
+                        etaA = a.ek * exp2(a.ee * la);
+                        etaB = a.ek * exp2(a.ee * lb);
+                    }
+                } else {
+                    tauA = tpa[J * m];
+                    tauB = tpb[J * m];
+                    if constexpr (A2) {
+                        etaA = epa[J * m];
+                        etaB = epb[J * m];
+                    }
+                }
+                T ea = rta + tauA * e[m].x;
+                T eb = rtb + tauB * e[m].y;
+                if constexpr (A2) {
+                    ea -= etaA * v[m].x;
+                    eb -= etaB * v[m].y;
+                }
+                if constexpr (NLIN) {
+                    ea += bpa[J * m] * (rta * rta) / (T(2) * rpa[J * m]);
+                    eb += bpb[J * m] * (rtb * rtb) / (T(2) * rpb[J * m]);
+                }
+                const T pA = cA * cA * ea, pB = cB * cB * eb;
+                if (va) a.p[ca] = pA;
+                if (vb) a.p[cb] = pB;
+                e[m] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
             }
-            T ea = rta + tauA * e[m].x;
-            T eb = rtb + tauB * e[m].y;
-            if (a.A2) {
-                ea -= etaA * v[m].x;
-                eb -= etaB * v[m].y;
-            }
-            if (a.nonlinear) {
-                ea += bpa[J * m] * (rta * rta) / (T(2) * rpa[J * m]);
-                eb += bpb[J * m] * (rtb * rtb) / (T(2) * rpb[J * m]);
-            }
-            const T pA = cA * cA * ea, pB = cB * cB * eb;
-            if (va) a.p[ca] = pA;
-            if (vb) a.p[cb] = pB;
-            e[m] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
+            KWH_SCHED_FENCE();
         }
-        KWH_SCHED_FENCE();
+    };
+    using Y = std::true_type;
+    using N = std::false_type;
+#if KWH_EOS_VERSIONED
+    const bool has_a2 = a.A2 != nullptr;
+    if (a.tau_eta_from_c) {
+        if (has_a2) { if (a.nonlinear) elements(Y{}, Y{}, Y{}); else elements(Y{}, Y{}, N{}); }
+        else { if (a.nonlinear) elements(Y{}, N{}, Y{}); else elements(Y{}, N{}, N{}); }
+    } else {
+        if (has_a2) { if (a.nonlinear) elements(N{}, Y{}, Y{}); else elements(N{}, Y{}, N{}); }
+        else { if (a.nonlinear) elements(N{}, N{}, Y{}); else elements(N{}, N{}, N{}); }
     }
-    S::forward_store(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, e);
+#else
+    (void)elements;
+    {
+#pragma unroll
+        for (int g0 = 0; g0 < E; g0 += kGroup) {
+#pragma unroll
+            for (int m = g0; m < g0 + kGroup && m < E; ++m) {
+                const int64_t ca = oa + J * m, cb = ob + J * m;
+                const T rta = PRE ? rtp[m].x : a.rhoT[ca], rtb = PRE ? rtp[m].y : a.rhoT[cb];
+                const T cA = PRE ? cp[m].x : cpa[J * m], cB = PRE ? cp[m].y : cpb[J * m];
+                T tauA, etaA, tauB, etaB;
+                if (a.tau_eta_from_c) {
+                    const T la = log2(cA), lb = log2(cB);
+                    tauA = a.tk * exp2(a.te * la);
+                    tauB = a.tk * exp2(a.te * lb);
+                    etaA = a.ek * exp2(a.ee * la);
+                    etaB = a.ek * exp2(a.ee * lb);
+                } else {
+                    tauA = tpa[J * m];
+                    tauB = tpb[J * m];
+                    etaA = epa[J * m];
+                    etaB = epb[J * m];
+                }
+                T ea = rta + tauA * e[m].x;
+                T eb = rtb + tauB * e[m].y;
+                if (a.A2) {
+                    ea -= etaA * v[m].x;
+                    eb -= etaB * v[m].y;
+                }
+                if (a.nonlinear) {
+                    ea += bpa[J * m] * (rta * rta) / (T(2) * rpa[J * m]);
+                    eb += bpb[J * m] * (rtb * rtb) / (T(2) * rpb[J * m]);
+                }
+                const T pA = cA * cA * ea, pB = cB * cB * eb;
+                if (va) a.p[ca] = pA;
+                if (vb) a.p[cb] = pB;
+                e[m] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
+            }
+            KWH_SCHED_FENCE();
+        }
+    }
+#endif
+    S::template forward_store<FULL>(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, e);
 }
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>& a, const Geo& g, int P, const AxisTabs<T>& tab) {
+    if (KWH_ROW_FULL_TILES && reg_tile_full(bi, P, g.nrows))
+        reg_z_eos_impl<T, J, E, true>(bi, a, g, P, tab);
+    else
+        reg_z_eos_impl<T, J, E, false>(bi, a, g, P, tab);
+}
+
 template <typename T, int J, int E>
 __global__ void __launch_bounds__(256, (eos_waves<T, E>()))
     reg_z_eos(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
@@ -896,12 +1005,36 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
             const cx<T> fixed_vec = use_vec && !vec_on_mid ? tab1[fixed_idx] : cx<T>{T(1), T(0)};
             const cx<T>* tj = tab1 + j;
             cx<T> v[E];
+#if KWH_MIDDLE_VERSIONED
+            // The operator's kind is uniform over the workgroup: choose the element loop once, so
+            // the loop body is straight-line (its table loads issue together, no per-element branch).
+            if (use_vec && vec_on_mid) {
+                cx<T> mv[E];
+#pragma unroll
+                for (int m = 0; m < E; ++m) mv[m] = tj[J * m];
+                if (use_kappa) {
+#pragma unroll
+                    for (int m = 0; m < E; ++m) v[m] = cmul(X[m], mv[m] * kap[m]);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < E; ++m) v[m] = cmul(X[m], mv[m] * scale);
+                }
+            } else if (use_kappa) {
+#pragma unroll
+                for (int m = 0; m < E; ++m) v[m] = cmul(X[m], fixed_vec * kap[m]);
+            } else {
+                const cx<T> f = fixed_vec * scale;
+#pragma unroll
+                for (int m = 0; m < E; ++m) v[m] = cmul(X[m], f);
+            }
+#else
 #pragma unroll
             for (int m = 0; m < E; ++m) {
                 cx<T> mv = use_vec ? (vec_on_mid ? tj[J * m] : fixed_vec) : cx<T>{T(1), T(0)};
                 const T s = use_kappa ? kap[m] : scale;
                 v[m] = cmul(X[m], cx<T>{mv.x * s, mv.y * s});
             }
+#endif
             S::template transform<true>(v, lds, j, w, lt);
             if (valid) {
                 cx<T>* q = reinterpret_cast<cx<T>*>(op.out) + base;
