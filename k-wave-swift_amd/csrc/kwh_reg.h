@@ -779,9 +779,10 @@ __global__ void __launch_bounds__(256, (eos_waves<T, E>()))
 }
 
 // Adjoint forward row kernel (AdjRowArgs, kwh_kernels.h): the diagonal update of the reverse sweep
-// computed on the row pair in registers, then R2C_z into a.spec.
-template <typename T, int J, int E, int MODE>
-__device__ __forceinline__ void reg_adj_fwd_body(const BIdx& bi, const AdjRowArgs<T>& a, const Geo& g, int P,
+// computed on the row pair in registers, then R2C_z into a.spec. Instantiated per launch-uniform
+// case (PML along the row or across it, s or s^2, full tile), so no element carries a branch.
+template <typename T, int J, int E, int MODE, bool FULL, bool ZAX, bool SQ>
+__device__ __forceinline__ void reg_adj_fwd_impl(const BIdx& bi, const AdjRowArgs<T>& a, const Geo& g, int P,
                                                  const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -789,21 +790,20 @@ __device__ __forceinline__ void reg_adj_fwd_body(const BIdx& bi, const AdjRowArg
     RegZPos<J> ps(P, bi);
     LaneTw<T, J, E> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
-    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
     const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
-    const bool zax = a.ia == 2;
     const T* sz = a.s + ps.j;
-    const T sa0 = zax ? T(1) : a.s[axis_pos(g, ra, 0, a.ia)];
-    const T sb0 = zax ? T(1) : a.s[axis_pos(g, rb, 0, a.ia)];
+    const T sa0 = ZAX ? T(1) : a.s[axis_pos(g, ra, 0, a.ia)];
+    const T sb0 = ZAX ? T(1) : a.s[axis_pos(g, rb, 0, a.ia)];
     cx<T> v[E];
     if constexpr (MODE == 0) {
         const T* rpa = a.r.lane(oa, ps.j);
         const T* rpb = a.r.lane(ob, ps.j);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-            const T qa = zax ? sz[J * m] : sa0, qb = zax ? sz[J * m] : sb0;
-            const T fa = a.spow == 2 ? qa * qa : qa, fb = a.spow == 2 ? qb * qb : qb;
+            const T qa = ZAX ? sz[J * m] : sa0, qb = ZAX ? sz[J * m] : sb0;
+            const T fa = SQ ? qa * qa : qa, fb = SQ ? qb * qb : qb;
             const T xa = a.scale * rpa[J * m] * fa * a.lu[oa + J * m];
             const T xb = a.scale * rpb[J * m] * fb * a.lu[ob + J * m];
             v[m] = cx<T>{va ? xa : T(0), vb ? xb : T(0)};
@@ -815,7 +815,7 @@ __device__ __forceinline__ void reg_adj_fwd_body(const BIdx& bi, const AdjRowArg
         const T* mpb = a.rho0.lane(ob, ps.j);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-            const T qa = zax ? sz[J * m] : sa0, qb = zax ? sz[J * m] : sb0;
+            const T qa = ZAX ? sz[J * m] : sa0, qb = ZAX ? sz[J * m] : sb0;
             const T ca = cpa[J * m], cb = cpb[J * m];
             const T la = ca * ca * a.lp[oa + J * m] + qa * qa * a.lr[oa + J * m];
             const T lb = cb * cb * a.lp[ob + J * m] + qb * qb * a.lr[ob + J * m];
@@ -824,7 +824,25 @@ __device__ __forceinline__ void reg_adj_fwd_body(const BIdx& bi, const AdjRowArg
             v[m] = cx<T>{va ? a.dt * mpa[J * m] * qa * la : T(0), vb ? a.dt * mpb[J * m] * qb * lb : T(0)};
         }
     }
-    S::forward_store(a.spec, ps.ra, ps.rb, g.nrows, g.mzh, ldsb + S::L + ps.w * S::LP, ps.j, lt, v);
+    S::template forward_store<FULL>(a.spec, ps.ra, ps.rb, g.nrows, g.mzh, ldsb + S::L + ps.w * S::LP, ps.j, lt, v);
+}
+template <typename T, int J, int E, int MODE>
+__device__ __forceinline__ void reg_adj_fwd_body(const BIdx& bi, const AdjRowArgs<T>& a, const Geo& g, int P,
+                                                 const AxisTabs<T>& tab) {
+    const bool full = KWH_ROW_FULL_TILES && reg_tile_full(bi, P, g.nrows);
+    const bool zax = a.ia == 2, sq = MODE == 0 && a.spow == 2;
+    auto go = [&](auto F, auto Z, auto Q) {
+        reg_adj_fwd_impl<T, J, E, MODE, decltype(F)::value, decltype(Z)::value, decltype(Q)::value>(bi, a, g, P, tab);
+    };
+    using Y = std::true_type;
+    using N = std::false_type;
+    if (full) {
+        if (zax) { if (sq) go(Y{}, Y{}, Y{}); else go(Y{}, Y{}, N{}); }
+        else { if (sq) go(Y{}, N{}, Y{}); else go(Y{}, N{}, N{}); }
+    } else {
+        if (zax) { if (sq) go(N{}, Y{}, Y{}); else go(N{}, Y{}, N{}); }
+        else { if (sq) go(N{}, N{}, Y{}); else go(N{}, N{}, N{}); }
+    }
 }
 template <typename T, int J, int E, int MODE>
 __global__ void __launch_bounds__(256) reg_adj_fwd(AdjRowArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
@@ -832,10 +850,10 @@ __global__ void __launch_bounds__(256) reg_adj_fwd(AdjRowArgs<T> a, Geo g, int P
 }
 
 // Adjoint inverse row kernel: C2R_z(spec) stored (mode 0), accumulated (1) or added to the
-// decayed field s^2 dst (2).
-template <typename T, int J, int E>
-__device__ __forceinline__ void reg_adj_inv_body(const BIdx& bi, const cx<T>* spec, T* dst, const T* s, int ia,
-                                                 int mode, const Geo& g, int P, const AxisTabs<T>& tab) {
+// decayed field s^2 dst (2). Instantiated per mode, PML orientation and full tile.
+template <typename T, int J, int E, int MODE, bool FULL, bool ZAX>
+__device__ __forceinline__ void reg_adj_inv_impl(const BIdx& bi, const cx<T>* spec, T* dst, const T* s, int ia,
+                                                 const Geo& g, int P, const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
@@ -843,22 +861,48 @@ __device__ __forceinline__ void reg_adj_inv_body(const BIdx& bi, const cx<T>* sp
     LaneTw<T, J, E> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     cx<T> v[E];
-    S::load_spectrum(spec, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+    S::template load_spectrum<FULL>(spec, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
     S::template transform<true>(v, ldsb + S::L + ps.w * S::LP, ps.j, lt);
-    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
     const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
-    const bool zax = ia == 2;
-    const T sa0 = mode == 2 && !zax ? s[axis_pos(g, ra, 0, ia)] : T(1);
-    const T sb0 = mode == 2 && !zax ? s[axis_pos(g, rb, 0, ia)] : T(1);
-    const T* sz = mode == 2 && zax ? s + ps.j : nullptr;
+    const T sa0 = MODE == 2 && !ZAX ? s[axis_pos(g, ra, 0, ia)] : T(1);
+    const T sb0 = MODE == 2 && !ZAX ? s[axis_pos(g, rb, 0, ia)] : T(1);
+    const T* sz = s + ps.j;
     T* da = dst + (int64_t)ra * S::L + ps.j;
     T* db = dst + (int64_t)rb * S::L + ps.j;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-        const T qa = sz ? sz[J * m] : sa0, qb = sz ? sz[J * m] : sb0;
-        if (va) da[J * m] = mode == 0 ? v[m].x : (mode == 1 ? da[J * m] + v[m].x : da[J * m] * (qa * qa) + v[m].x);
-        if (vb) db[J * m] = mode == 0 ? v[m].y : (mode == 1 ? db[J * m] + v[m].y : db[J * m] * (qb * qb) + v[m].y);
+        if constexpr (MODE == 0) {
+            if (va) da[J * m] = v[m].x;
+            if (vb) db[J * m] = v[m].y;
+        } else if constexpr (MODE == 1) {
+            if (va) da[J * m] = da[J * m] + v[m].x;
+            if (vb) db[J * m] = db[J * m] + v[m].y;
+        } else {
+            const T qa = ZAX ? sz[J * m] : sa0, qb = ZAX ? sz[J * m] : sb0;
+            if (va) da[J * m] = da[J * m] * (qa * qa) + v[m].x;
+            if (vb) db[J * m] = db[J * m] * (qb * qb) + v[m].y;
+        }
     }
+}
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_adj_inv_body(const BIdx& bi, const cx<T>* spec, T* dst, const T* s, int ia,
+                                                 int mode, const Geo& g, int P, const AxisTabs<T>& tab) {
+    const bool full = KWH_ROW_FULL_TILES && reg_tile_full(bi, P, g.nrows);
+    const bool zax = ia == 2;
+    auto go = [&](auto M, auto F, auto Z) {
+        reg_adj_inv_impl<T, J, E, decltype(M)::value, decltype(F)::value, decltype(Z)::value>(bi, spec, dst, s, ia, g,
+                                                                                              P, tab);
+    };
+    using Y = std::true_type;
+    using N = std::false_type;
+    using M0 = std::integral_constant<int, 0>;
+    using M1 = std::integral_constant<int, 1>;
+    using M2 = std::integral_constant<int, 2>;
+    if (mode == 0) { if (full) go(M0{}, Y{}, N{}); else go(M0{}, N{}, N{}); }
+    else if (mode == 1) { if (full) go(M1{}, Y{}, N{}); else go(M1{}, N{}, N{}); }
+    else if (full) { if (zax) go(M2{}, Y{}, Y{}); else go(M2{}, Y{}, N{}); }
+    else { if (zax) go(M2{}, N{}, Y{}); else go(M2{}, N{}, N{}); }
 }
 template <typename T, int J, int E>
 __global__ void __launch_bounds__(256) reg_adj_inv(const cx<T>* spec, T* dst, const T* s, int ia, int mode, Geo g,
