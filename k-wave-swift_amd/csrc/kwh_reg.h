@@ -240,8 +240,8 @@ __device__ void reg_apply_sparse(const SparseSrc<T>& s, const Geo& g, int row0, 
     __syncthreads();
 }
 
-template <typename T, int J, int E>
-__device__ __forceinline__ void reg_z_forward_real_body(const BIdx& bi, const T* __restrict__ src, cx<T>* spec,
+template <typename T, int J, int E, bool FULL>
+__device__ __forceinline__ void reg_z_forward_real_impl(const BIdx& bi, const T* __restrict__ src, cx<T>* spec,
                                                         T scale, const Geo& g, int P, const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -250,7 +250,7 @@ __device__ __forceinline__ void reg_z_forward_real_body(const BIdx& bi, const T*
     RegZPos<J> ps(P, bi);
     LaneTw<T, J, E> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
-    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
     cx<T> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -258,7 +258,15 @@ __device__ __forceinline__ void reg_z_forward_real_body(const BIdx& bi, const T*
         v[m].x = va ? src[(int64_t)ps.ra * S::L + n] * scale : T(0);
         v[m].y = vb ? src[(int64_t)ps.rb * S::L + n] * scale : T(0);
     }
-    S::forward_store(spec, ps.ra, ps.rb, g.nrows, g.mzh, lds + ps.w * S::LP, ps.j, lt, v);
+    S::template forward_store<FULL>(spec, ps.ra, ps.rb, g.nrows, g.mzh, lds + ps.w * S::LP, ps.j, lt, v);
+}
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_z_forward_real_body(const BIdx& bi, const T* __restrict__ src, cx<T>* spec,
+                                                        T scale, const Geo& g, int P, const AxisTabs<T>& tab) {
+    if (KWH_ROW_FULL_TILES && reg_tile_full(bi, P, g.nrows))
+        reg_z_forward_real_impl<T, J, E, true>(bi, src, spec, scale, g, P, tab);
+    else
+        reg_z_forward_real_impl<T, J, E, false>(bi, src, spec, scale, g, P, tab);
 }
 template <typename T, int J, int E>
 __global__ void __launch_bounds__(256) reg_z_forward_real(const T* __restrict__ src, cx<T>* spec, T scale, Geo g,
@@ -266,10 +274,9 @@ __global__ void __launch_bounds__(256) reg_z_forward_real(const T* __restrict__ 
     reg_z_forward_real_body<T, J, E>(bidx_here(), src, spec, scale, g, P, tab);
 }
 
-template <typename T, int J, int E>
-__device__ __forceinline__ void reg_z_inverse_real_body(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst,
-                                                        T scale, int accumulate, const Geo& g, int P,
-                                                        const AxisTabs<T>& tab) {
+template <typename T, int J, int E, bool FULL, bool ACC>
+__device__ __forceinline__ void reg_z_inverse_real_impl(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst,
+                                                        T scale, const Geo& g, int P, const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
@@ -278,20 +285,33 @@ __device__ __forceinline__ void reg_z_inverse_real_body(const BIdx& bi, const cx
     LaneTw<T, J, E> lt;
     lt.load(ldsb, tab.tw[2], ps.j);
     cx<T> v[E];
-    S::load_spectrum(spec, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+    S::template load_spectrum<FULL>(spec, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
     S::template transform<true>(v, lds + ps.w * S::LP, ps.j, lt);
-    const bool va = ps.ra < g.nrows, vb = ps.rb < g.nrows;
+    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int n = ps.j + J * m;
         if (va) {
             const int64_t c = (int64_t)ps.ra * S::L + n;
-            dst[c] = accumulate ? dst[c] + v[m].x * scale : v[m].x * scale;
+            dst[c] = ACC ? dst[c] + v[m].x * scale : v[m].x * scale;
         }
         if (vb) {
             const int64_t c = (int64_t)ps.rb * S::L + n;
-            dst[c] = accumulate ? dst[c] + v[m].y * scale : v[m].y * scale;
+            dst[c] = ACC ? dst[c] + v[m].y * scale : v[m].y * scale;
         }
+    }
+}
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_z_inverse_real_body(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst,
+                                                        T scale, int accumulate, const Geo& g, int P,
+                                                        const AxisTabs<T>& tab) {
+    const bool full = KWH_ROW_FULL_TILES && reg_tile_full(bi, P, g.nrows);
+    if (accumulate) {
+        if (full) reg_z_inverse_real_impl<T, J, E, true, true>(bi, spec, dst, scale, g, P, tab);
+        else reg_z_inverse_real_impl<T, J, E, false, true>(bi, spec, dst, scale, g, P, tab);
+    } else {
+        if (full) reg_z_inverse_real_impl<T, J, E, true, false>(bi, spec, dst, scale, g, P, tab);
+        else reg_z_inverse_real_impl<T, J, E, false, false>(bi, spec, dst, scale, g, P, tab);
     }
 }
 template <typename T, int J, int E>
