@@ -259,9 +259,13 @@ int kwh_time_reversal(kwh_ctx* ctx, const kwh_sim_desc* desc, const float* recor
 
 /* Autotuner (no counterpart in the reference): before the first step, times one full time step
  * under each launch-schedule candidate (tile columns of the strided / middle passes, row pairs
- * per workgroup of the row kernels, LDS budget of the generic FFT passes) on this sim's buffers,
- * keeps the fastest and caches it per grid shape and precision for the process. report (may be
- * NULL) receives the candidates' step times. The state is reset afterwards. */
+ * per workgroup of the row kernels, LDS budget of the generic FFT passes) on this sim's buffers
+ * and keeps the fastest. The choice is cached per device (gfx architecture and CU count), grid
+ * shape and precision: in memory for the process and in a text file (KWH_AUTOTUNE_CACHE, a path
+ * or "off"; default $XDG_CACHE_HOME/kwavehip/autotune.txt or $HOME/.cache/kwavehip/autotune.txt),
+ * so later sims and later processes on the same kind of GPU take it without timing. Schedules
+ * change launch geometry only: results are bitwise those of the default schedule. report (may be
+ * NULL) receives the candidates' step times or the cached entry. The state is reset afterwards. */
 int kwh_sim_autotune(kwh_sim* sim, char* report, size_t report_bytes);
 
 /* Adjoint (reverse-time) gradient; no counterpart in the reference.

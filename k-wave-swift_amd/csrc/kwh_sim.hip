@@ -16,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <sys/stat.h>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -153,6 +154,65 @@ constexpr int kMaxDevices = 64;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 std::mutex g_scratch_mu;
 void* g_scratch_ptr[kMaxDevices] = {};
+
+// Launch-schedule cache of the autotuner, keyed by device and problem (grid shape, precision,
+// register-resident axes): in memory for the process and persisted as text, so a schedule is
+// timed once per machine. File: KWH_AUTOTUNE_CACHE (a path, or "off"), else
+// $XDG_CACHE_HOME/kwavehip/autotune.txt, else $HOME/.cache/kwavehip/autotune.txt. One line per
+// entry: "<device> <problem> <ntr_shift> <p_shift> <lds bytes> <min_blocks>"; the device is the
+// gfx architecture name and compute-unit count, since schedules do not carry across GPUs.
+struct TunedSchedule {
+    int ntr_shift = 0, p_shift = 0;
+    size_t lds = 64 * 1024;
+    int min_blocks = 1024;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TunedSchedule> g_tune;
+std::map<std::string, bool> g_tune_files_read;
+
+std::string tune_cache_path() {
+    if (const char* e = std::getenv("KWH_AUTOTUNE_CACHE")) return std::strcmp(e, "off") == 0 ? std::string() : e;
+    if (const char* x = std::getenv("XDG_CACHE_HOME")) return std::string(x) + "/kwavehip/autotune.txt";
+    if (const char* h = std::getenv("HOME")) return std::string(h) + "/.cache/kwavehip/autotune.txt";
+    return std::string();
+}
+
+// with g_tune_mu held: merge the file's entries into the in-memory cache (once per path)
+void tune_read(const std::string& path) {
+    if (path.empty() || g_tune_files_read[path]) return;
+    g_tune_files_read[path] = true;
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    char dev[128], prob[128];
+    TunedSchedule t;
+    unsigned long lds = 0;
+    while (std::fscanf(f, "%127s %127s %d %d %lu %d", dev, prob, &t.ntr_shift, &t.p_shift, &lds, &t.min_blocks) == 6) {
+        t.lds = (size_t)lds;
+        g_tune[std::string(dev) + " " + prob] = t;
+    }
+    std::fclose(f);
+}
+
+// with g_tune_mu held: append one entry (directories created as needed); false if not written
+bool tune_write(const std::string& path, const std::string& key, const TunedSchedule& t) {
+    if (path.empty()) return false;
+    for (size_t i = 1; i < path.size(); ++i)
+        if (path[i] == '/') (void)mkdir(path.substr(0, i).c_str(), 0755);
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) return false;
+    std::fprintf(f, "%s %d %d %lu %d\n", key.c_str(), t.ntr_shift, t.p_shift, (unsigned long)t.lds, t.min_blocks);
+    std::fclose(f);
+    return true;
+}
+
+std::string device_tag(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+    std::string arch(prop.gcnArchName);
+    for (char& c : arch)
+        if (c == ' ' || c == ':') c = '_';
+    return arch + "-" + std::to_string(prop.multiProcessorCount) + "cu";
+}
 
 }  // namespace
 
@@ -1969,18 +2029,27 @@ struct SimT final : kwh_sim {
         KWH_REQUIRE(!axisym, KWH_EUNSUPPORTED, "the autotuner covers kspaceFirstOrder");
         HIPCHK(hipSetDevice(dev));
         sync();
-        static std::vector<std::pair<std::string, Schedule>> cache;
         char keyb[96];
         std::snprintf(keyb, sizeof(keyb), "%d:%d:%d:%d:%zu:%d%d%d", dim, g.m0, g.m1, g.m2, sizeof(T), rx[0] ? 1 : 0,
                       rx[1] ? 1 : 0, rx[2] ? 1 : 0);
-        const std::string key(keyb);
+        const std::string key = device_tag(dev) + " " + keyb;
+        const std::string path = tune_cache_path();
         std::string report;
-        for (auto& e : cache)
-            if (e.first == key) {
-                sched = e.second;
-                report = "cached schedule for " + key + "\n";
-                return report;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mu);
+            tune_read(path);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) {
+                sched.ntr_shift = it->second.ntr_shift;
+                sched.p_shift = it->second.p_shift;
+                sched.lds = it->second.lds;
+                sched.min_blocks = it->second.min_blocks;
+                char line[200];
+                std::snprintf(line, sizeof(line), "cached schedule for %s: ntr_shift=%d p_shift=%d lds=%zu min_blocks=%d\n",
+                              key.c_str(), sched.ntr_shift, sched.p_shift, sched.lds, sched.min_blocks);
+                return line;
             }
+        }
         const bool reg_cols = sizeof(T) == 4 && (rx[mid_axis] || (plain1 && rx[1]));
         bool generic = false;
         const int m[3] = {g.m0, g.m1, g.m2};
@@ -2034,10 +2103,21 @@ struct SimT final : kwh_sim {
         sched = chosen;
         reset_state();
         sync();
-        cache.emplace_back(key, chosen);
-        char line[160];
-        std::snprintf(line, sizeof(line), "chosen: ntr_shift=%d p_shift=%d lds=%zu min_blocks=%d\n", chosen.ntr_shift,
-                      chosen.p_shift, chosen.lds, chosen.min_blocks);
+        bool persisted = false;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mu);
+            TunedSchedule t;
+            t.ntr_shift = chosen.ntr_shift;
+            t.p_shift = chosen.p_shift;
+            t.lds = chosen.lds;
+            t.min_blocks = chosen.min_blocks;
+            g_tune[key] = t;
+            persisted = tune_write(path, key, t);
+        }
+        char line[256];
+        std::snprintf(line, sizeof(line), "chosen: ntr_shift=%d p_shift=%d lds=%zu min_blocks=%d (%s)\n", chosen.ntr_shift,
+                      chosen.p_shift, chosen.lds, chosen.min_blocks,
+                      persisted ? ("cached in " + path).c_str() : "cached for this process");
         return report + line;
     }
 
