@@ -112,47 +112,59 @@ def scan_line_cases(n_runs=48, nt=800, seed=7):
 def batch_comparison(n_runs=48, nt=800):
     """Batched runs against the same runs launched one after another (kwh_sim_run each), both
     timed from device-resident inputs to combined traces. Batched modes: fused (one launch per
-    pass of the step over all runs, the batched step replayed from a hipGraph), graphs (each run
-    replays its own step graph on its own stream), streams (each run enqueued on its own stream)."""
+    pass of the step over all runs, the batched step replayed from a hipGraph), fused_plane (the
+    same with the y-z plane kernels, kwh_sim_set_plane), graphs (each run replays its own step graph
+    on its own stream), streams (each run enqueued on its own stream). Sequential baselines: plain
+    (default launch schedule, split passes), plain_autotuned, and plain_plane (each run alone in
+    plane mode, the per-run effect of the plane kernels without batching)."""
     from kwavehip import Simulation, SimulationBatch
     cases = scan_line_cases(n_runs, nt)
     cells = sum(c[0][0].totalGridPoints for c in cases) * nt
-    plain_s = 0.0
-    ref = []
-    for args, tr in cases:
-        sim = Simulation(*args)
-        t0 = time.perf_counter()
-        sim.run()
-        traces = sim.transducer_traces(tr)
-        plain_s += time.perf_counter() - t0
-        ref.append(traces)
-        sim.close()
+
+    def sequential(plane=False, tune=False):
+        total, traces = 0.0, []
+        for args, tr in cases:
+            sim = Simulation(*args)
+            if tune:
+                sim.autotune()   # tuned once per grid shape, outside the timing
+            if plane:
+                sim.set_plane(True)
+            t0 = time.perf_counter()
+            sim.run()
+            traces.append(sim.transducer_traces(tr))
+            total += time.perf_counter() - t0
+            sim.close()
+        return total, traces
+
+    def rel_l2(a, b):
+        d = np.asarray(a, np.float64) - b
+        return float(np.sqrt(np.sum(d * d) / max(np.sum(np.asarray(b, np.float64) ** 2), 1e-300)))
+
+    plain_s, ref = sequential()
     res = {"runs": n_runs, "steps": nt, "cells_total": int(cells // nt),
            "plain_Mcell_steps_per_s": round(cells / plain_s / 1e6, 1)}
-    # the same sequential runs with autotuned launch schedules (tuned once per grid shape, outside
-    # the timing; a small grid gains from smaller tiles that a batch does not need)
-    tuned_s = 0.0
-    for args, tr in cases:
-        sim = Simulation(*args)
-        sim.autotune()
-        t0 = time.perf_counter()
-        sim.run()
-        sim.transducer_traces(tr)
-        tuned_s += time.perf_counter() - t0
-        sim.close()
+    tuned_s, _ = sequential(tune=True)
     res["plain_autotuned_Mcell_steps_per_s"] = round(cells / tuned_s / 1e6, 1)
-    for mode, use_graph, fused in (("fused", True, True), ("graphs", True, False), ("streams", False, False)):
+    plane_seq_s, ref_plane = sequential(plane=True)
+    res["plain_plane_Mcell_steps_per_s"] = round(cells / plane_seq_s / 1e6, 1)
+    for mode, use_graph, fused, plane in (("fused", True, True, False), ("fused_plane", True, True, True),
+                                          ("graphs", True, False, False), ("streams", False, False, False)):
         batch = SimulationBatch([c[0] for c in cases])
         t0 = time.perf_counter()
-        batch.run(use_graph=use_graph, fused=fused)
+        batch.run(use_graph=use_graph, fused=fused, plane=plane)
         traces = [sim.transducer_traces(tr) for sim, (_, tr) in zip(batch.sims, cases)]
         batch_s = time.perf_counter() - t0
-        same = all(np.array_equal(a, b) for a, b in zip(traces, ref))
+        # bitwise against the sequential runs in the same mode; relL2 against the plain runs
+        same = all(np.array_equal(a, b) for a, b in zip(traces, ref_plane if plane else ref))
         batch.close()
         res[f"batched_{mode}_Mcell_steps_per_s"] = round(cells / batch_s / 1e6, 1)
         res[f"speedup_{mode}"] = round(plain_s / batch_s, 3)
         res[f"traces_identical_{mode}"] = bool(same)
-    res["speedup"] = res["speedup_fused"]
+        res[f"traces_relL2_vs_plain_{mode}"] = float(f"{max(rel_l2(a, b) for a, b in zip(traces, ref)):.3e}")
+        if mode == "fused_plane":
+            res["speedup_fused_plane_vs_plain_plane"] = round(plane_seq_s / batch_s, 3)
+            res["speedup_fused_plane_vs_autotuned_plain"] = round(tuned_s / batch_s, 3)
+    res["speedup"] = max(res["speedup_fused"], res["speedup_fused_plane"])
     res["speedup_fused_vs_autotuned_plain"] = round(tuned_s / plain_s * res["speedup_fused"], 3)
     return res
 

@@ -241,6 +241,17 @@ int kwh_sim_set_nonfinite_check(kwh_sim* sim, int32_t every);
  * k-space-corrected sources, captured on first use; results are bitwise those of eager runs. */
 int kwh_sim_set_graph(kwh_sim* sim, int enable);
 
+/* Plane mode (1 = on; no counterpart in the reference, whose MLX transforms are whole-array
+ * calls, KSpaceFirstOrder.swift:647-686): 3-D grids whose (y, z) plane is 96 x 96 or 128 x 128 run
+ * the y and z transforms of each field in one kernel per x plane (the plane held in LDS), so a
+ * step reads and writes each field once for the y and z axes together instead of once per axis.
+ * Covers lossless and Stokes media, every source kind and pressure recording; returns
+ * KWH_EUNSUPPORTED for other configurations (FFT power-law absorption, collocated velocity
+ * output, checkpointing / the adjoint) and other plane shapes. May be switched between steps.
+ * Results agree with the split passes to rounding (the transforms are factored differently);
+ * a sim is deterministic in either mode. */
+int kwh_sim_set_plane(kwh_sim* sim, int enable);
+
 /* Receive combination on the device (combineSensorData, KWaveTransducer.swift:263-289):
  * out[e][t] = scale * sum over sensor points k with element[k] == e of
  * series[k][t + delay[k]] (terms with t + delay[k] beyond the recorded steps are dropped).
@@ -290,11 +301,14 @@ int kwh_sim_vjp(kwh_sim* sim, const void* dl_dp, int32_t checkpoint_every, void*
  *   KWH_BATCH_FUSED    one launch per pass of the step covers every sim whose step has the same
  *                      launch sequence and launch geometry (the sims of one grid shape); the
  *                      shapes of one step run one after another on the batch's stream; with
- *                      KWH_BATCH_GRAPH each shape's batched step is captured once and replayed.
- * Every mode gives results bitwise equal to running the sims one by one. Outputs are read per sim
+ *                      KWH_BATCH_GRAPH each shape's batched step is captured once and replayed;
+ *   KWH_BATCH_PLANE    the sims that plane mode covers run in it (kwh_sim_set_plane); without
+ *                      the flag every sim runs the split passes.
+ * Every mode gives results bitwise equal to running the sims one by one in the same plane mode. Outputs are read per sim
  * (kwh_batch_read_output, or the borrowed handle from kwh_batch_sim with the kwh_sim_* readers). */
 #define KWH_BATCH_GRAPH 1
 #define KWH_BATCH_FUSED 2
+#define KWH_BATCH_PLANE 4
 int kwh_batch_create(kwh_ctx* ctx, const kwh_sim_desc* descs, int32_t n, kwh_batch** out);
 int kwh_batch_run(kwh_batch* batch, int32_t mode);
 int kwh_batch_sim(kwh_batch* batch, int32_t i, kwh_sim** out);

@@ -23,9 +23,11 @@ namespace kwh {
 // A kernel body F with its single-launch wrapper K. W: minimum waves per SIMD of the batched
 // variant (the wrapper's own launch bound). FLEX: the body accepts any workgroup size (loops
 // over blockDim), so a batched launch may run it with the largest size of its simulations.
-template <auto F, auto K, int W = 1, bool FLEX = true>
+// NTH: the largest workgroup size the body is launched with (the batched variant's bound).
+template <auto F, auto K, int W = 1, bool FLEX = true, int NTH = 256>
 struct Body {
     static constexpr int kWaves = W;
+    static constexpr int kThreads = NTH;
     static constexpr bool kFlex = FLEX;
     static constexpr auto kernel = K;
     template <class... A>
@@ -42,7 +44,7 @@ struct BatchPack {
 };
 
 template <class B, class... A>
-__global__ void __launch_bounds__(256, B::kWaves) batched_kernel(const BatchPack<A...>* packs, const int* owner) {
+__global__ void __launch_bounds__(B::kThreads, B::kWaves) batched_kernel(const BatchPack<A...>* packs, const int* owner) {
     using CPack = const __attribute__((address_space(4))) BatchPack<A...>;
     using CInt = const __attribute__((address_space(4))) int;
     const int gb = xcd_tile((int)blockIdx.x, (int)gridDim.x);

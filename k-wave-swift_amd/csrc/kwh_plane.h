@@ -1,0 +1,576 @@
+// kwh_plane.h -- y-z plane kernels: one workgroup transforms a whole (y, z) plane of a 3-D field
+// in LDS, so the strided (y) pass and the contiguous (z) pass of a step are one HBM round trip.
+//
+// The step of the reference (KSpaceFirstOrder.swift:669-692) transforms every field along all
+// three axes. The split passes (kwh_reg.h) read and write each field once per axis: z row
+// kernels, y column kernels and x middle kernels. When a plane [m1][m2/2+1] of a half spectrum
+// fits in LDS (64 KiB at 128 x 128, 36 KiB at 96 x 96 in float32), a plane kernel does the y and
+// z transforms of one x plane together: load the plane (contiguous, coalesced), inverse along y
+// and z, elementwise update, forward along z and y, store. Only the x pass stays separate.
+//
+// Layout in LDS: complex slots pl[r * PITCH + c], r < LY rows, c < M = LZ / 2 columns.
+//  * Spectrum view: slot (ky, kz) for 0 < kz < M; slot (ky, 0) packs the kz = 0 and kz = M
+//    (Nyquist) columns as G0 + i GN. Both columns are transforms along y of real sequences
+//    (the z spectrum of real rows is real at DC and Nyquist), so one complex column transform
+//    carries both, and every row has exactly M complex slots.
+//  * Real view: row r holds its LZ real values x[2m] + i x[2m+1] in slot m, the half-length
+//    complex sequence of the real transform identity (Cooley-Tukey split of the row into even
+//    and odd samples), so the spectrum and the real plane share one buffer.
+// Row transforms are length-M complex DFTs with the even/odd split before (inverse) or after
+// (forward); column transforms are length-LY complex DFTs. Each is a two-factor split L = R1 R2
+// done in place by all threads of the workgroup (stage A: DFT_R2 over n2 per n1, twiddles, back
+// to the same slots; stage B: DFT_R1 over n1 per k2, natural-order output after a barrier).
+// Lanes are mapped transform-fastest: consecutive lanes touch consecutive columns (column
+// stages) or rows PITCH apart (row stages, PITCH odd so rows spread over the LDS banks).
+// All transforms are unnormalised; the x pass applies 1/N, as with the split passes.
+#pragma once
+#include "kwh_kernels.h"
+
+namespace kwh {
+
+// threadIdx.x behind an empty asm statement: the lane-dependent addresses of each phase are then
+// computed where the phase runs instead of once for the whole kernel, where the compiler would
+// keep every phase's addresses live across all of them (and spill).
+__device__ __forceinline__ int plane_tid() {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+// Split L = R1 * R2 of the in-place transforms (n = n1 + R1 n2).
+constexpr int plane_r1(int L) {
+    return L == 128 ? 8 : L == 96 ? 12 : L == 64 ? 8 : L == 48 ? 8 : L == 32 ? 4 : L == 24 ? 4 : L == 16 ? 4 : 0;
+}
+
+template <typename T, int LY, int LZ>
+struct PlaneShape {
+    static constexpr int M = LZ / 2;                  // complex slots per row
+    static constexpr int MZH = M + 1;                 // half-spectrum row length in HBM
+    static constexpr int PITCH = M + 1;               // LDS row pitch (odd: rows land on other banks)
+    static constexpr int TWY = LY * PITCH;            // LDS offset of the y twiddles (LY entries)
+    static constexpr int TWZ = TWY + LY;              // LDS offset of the z twiddles (LZ entries)
+    static constexpr size_t kLds = (size_t)(TWZ + LZ) * sizeof(cx<T>);
+    static_assert(LZ % 4 == 0 && plane_r1(LY) > 0 && plane_r1(M) > 0, "unsupported plane shape");
+};
+
+// Occupancy floor (waves per SIMD) of a plane kernel with NT lanes: as many workgroups per CU as
+// the 160 KiB of LDS hold. Without it the register allocator hoists the element loops' loads
+// across the whole plane share and drops to one wave per SIMD.
+template <typename T, int LY, int LZ, int NT>
+constexpr int plane_waves() {
+    constexpr size_t lds_cu = 160 * 1024;
+    constexpr int wg = (int)(lds_cu / PlaneShape<T, LY, LZ>::kLds);
+    constexpr int w = wg * (NT / 64) / 4;
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+// Twiddle tables W_LY^m and W_LZ^m (forward) into LDS. Ordered before first use by the barrier
+// that follows the first plane load.
+template <typename T, int LY, int LZ, int NT>
+__device__ __forceinline__ void plane_twiddles(cx<T>* lds, const AxisTabs<T>& tab) {
+    using P = PlaneShape<T, LY, LZ>;
+    for (int i = threadIdx.x; i < LY + LZ; i += NT)
+        lds[P::TWY + i] = i < LY ? tab.tw[1][i] : tab.tw[2][i - LY];
+}
+
+// In-place DFTs of NTR transforms of length L = R1 * R2 held in LDS: element n of transform t
+// at buf[t * TS + n * ES]; consecutive lanes take consecutive transforms. tw[e * TWS] = W_L^e
+// (conjugated for the inverse). Ends with a barrier.
+template <typename T, int L, int NTR, int TS, int ES, int NT, bool INV, int TWS>
+__device__ __forceinline__ void plane_dft(cx<T>* buf, const cx<T>* tw) {
+    constexpr int R1 = plane_r1(L), R2 = L / R1;
+    static_assert(R1 > 0 && R1 * R2 == L, "no split for this length");
+    int tid = plane_tid();
+    constexpr int NA = NTR * R1, IA = (NA + NT - 1) / NT;
+#pragma unroll
+    for (int it = 0; it < IA; ++it) {
+        const int q = tid + it * NT;
+        if (NA % NT == 0 || q < NA) {
+            const int t = q % NTR, n1 = q / NTR;
+            cx<T>* b = buf + t * TS + n1 * ES;
+            cx<T> v[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) v[n2] = b[n2 * R1 * ES];
+            dftN<T, R2, INV>(v);
+#pragma unroll
+            for (int k2 = 1; k2 < R2; ++k2) {
+                const cx<T> w = tw[n1 * k2 * TWS];
+                v[k2] = cmul(v[k2], INV ? conjv(w) : w);
+            }
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) b[k2 * R1 * ES] = v[k2];
+        }
+    }
+    __syncthreads();
+    tid = plane_tid();
+    constexpr int NB = NTR * R2, IB = (NB + NT - 1) / NT;
+    cx<T> s[IB][R1];
+#pragma unroll
+    for (int it = 0; it < IB; ++it) {
+        const int q = tid + it * NT;
+        if (NB % NT == 0 || q < NB) {
+            const int t = q % NTR, k2 = q / NTR;
+            const cx<T>* b = buf + t * TS + k2 * R1 * ES;
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) s[it][n1] = b[n1 * ES];
+            dftN<T, R1, INV>(s[it]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IB; ++it) {
+        const int q = tid + it * NT;
+        if (NB % NT == 0 || q < NB) {
+            const int t = q % NTR, k2 = q / NTR;
+            cx<T>* b = buf + t * TS + k2 * ES;
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) b[k1 * R2 * ES] = s[it][k1];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int LY, int LZ, int NT, bool INV>
+__device__ __forceinline__ void plane_cols(cx<T>* pl) {
+    using P = PlaneShape<T, LY, LZ>;
+    plane_dft<T, LY, P::M, 1, P::PITCH, NT, INV, 1>(pl, pl + P::TWY);
+}
+template <typename T, int LY, int LZ, int NT, bool INV>
+__device__ __forceinline__ void plane_rows(cx<T>* pl) {
+    using P = PlaneShape<T, LY, LZ>;
+    plane_dft<T, P::M, LY, P::PITCH, 1, NT, INV, 2>(pl, pl + P::TWZ);   // W_M^e = W_LZ^(2e)
+}
+
+// Inverse real transform of every row, before plane_rows<INV = true>: the packed half spectrum X
+// (slot 0 = X[0] + i X[M]) becomes Z[k] = E2[k] + i O2[k] with E2 = X[k] + conj X[M-k] and
+// O2 = (X[k] - conj X[M-k]) W_LZ^-k, whose length-M inverse DFT is x[2m] + i x[2m+1] (LZ times the
+// normalised inverse). Task (r, k) owns slots k and M - k. Ends with a barrier.
+template <typename T, int LY, int LZ, int NT>
+__device__ __forceinline__ void plane_c2r_split(cx<T>* pl) {
+    using P = PlaneShape<T, LY, LZ>;
+    constexpr int M = P::M, H = M / 2 + 1, NQ = LY * H, IQ = (NQ + NT - 1) / NT;
+    const cx<T>* twz = pl + P::TWZ;
+    const int tid = plane_tid();
+#pragma unroll
+    for (int it = 0; it < IQ; ++it) {
+        const int q = tid + it * NT;
+        if (NQ % NT == 0 || q < NQ) {
+            const int r = q % LY, k = q / LY;
+            cx<T>* row = pl + r * P::PITCH;
+            if (k == 0) {
+                const cx<T> a = row[0];
+                row[0] = cx<T>{a.x + a.y, a.x - a.y};
+            } else {
+                const cx<T> a = row[k], b = row[M - k];
+                const cx<T> e2{a.x + b.x, a.y - b.y};
+                const cx<T> o2 = cmul(cx<T>{a.x - b.x, a.y + b.y}, conjv(twz[k]));
+                row[k] = cx<T>{e2.x - o2.y, e2.y + o2.x};
+                if (k != M - k) row[M - k] = cx<T>{e2.x + o2.y, o2.x - e2.y};
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Forward real transform of every row, after plane_rows<INV = false> of x[2m] + i x[2m+1]:
+// X[k] = E + W_LZ^k O, X[M-k] = conj(E - W_LZ^k O) with E = (Z[k] + conj Z[M-k]) / 2,
+// O = (Z[k] - conj Z[M-k]) / 2i; slot 0 = X[0] + i X[M]. Ends with a barrier.
+template <typename T, int LY, int LZ, int NT>
+__device__ __forceinline__ void plane_r2c_split(cx<T>* pl) {
+    using P = PlaneShape<T, LY, LZ>;
+    constexpr int M = P::M, H = M / 2 + 1, NQ = LY * H, IQ = (NQ + NT - 1) / NT;
+    const cx<T>* twz = pl + P::TWZ;
+    const int tid = plane_tid();
+#pragma unroll
+    for (int it = 0; it < IQ; ++it) {
+        const int q = tid + it * NT;
+        if (NQ % NT == 0 || q < NQ) {
+            const int r = q % LY, k = q / LY;
+            cx<T>* row = pl + r * P::PITCH;
+            if (k == 0) {
+                const cx<T> a = row[0];
+                row[0] = cx<T>{a.x + a.y, a.x - a.y};
+            } else {
+                const cx<T> a = row[k], b = row[M - k];
+                const cx<T> e{T(0.5) * (a.x + b.x), T(0.5) * (a.y - b.y)};
+                const cx<T> o{T(0.5) * (a.y + b.y), T(0.5) * (b.x - a.x)};
+                const cx<T> wo = cmul(twz[k], o);
+                row[k] = e + wo;
+                if (k != M - k) row[M - k] = conjv(e - wo);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// HBM half-spectrum plane [LY][M + 1] -> LDS slots (columns 0 and M packed into slot 0).
+// The caller places a barrier before the slots are read.
+template <typename T, int LY, int LZ, int NT>
+__device__ __forceinline__ void plane_load_spec(cx<T>* pl, const cx<T>* __restrict__ src) {
+    using P = PlaneShape<T, LY, LZ>;
+    constexpr int M = P::M, NE = LY * P::MZH, IE = (NE + NT - 1) / NT;
+    const int tid = plane_tid();
+#pragma unroll 4
+    for (int it = 0; it < IE; ++it) {
+        const int i = tid + it * NT;
+        if (i < NE) {
+            const int r = i / P::MZH, k = i - r * P::MZH;
+            if (k < M) {
+                cx<T> v = src[i];
+                if (k == 0) {
+                    const cx<T> b = src[i + M];
+                    v = cx<T>{v.x - b.y, v.y + b.x};
+                }
+                pl[r * P::PITCH + k] = v;
+            }
+        }
+    }
+}
+
+// LDS slots -> HBM half-spectrum plane, unpacking slot 0 into columns 0 and M:
+// G0 = (C[ky] + conj C[-ky]) / 2, GN = (C[ky] - conj C[-ky]) / 2i.
+template <typename T, int LY, int LZ, int NT>
+__device__ __forceinline__ void plane_store_spec(const cx<T>* pl, cx<T>* __restrict__ dst) {
+    using P = PlaneShape<T, LY, LZ>;
+    constexpr int M = P::M, NE = LY * P::MZH, IE = (NE + NT - 1) / NT;
+    const int tid = plane_tid();
+#pragma unroll 4
+    for (int it = 0; it < IE; ++it) {
+        const int i = tid + it * NT;
+        if (i < NE) {
+            const int r = i / P::MZH, k = i - r * P::MZH;
+            cx<T> v;
+            if (k == 0 || k == M) {
+                const cx<T> c = pl[r * P::PITCH];
+                const cx<T> m = pl[(r == 0 ? 0 : LY - r) * P::PITCH];
+                v = k == 0 ? cx<T>{T(0.5) * (c.x + m.x), T(0.5) * (c.y - m.y)}
+                           : cx<T>{T(0.5) * (c.y + m.y), T(0.5) * (m.x - c.x)};
+            } else {
+                v = pl[r * P::PITCH + k];
+            }
+            dst[i] = v;
+        }
+    }
+}
+
+// Inverse transform of the spectrum plane into the real view (ends with a barrier).
+template <typename T, int LY, int LZ, int NT>
+__device__ __forceinline__ void plane_inverse(cx<T>* pl, const cx<T>* __restrict__ src) {
+    plane_load_spec<T, LY, LZ, NT>(pl, src);
+    __syncthreads();
+    plane_cols<T, LY, LZ, NT, true>(pl);
+    plane_c2r_split<T, LY, LZ, NT>(pl);
+    plane_rows<T, LY, LZ, NT, true>(pl);
+}
+
+// Forward transform of the real view, stored as a half-spectrum plane. The caller places a
+// barrier between the last write of the real view and this call, and another before the plane
+// is written again.
+template <typename T, int LY, int LZ, int NT>
+__device__ __forceinline__ void plane_forward(cx<T>* pl, cx<T>* __restrict__ dst) {
+    plane_rows<T, LY, LZ, NT, false>(pl);
+    plane_r2c_split<T, LY, LZ, NT>(pl);
+    plane_cols<T, LY, LZ, NT, false>(pl);
+    plane_store_spec<T, LY, LZ, NT>(pl, dst);
+}
+
+// Two adjacent cells (2m, 2m+1) of a real row: the element of the real view's slot m.
+template <typename T> struct alignas(2 * sizeof(T)) Pair { T a, b; };
+template <typename T>
+__device__ __forceinline__ Pair<T> ld2(const T* p) { return *reinterpret_cast<const Pair<T>*>(p); }
+template <typename T>
+__device__ __forceinline__ void st2(T* p, T a, T b) { *reinterpret_cast<Pair<T>*>(p) = Pair<T>{a, b}; }
+template <typename T>
+__device__ __forceinline__ Pair<T> ld2m(const SMap<T>& m, int64_t c) {
+    return m.p ? ld2(m.p + c) : Pair<T>{m.s, m.s};
+}
+
+// Element loop over the real view: f(iteration, row r, column pair m, cell offset in plane);
+// slot r * M + m of iteration `it` is threadIdx.x + it * NT. Iterations run in groups of G
+// between scheduling fences, so the loads of later groups are not hoisted above earlier ones
+// (bounded live registers; the other waves of the CU hide the latency).
+template <typename T, int LY, int LZ, int NT, int G = 2, class F>
+__device__ __forceinline__ void plane_elements(F&& f) {
+    using P = PlaneShape<T, LY, LZ>;
+    constexpr int NS = LY * P::M;
+    static_assert(NS % NT == 0, "workgroup size must divide the plane");
+    const int tid = plane_tid();
+#pragma unroll
+    for (int it = 0; it < NS / NT; ++it) {
+        const int i = tid + it * NT;
+        const int r = i / P::M, m = i - r * P::M;
+        f(it, r, m, r * LZ + 2 * m);
+        if ((it + 1) % G == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Sparse sources of rows [x LY, (x+1) LY) into the real view (dirichlet or additive).
+template <typename T, int LY, int LZ, int NT>
+__device__ void plane_apply_sparse(const SparseSrc<T>& s, int x, cx<T>* pl, int t) {
+    using P = PlaneShape<T, LY, LZ>;
+    if (t >= s.len) return;
+    const int r0 = x * LY;
+    const int e0 = s.rowptr[r0], e1 = s.rowptr[r0 + LY];
+    T* f = reinterpret_cast<T*>(pl);
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += NT) {
+        const int64_t cell = s.index[e];
+        const int row = (int)(cell / LZ);
+        const int n = (int)(cell - (int64_t)row * LZ);
+        const T val = s.signal[(int64_t)(s.rows == 1 ? 0 : e) * s.len + t] * s.base[e] * s.factor;
+        T& slot = f[(row - r0) * 2 * P::PITCH + n];
+        slot = s.mode == 0 ? val : slot + val;
+    }
+}
+__device__ __forceinline__ bool sparse_in_plane(const int* rowptr, int x, int ly) {
+    return rowptr && rowptr[(x + 1) * ly] > rowptr[x * ly];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernels. One workgroup per x plane; grid.x = number of planes (block coordinate = plane).
+// ---------------------------------------------------------------------------------------------
+
+// spec = R2C_yz(scale * src), the plane-wise counterpart of the z pass + y pass.
+template <typename T, int LY, int LZ, int NT>
+__device__ __forceinline__ void plane_fwd_real_body(const BIdx& bi, const T* __restrict__ src, cx<T>* spec,
+                                                    T scale, const Geo& g, const AxisTabs<T>& tab) {
+    using P = PlaneShape<T, LY, LZ>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* pl = reinterpret_cast<cx<T>*>(smem);
+    plane_twiddles<T, LY, LZ, NT>(pl, tab);
+    const int x = bi.x;
+    const T* s = src + (int64_t)x * LY * LZ;
+    plane_elements<T, LY, LZ, NT>([&](int, int r, int m, int c) {
+        const Pair<T> v = ld2(s + c);
+        pl[r * P::PITCH + m] = cx<T>{v.a * scale, v.b * scale};
+    });
+    __syncthreads();
+    plane_forward<T, LY, LZ, NT>(pl, spec + (int64_t)x * LY * P::MZH);
+    (void)g;
+}
+template <typename T, int LY, int LZ, int NT>
+__global__ void __launch_bounds__(NT, (plane_waves<T, LY, LZ, NT>())) plane_fwd_real(const T* __restrict__ src, cx<T>* spec, T scale, Geo g,
+                                                     AxisTabs<T> tab) {
+    plane_fwd_real_body<T, LY, LZ, NT>(bidx_here(), src, spec, scale, g, tab);
+}
+
+// dst (= or +=) scale * C2R_yz(spec).
+template <typename T, int LY, int LZ, int NT, bool ACC>
+__device__ __forceinline__ void plane_inv_real_impl(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst,
+                                                    T scale, const AxisTabs<T>& tab) {
+    using P = PlaneShape<T, LY, LZ>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* pl = reinterpret_cast<cx<T>*>(smem);
+    plane_twiddles<T, LY, LZ, NT>(pl, tab);
+    const int x = bi.x;
+    plane_inverse<T, LY, LZ, NT>(pl, spec + (int64_t)x * LY * P::MZH);
+    T* d = dst + (int64_t)x * LY * LZ;
+    plane_elements<T, LY, LZ, NT>([&](int, int r, int m, int c) {
+        const cx<T> v = pl[r * P::PITCH + m];
+        if (ACC) {
+            const Pair<T> o = ld2(d + c);
+            st2(d + c, o.a + v.x * scale, o.b + v.y * scale);
+        } else {
+            st2(d + c, v.x * scale, v.y * scale);
+        }
+    });
+}
+template <typename T, int LY, int LZ, int NT>
+__device__ __forceinline__ void plane_inv_real_body(const BIdx& bi, const cx<T>* spec, T* __restrict__ dst,
+                                                    T scale, int accumulate, const Geo& g, const AxisTabs<T>& tab) {
+    if (accumulate) plane_inv_real_impl<T, LY, LZ, NT, true>(bi, spec, dst, scale, tab);
+    else plane_inv_real_impl<T, LY, LZ, NT, false>(bi, spec, dst, scale, tab);
+    (void)g;
+}
+template <typename T, int LY, int LZ, int NT>
+__global__ void __launch_bounds__(NT, (plane_waves<T, LY, LZ, NT>())) plane_inv_real(const cx<T>* spec, T* __restrict__ dst, T scale,
+                                                     int accumulate, Geo g, AxisTabs<T> tab) {
+    plane_inv_real_body<T, LY, LZ, NT>(bidx_here(), spec, dst, scale, accumulate, g, tab);
+}
+
+// Velocity update of plane x, per axis a (KSpaceFirstOrder.swift:669-678, sources :705-711):
+// g_a = C2R_yz(G_a); u_a = pml_sg_a (pml_sg_a u_a - dt/rho_a g_a) [+ corrected source]
+// [+ sparse sources]; G_a = R2C_yz(u_a) for the divergence's x pass.
+template <typename T, int LY, int LZ, int NT, bool CORR>
+__device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArgs<T>& a, const Geo& g,
+                                                    const AxisTabs<T>& tab) {
+    using P = PlaneShape<T, LY, LZ>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* pl = reinterpret_cast<cx<T>*>(smem);
+    plane_twiddles<T, LY, LZ, NT>(pl, tab);
+    const int x = bi.x;
+    const int64_t rb = (int64_t)x * LY * LZ, sb = (int64_t)x * LY * P::MZH;
+    sfor<0, 3>([&](auto cax) {
+        constexpr int ax = decltype(cax)::value;
+        plane_inverse<T, LY, LZ, NT>(pl, a.G[ax] + sb);
+        T* u = a.u[ax] + rb;
+        const SMap<T> dtr = a.dtr[ax], corr = a.corr[ax];
+        const T* pm = tab.pmlsg[ax];
+        const bool sp = sparse_in_plane(a.sp[ax].rowptr, x, LY);
+        const T qx = pm[ax == 0 ? x : 0];
+        plane_elements<T, LY, LZ, NT>([&](int, int r, int m, int c) {
+            const cx<T> gv = pl[r * P::PITCH + m];
+            const Pair<T> uu = ld2(u + c);
+            const Pair<T> dt = ld2m(dtr, rb + c);
+            T q0, q1;
+            if constexpr (ax == 0) { q0 = qx; q1 = qx; }
+            else if constexpr (ax == 1) { q0 = pm[r]; q1 = q0; }
+            else { const Pair<T> qq = ld2(pm + 2 * m); q0 = qq.a; q1 = qq.b; }
+            T v0 = q0 * (q0 * uu.a - dt.a * gv.x);
+            T v1 = q1 * (q1 * uu.b - dt.b * gv.y);
+            if constexpr (CORR) {
+                const Pair<T> cr = ld2m(corr, rb + c);
+                v0 += cr.a;
+                v1 += cr.b;
+            }
+            pl[r * P::PITCH + m] = cx<T>{v0, v1};
+            if (!sp) st2(u + c, v0, v1);
+        });
+        __syncthreads();
+        if (sp) {
+            plane_apply_sparse<T, LY, LZ, NT>(a.sp[ax], x, pl, a.ctr[0]);
+            __syncthreads();
+            plane_elements<T, LY, LZ, NT>([&](int, int r, int m, int c) {
+                const cx<T> v = pl[r * P::PITCH + m];
+                st2(u + c, v.x, v.y);
+            });
+            __syncthreads();
+        }
+        plane_forward<T, LY, LZ, NT>(pl, a.G[ax] + sb);
+        __syncthreads();
+    });
+    (void)g;
+}
+template <typename T, int LY, int LZ, int NT, bool CORR>
+__global__ void __launch_bounds__(NT, (plane_waves<T, LY, LZ, NT>())) plane_velocity(VelArgs<T> a, Geo g, AxisTabs<T> tab) {
+    plane_velocity_body<T, LY, LZ, NT, CORR>(bidx_here(), a, g, tab);
+}
+
+// Density update and EOS of plane x (KSpaceFirstOrder.swift:680-702): per axis a,
+// du_a = C2R_yz(D_a); rho_a = pml_a (pml_a rho_a - dt rho0 [B/A factor] du_a) [+ sources];
+// then p = c0^2 (sum_a rho_a [+ tau rho0 sum_a du_a] [+ B/A (sum rho)^2 / (2 rho0)]) (MODE 0:
+// lossless, 2: Stokes) and P = R2C_yz(p) for the next step's x pass. The sums are held in
+// registers (one slot pair per element of the thread's share of the plane).
+template <typename T, int LY, int LZ, int NT, bool NL, int MODE, bool CORR>
+__device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g,
+                                                   const AxisTabs<T>& tab) {
+    using P = PlaneShape<T, LY, LZ>;
+    static_assert(MODE == 0 || MODE == 2, "plane density kernel: lossless or Stokes");
+    constexpr int NV = LY * P::M / NT;
+    constexpr bool HOLD = !NL && MODE == 0;     // rho0 kept in registers across the axes
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* pl = reinterpret_cast<cx<T>*>(smem);
+    plane_twiddles<T, LY, LZ, NT>(pl, tab);
+    const int x = bi.x;
+    const int64_t rb = (int64_t)x * LY * LZ, sb = (int64_t)x * LY * P::MZH;
+    cx<T> rt[NV];
+    cx<T> sd[MODE == 2 ? NV : 1];
+    cx<T> rp[NL ? NV : 1];
+    cx<T> r0h[HOLD ? NV : 1];
+    if constexpr (NL) {
+        plane_elements<T, LY, LZ, NT>([&](int it, int, int, int c) {
+            T s0 = 0, s1 = 0;
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const Pair<T> v = ld2(a.rho[ax] + rb + c);
+                s0 += v.a;
+                s1 += v.b;
+            }
+            rp[it] = cx<T>{s0, s1};
+        });
+    }
+    if constexpr (HOLD) {
+        plane_elements<T, LY, LZ, NT>([&](int it, int, int, int c) {
+            const Pair<T> v = ld2m(a.rho0, rb + c);
+            r0h[it] = cx<T>{v.a, v.b};
+        });
+    }
+    sfor<0, 3>([&](auto cax) {
+        constexpr int ax = decltype(cax)::value;
+        plane_inverse<T, LY, LZ, NT>(pl, a.D[ax] + sb);
+        T* rho = a.rho[ax] + rb;
+        const T* pm = tab.pml[ax];
+        const T cf = a.corr_factor[ax];
+        const bool sp = sparse_in_plane(a.sp[ax].rowptr, x, LY);
+        const T qx = pm[ax == 0 ? x : 0];
+        plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int c) {
+            const cx<T> du = pl[r * P::PITCH + m];
+            Pair<T> r0;
+            if constexpr (HOLD) r0 = Pair<T>{r0h[it].x, r0h[it].y};
+            else r0 = ld2m(a.rho0, rb + c);
+            const Pair<T> ro = ld2(rho + c);
+            T q0, q1;
+            if constexpr (ax == 0) { q0 = qx; q1 = qx; }
+            else if constexpr (ax == 1) { q0 = pm[r]; q1 = q0; }
+            else { const Pair<T> qq = ld2(pm + 2 * m); q0 = qq.a; q1 = qq.b; }
+            const T d0 = r0.a * du.x, d1 = r0.b * du.y;
+            if constexpr (MODE == 2) {
+                if (ax == 0) sd[it] = cx<T>{d0, d1};
+                else { sd[it].x += d0; sd[it].y += d1; }
+            }
+            T m0 = a.dt * d0, m1 = a.dt * d1;
+            if constexpr (NL) {
+                m0 *= (T(2) * rp[it].x + r0.a) / r0.a;
+                m1 *= (T(2) * rp[it].y + r0.b) / r0.b;
+            }
+            T v0 = q0 * (q0 * ro.a - m0);
+            T v1 = q1 * (q1 * ro.b - m1);
+            if constexpr (CORR) {
+                const Pair<T> cr = ld2m(a.corr, rb + c);
+                v0 += cr.a * cf;
+                v1 += cr.b * cf;
+            }
+            if (sp) {
+                pl[r * P::PITCH + m] = cx<T>{v0, v1};
+            } else {
+                st2(rho + c, v0, v1);
+                if (ax == 0) rt[it] = cx<T>{v0, v1};
+                else { rt[it].x += v0; rt[it].y += v1; }
+            }
+        });
+        __syncthreads();
+        if (sp) {
+            plane_apply_sparse<T, LY, LZ, NT>(a.sp[ax], x, pl, a.ctr[0]);
+            __syncthreads();
+            plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int c) {
+                const cx<T> v = pl[r * P::PITCH + m];
+                st2(rho + c, v.x, v.y);
+                if (ax == 0) rt[it] = v;
+                else { rt[it].x += v.x; rt[it].y += v.y; }
+            });
+            __syncthreads();
+        }
+    });
+    // EOS into the real view, then P = R2C_yz(p)
+    T* p = a.p + rb;
+    plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int c) {
+        T e0 = rt[it].x, e1 = rt[it].y;
+        Pair<T> r0{T(1), T(1)};
+        if constexpr (NL) {
+            if constexpr (HOLD) r0 = Pair<T>{r0h[it].x, r0h[it].y};
+            else r0 = ld2m(a.rho0, rb + c);
+        }
+        if constexpr (MODE == 2) {
+            const Pair<T> tu = ld2m(a.tau, rb + c);
+            e0 += tu.a * sd[it].x;
+            e1 += tu.b * sd[it].y;
+        }
+        if constexpr (NL) {
+            const Pair<T> bn = ld2m(a.bona, rb + c);
+            e0 += bn.a * (rt[it].x * rt[it].x) / (T(2) * r0.a);
+            e1 += bn.b * (rt[it].y * rt[it].y) / (T(2) * r0.b);
+        }
+        const Pair<T> cc = ld2m(a.c0, rb + c);
+        const T p0 = cc.a * cc.a * e0, p1 = cc.b * cc.b * e1;
+        st2(p + c, p0, p1);
+        pl[r * P::PITCH + m] = cx<T>{p0, p1};
+    });
+    __syncthreads();
+    plane_forward<T, LY, LZ, NT>(pl, a.outP + sb);
+    (void)g;
+}
+template <typename T, int LY, int LZ, int NT, bool NL, int MODE, bool CORR>
+__global__ void __launch_bounds__(NT, (plane_waves<T, LY, LZ, NT>())) plane_density(RhoArgs<T> a, Geo g, AxisTabs<T> tab) {
+    plane_density_body<T, LY, LZ, NT, NL, MODE, CORR>(bidx_here(), a, g, tab);
+}
+
+}  // namespace kwh

@@ -1,0 +1,79 @@
+// kwh_planeops_impl.h -- builds the PlaneOps table of one plane shape for one working type.
+// Included only by the per-shape translation units kwh_plane_<m1>x<m2>.hip.
+#pragma once
+#include "kwh_plane.h"
+#include "kwh_planeops.h"
+
+namespace kwh {
+
+template <typename T, int LY, int LZ, int NT>
+struct PlaneImpl {
+    using S = PlaneShape<T, LY, LZ>;
+    static constexpr int W = plane_waves<T, LY, LZ, NT>();
+    using FwdB = Body<&plane_fwd_real_body<T, LY, LZ, NT>, &plane_fwd_real<T, LY, LZ, NT>, W, false, NT>;
+    using InvB = Body<&plane_inv_real_body<T, LY, LZ, NT>, &plane_inv_real<T, LY, LZ, NT>, W, false, NT>;
+    template <bool CORR>
+    using VelB = Body<&plane_velocity_body<T, LY, LZ, NT, CORR>, &plane_velocity<T, LY, LZ, NT, CORR>, W, false, NT>;
+    template <bool NL, int MODE, bool CORR>
+    using DenB = Body<&plane_density_body<T, LY, LZ, NT, NL, MODE, CORR>, &plane_density<T, LY, LZ, NT, NL, MODE, CORR>,
+                      W, false, NT>;
+
+    static hipError_t set_lds(const void* fn) {
+        return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::kLds);
+    }
+    template <bool NL, int MODE>
+    static hipError_t set_density() {
+        hipError_t e = set_lds((const void*)&plane_density<T, LY, LZ, NT, NL, MODE, false>);
+        return e != hipSuccess ? e : set_lds((const void*)&plane_density<T, LY, LZ, NT, NL, MODE, true>);
+    }
+    static hipError_t prepare() {
+        if (S::kLds <= 65536) return hipSuccess;
+        const void* fns[] = {(const void*)&plane_fwd_real<T, LY, LZ, NT>, (const void*)&plane_inv_real<T, LY, LZ, NT>,
+                             (const void*)&plane_velocity<T, LY, LZ, NT, false>,
+                             (const void*)&plane_velocity<T, LY, LZ, NT, true>};
+        for (const void* f : fns)
+            if (hipError_t e = set_lds(f); e != hipSuccess) return e;
+        hipError_t e = set_density<false, 0>();
+        if (e == hipSuccess) e = set_density<true, 0>();
+        if (e == hipSuccess) e = set_density<false, 2>();
+        if (e == hipSuccess) e = set_density<true, 2>();
+        return e;
+    }
+
+    static void fwd_real(const Emit& em, int planes, const T* src, cx<T>* spec, T scale, const Geo& g,
+                         const AxisTabs<T>& tab) {
+        em.go<FwdB>(dim3(planes), dim3(NT), S::kLds, src, spec, scale, g, tab);
+    }
+    static void inv_real(const Emit& em, int planes, const cx<T>* spec, T* dst, T scale, int accumulate, const Geo& g,
+                         const AxisTabs<T>& tab) {
+        em.go<InvB>(dim3(planes), dim3(NT), S::kLds, spec, dst, scale, accumulate, g, tab);
+    }
+    static void velocity(const Emit& em, int planes, const VelArgs<T>& va, const Geo& g, const AxisTabs<T>& tab,
+                         bool corr) {
+        if (corr) em.go<VelB<true>>(dim3(planes), dim3(NT), S::kLds, va, g, tab);
+        else em.go<VelB<false>>(dim3(planes), dim3(NT), S::kLds, va, g, tab);
+    }
+    template <bool NL, int MODE>
+    static void density_m(const Emit& em, int planes, const RhoArgs<T>& ra, const Geo& g, const AxisTabs<T>& tab,
+                          bool corr) {
+        if (corr) em.go<DenB<NL, MODE, true>>(dim3(planes), dim3(NT), S::kLds, ra, g, tab);
+        else em.go<DenB<NL, MODE, false>>(dim3(planes), dim3(NT), S::kLds, ra, g, tab);
+    }
+    static void density(const Emit& em, int planes, const RhoArgs<T>& ra, const Geo& g, const AxisTabs<T>& tab,
+                        bool nonlinear, int mode, bool corr) {
+        if (mode == 2) {
+            if (nonlinear) density_m<true, 2>(em, planes, ra, g, tab, corr);
+            else density_m<false, 2>(em, planes, ra, g, tab, corr);
+        } else {
+            if (nonlinear) density_m<true, 0>(em, planes, ra, g, tab, corr);
+            else density_m<false, 0>(em, planes, ra, g, tab, corr);
+        }
+    }
+
+    static const PlaneOps<T>* ops() {
+        static const PlaneOps<T> o{LY, LZ, NT, S::kLds, &prepare, &fwd_real, &inv_real, &velocity, &density};
+        return &o;
+    }
+};
+
+}  // namespace kwh

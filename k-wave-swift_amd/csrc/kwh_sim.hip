@@ -29,6 +29,7 @@
 #include "kwh_batch.h"
 #include "kwh_launch.h"
 #include "kwh_regops.h"
+#include "kwh_planeops.h"
 
 using namespace kwh;
 
@@ -245,6 +246,9 @@ struct kwh_sim {
     virtual void set_timing_only(const char* name) = 0;
     virtual int kernel_stats(kwh_kernel_stat* st, int cap) = 0;
     virtual void set_graph(bool on) = 0;
+    virtual void set_plane(bool on) = 0;
+    virtual bool plane_mode() const = 0;
+    virtual bool plane_available() const = 0;
     virtual void element_traces(int field, const int32_t* element, const int32_t* delay, int nel, double scale,
                                 void* dst, size_t bytes) = 0;
     virtual void set_checkpointing(int every) = 0;
@@ -265,12 +269,14 @@ struct kwh_sim {
 
 namespace {
 
-constexpr int kNames = 17;
+constexpr int kNames = 21;
 const char* kKernelNames[kNames] = {"strided_fwd", "strided_inv", "middle_grad", "middle_div", "middle_abs",
                                     "z_velocity",  "z_density",   "z_eos",       "z_init",     "z_forward_real",
                                     "z_inverse_real", "middle_table", "gather", "source_scatter", "fill", "other",
-                                    "z_colloc_rows"};
-enum KId { K_SF, K_SI, K_MG, K_MD, K_MA, K_ZV, K_ZD, K_ZE, K_ZI, K_ZFR, K_ZIR, K_MT, K_GA, K_SC, K_FI, K_OT, K_ZCR };
+                                    "z_colloc_rows", "plane_velocity", "plane_density", "plane_forward_real",
+                                    "plane_inverse_real"};
+enum KId { K_SF, K_SI, K_MG, K_MD, K_MA, K_ZV, K_ZD, K_ZE, K_ZI, K_ZFR, K_ZIR, K_MT, K_GA, K_SC, K_FI, K_OT, K_ZCR,
+           K_PV, K_PD, K_PFR, K_PIR };
 
 template <typename T>
 struct SourceDev {
@@ -322,7 +328,7 @@ struct SimT final : kwh_sim {
     DBuf src_spec;
     int slab_lo = 0, slab_n = 0;
     bool has_p0 = false;
-    bool P_valid = false;  // spec[0] holds R2C_z(p) for the next step
+    bool P_valid = false;  // spec[0] holds R2C_z(p) (R2C_yz(p) in plane mode) for the next step
 
     // sources
     SourceDev<T> psrc, usrc[3];
@@ -387,6 +393,10 @@ struct SimT final : kwh_sim {
     }
     // register-resident launchers per internal axis (null: generic Stockham kernels)
     const RegOps<T>* rx[3] = {nullptr, nullptr, nullptr};
+    // y-z plane kernels for this grid's (m1, m2) plane (null: none fits); plane mode runs the y
+    // and z passes of a step inside them, so spec[0] carries R2C_yz(p) between steps
+    const PlaneOps<T>* px = nullptr;
+    bool plane = false;
     // launch schedule chosen by the autotuner: halve the tile columns of the strided / middle
     // passes (ntr_shift), halve the row pairs per workgroup of the row kernels (p_shift), LDS
     // budget of the generic Stockham passes and the workgroup count below which they take fewer
@@ -646,6 +656,12 @@ struct SimT final : kwh_sim {
     // Full forward transform of a real field into `out` (z, then plain passes): leaves the
     // spectrum ready for a middle pass.
     void forward_to_middle(const T* src, C* out, T scale) {
+        if (plane) {
+            timed(K_PFR, real_bytes() + spec_bytes(), [&] {
+                px->fwd_real(em(), g.m0, src, out, scale, g, tab);
+            });
+            return;
+        }
         if (rx[2]) {
             const RegZ z = regz();
             timed(K_ZFR, real_bytes() + spec_bytes(), [&] {
@@ -662,6 +678,12 @@ struct SimT final : kwh_sim {
         if (plain1) strided(1, false, {out});
     }
     void inverse_from_middle(C* in, T* dst, T scale, bool accumulate) {
+        if (plane) {
+            timed(K_PIR, real_bytes() * (accumulate ? 2 : 1) + spec_bytes(), [&] {
+                px->inv_real(em(), g.m0, (const C*)in, dst, scale, accumulate ? 1 : 0, g, tab);
+            });
+            return;
+        }
         if (plain1) strided(1, true, {in});
         if (rx[2]) {
             const RegZ z = regz();
@@ -717,6 +739,13 @@ struct SimT final : kwh_sim {
             const int mm[3] = {g.m0, g.m1, g.m2};
             for (int ax = 0; ax < 3; ++ax)
                 rx[ax] = use_sq && mm[ax] > 1 ? reg_ops<T>(mm[ax]) : nullptr;
+        }
+        if (dim == 3 && !axisym && mid_axis == 0) {
+            px = plane_ops<T>(g.m1, g.m2);
+            int lds_max = 0;
+            HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+            if (px && px->lds > (size_t)lds_max) px = nullptr;
+            if (px) HIPCHK(px->prepare());
         }
         const int64_t N = g.N;
 
@@ -1460,6 +1489,7 @@ struct SimT final : kwh_sim {
             middle(K_MT, {s}, {MidOp{OP_COLL, a + g.off, 0, s, nullptr}});
             inverse_from_middle(s, ucol[a].as<T>(), T(1), false);
         }
+        if (plane) strided(1, false, {spec[0].p});   // R2C_z(p) -> R2C_yz(p)
         P_valid = true;
     }
 
@@ -1476,6 +1506,10 @@ struct SimT final : kwh_sim {
             // base scale 2 dt / (dim c); the 1/d_a factor is applied per axis in z_density
             compute_correction(psrc, t, corr_p.as<T>(), 1.0);
             cp = corr_p.as<T>();
+        }
+        if (plane) {
+            main_step_plane(cu, cp);
+            return;
         }
         C* P = spec[0].as<C>();
         // velocity: grad p (velStep :669-678)
@@ -1653,6 +1687,111 @@ struct SimT final : kwh_sim {
         }
         P_valid = true;
     }
+
+    // The step in plane mode: x pass of P (already R2C_yz(p)), plane velocity kernel (y and z
+    // inverse, update, y and z forward per axis), x pass of the divergence, plane density kernel
+    // (inverse, update, EOS, R2C_yz(p)). Same operations and order as main_step's split passes.
+    void main_step_plane(const T* const cu[3], const T* cp) {
+        C* P = spec[0].as<C>();
+        {
+            std::vector<MidOp> ops;
+            for (int a = 0; a < dim; ++a) ops.push_back(MidOp{OP_DDXP_KAPPA, a + g.off, 0, spec[a].p, kappa.p});
+            middle(K_MG, {P}, ops);
+        }
+        {
+            VelArgs<T> va{};
+            bool anycorr = false;
+            double bytes = 0;
+            for (int a = 0; a < dim; ++a) {
+                va.G[a] = spec[a].as<C>();
+                va.u[a] = u[a].as<T>();
+                va.dtr[a] = dtrs[a];
+                va.corr[a] = cu[a] ? mk_map(cu[a]) : zero_map;
+                va.sp[a] = sparse_of(usrc[a], a);
+                anycorr |= cu[a] != nullptr;
+                bytes += 2 * spec_bytes() + 2 * real_bytes() + mapb(dtrs[a].p) + (cu[a] ? real_bytes() : 0);
+            }
+            va.ctr = (const int*)ctr.p;
+            timed(K_PV, bytes, [&] { px->velocity(em(), g.m0, va, g, tab, anycorr); });
+        }
+        {
+            std::vector<const void*> ins;
+            std::vector<MidOp> ops;
+            for (int a = 0; a < dim; ++a) {
+                ins.push_back(spec[a].p);
+                ops.push_back(MidOp{OP_DDXN_KAPPA, a + g.off, a, spec[a].p, kappa.p});
+            }
+            middle(K_MD, ins, ops);
+        }
+        {
+            RhoArgs<T> ra{};
+            for (int a = 0; a < dim; ++a) {
+                ra.D[a] = spec[a].as<C>();
+                ra.rho[a] = rho[a].as<T>();
+                ra.corr_factor[a] = T(psrc.factor[a]);
+                ra.sp[a] = sparse_of(psrc, a);
+            }
+            ra.rho0 = rho0s;
+            ra.c0 = c0s;
+            ra.bona = bonas;
+            ra.tau = taus;
+            ra.corr = cp ? mk_map(cp) : zero_map;
+            ra.p = p.as<T>();
+            ra.outP = P;
+            ra.dt = T(dt);
+            ra.mode = stokes ? 2 : 0;
+            ra.nonlinear = nonlinear;
+            ra.stokes = stokes;
+            ra.ctr = (const int*)ctr.p;
+            double bytes = dim * (spec_bytes() + 2 * real_bytes()) + mapb(rho0s.p) + (cp ? real_bytes() : 0) +
+                           real_bytes() + mapb(c0s.p) + spec_bytes() + (stokes ? mapb(taus.p) : 0);
+            if (nonlinear) bytes += mapb(bonas.p) + dim * real_bytes();
+            timed(K_PD, bytes, [&] {
+                px->density(em(), g.m0, ra, g, tab, nonlinear, stokes ? 2 : 0, cp != nullptr);
+            });
+        }
+        P_valid = true;
+    }
+
+    // Plane mode covers 3-D grids whose (m1, m2) plane has plane kernels, lossless or Stokes
+    // media (the FFT absorption term has passes of its own), no collocated velocity output
+    // (computed from the z-only spectra of the split passes) and no checkpointing.
+    bool plane_supported() const {
+        if (!px || axisym || ckpt_every > 0) return false;
+        if (absorbing && !stokes) return false;
+        for (int a = 0; a < dim; ++a)
+            if (need_u[a]) return false;
+        return true;
+    }
+    void set_plane(bool on) override {
+        KWH_REQUIRE(!on || plane_supported(), KWH_EUNSUPPORTED,
+                    "plane kernels need a 3-D grid with a 96x96 or 128x128 (y, z) plane, lossless or Stokes media, "
+                    "no collocated velocity output and no checkpointing");
+        if (on == plane) return;
+        HIPCHK(hipSetDevice(dev));
+        sync();
+        // spec[0] between steps: R2C_z(p) in split mode, R2C_yz(p) in plane mode. The transforms
+        // are unnormalised, so the way back along y carries the 1/m1 of the inverse.
+        if (steps > 0) {
+            strided(1, !on, {spec[0].p});
+            if (!on) {
+                const int64_t n = 2 * g.NH;
+                hipLaunchKernelGGL((convert_kernel<T, T>), dim3(grid_blocks(n)), dim3(256), 0, stream, spec[0].as<T>(),
+                                   spec[0].as<T>(), T(1.0 / g.m1), n);
+                HIPCHK(hipGetLastError());
+            }
+        }
+        plane = on;
+        for (auto& sg : step_graphs) {
+            if (sg.second.exec) (void)hipGraphExecDestroy(sg.second.exec);
+            (void)hipGraphDestroy(sg.second.graph);
+        }
+        step_graphs.clear();
+        step_records.clear();
+        sync();
+    }
+    bool plane_mode() const override { return plane; }
+    bool plane_available() const override { return plane_supported(); }
 
     void record_step(int) {
         if (!(rec_p || rec_u)) return;
@@ -2126,6 +2265,7 @@ struct SimT final : kwh_sim {
     void set_checkpointing(int every) override {
         KWH_REQUIRE(every >= 0, KWH_EINVAL, "checkpoint interval must be >= 0");
         KWH_REQUIRE(!axisym || every == 0, KWH_EUNSUPPORTED, "the axisymmetric solver has no adjoint");
+        KWH_REQUIRE(!plane || every == 0, KWH_EUNSUPPORTED, "checkpointing runs with the split passes (plane mode off)");
         sync();
         ckpt_every = every;
         ckpt_have.clear();
@@ -2158,6 +2298,11 @@ struct SimT final : kwh_sim {
     }
     // spec[0] = R2C_z(p), the derived state a step starts from
     void rebuild_P() {
+        if (plane) {
+            forward_to_middle(p.as<T>(), spec[0].as<C>(), T(1));
+            P_valid = true;
+            return;
+        }
         if (rx[2]) {
             const RegZ z = regz();
             timed(K_ZFR, real_bytes() + spec_bytes(), [&] {
@@ -2224,6 +2369,7 @@ struct SimT final : kwh_sim {
         KWH_REQUIRE(dldy && grad_p0 && grad_c0, KWH_EINVAL, "NULL argument");
         KWH_REQUIRE(every > 0, KWH_EINVAL, "checkpoint interval must be > 0");
         KWH_REQUIRE(!axisym, KWH_EUNSUPPORTED, "the axisymmetric solver has no adjoint");
+        KWH_REQUIRE(!plane, KWH_EUNSUPPORTED, "the adjoint runs with the split passes (plane mode off)");
         KWH_REQUIRE(has_p0, KWH_EUNSUPPORTED, "the gradient is defined for the p0 problem");
         KWH_REQUIRE(!absorbing && !nonlinear, KWH_EUNSUPPORTED, "the adjoint covers the lossless linear loop");
         KWH_REQUIRE(!psrc.present && !usrc[0].present && !usrc[1].present && !usrc[2].present, KWH_EUNSUPPORTED,
@@ -2902,6 +3048,8 @@ struct kwh_batch {
             }
             fl->nblocks = (unsigned)owner.size();
             if (fl->nblocks == 0) continue;
+            if (fl->lds > 65536)   // plane kernels: raise the batched kernel's dynamic-LDS limit
+                HIPCHK(hipFuncSetAttribute(fl->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fl->lds));
             fl->packs.alloc(packs.size());
             HIPCHK(hipMemcpy(fl->packs.p, packs.data(), packs.size(), hipMemcpyHostToDevice));
             fl->owner.alloc(owner.size() * sizeof(int));
@@ -2982,6 +3130,8 @@ struct kwh_batch {
     void run(int mode) {
         HIPCHK(hipSetDevice(dev));
         for (kwh_sim* s : sims) s->sync();
+        for (kwh_sim* s : sims)
+            if (s->plane_available() || s->plane_mode()) s->set_plane((mode & KWH_BATCH_PLANE) != 0);
         if (mode & KWH_BATCH_FUSED) {
             run_fused((mode & KWH_BATCH_GRAPH) != 0);
             return;
@@ -3283,6 +3433,13 @@ int kwh_sim_set_graph(kwh_sim* sim, int enable) {
     });
 }
 
+int kwh_sim_set_plane(kwh_sim* sim, int enable) {
+    return guarded([&] {
+        KWH_REQUIRE(sim, KWH_EINVAL, "sim is NULL");
+        sim->set_plane(enable != 0);
+    });
+}
+
 void kwh_sim_destroy(kwh_sim* sim) { delete sim; }
 
 int kwh_sim_element_traces(kwh_sim* sim, int32_t field, const int32_t* element, const int32_t* delay,
@@ -3383,7 +3540,7 @@ int kwh_batch_create(kwh_ctx* ctx, const kwh_sim_desc* descs, int32_t n, kwh_bat
 int kwh_batch_run(kwh_batch* batch, int32_t mode) {
     return guarded([&] {
         KWH_REQUIRE(batch, KWH_EINVAL, "batch is NULL");
-        KWH_REQUIRE((mode & ~(KWH_BATCH_GRAPH | KWH_BATCH_FUSED)) == 0, KWH_EINVAL, "invalid batch run mode");
+        KWH_REQUIRE((mode & ~(KWH_BATCH_GRAPH | KWH_BATCH_FUSED | KWH_BATCH_PLANE)) == 0, KWH_EINVAL, "invalid batch run mode");
         batch->run(mode);
     });
 }

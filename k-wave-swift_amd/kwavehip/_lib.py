@@ -9,8 +9,9 @@ import ctypes as C
 import os
 
 ABI_VERSION = 2
-BATCH_GRAPH = 1    # kwh_batch_run mode flags (KWH_BATCH_GRAPH / KWH_BATCH_FUSED)
+BATCH_GRAPH = 1    # kwh_batch_run mode flags (KWH_BATCH_GRAPH / KWH_BATCH_FUSED / KWH_BATCH_PLANE)
 BATCH_FUSED = 2
+BATCH_PLANE = 4
 
 OK, EINVAL, ENOMEM, EDEVICE, ENONFINITE, EUNSUPPORTED, ESTATE = 0, -1, -2, -3, -4, -5, -6
 
@@ -27,6 +28,7 @@ EXPORTED = [
     "kwh_sim_run", "kwh_sim_launch", "kwh_sim_sync", "kwh_sim_output_shape", "kwh_sim_read_output",
     "kwh_sim_steps_done", "kwh_sim_state_bytes", "kwh_sim_save_state", "kwh_sim_load_state",
     "kwh_sim_set_kernel_timing", "kwh_sim_set_kernel_timing_only", "kwh_sim_kernel_stats", "kwh_sim_set_graph",
+    "kwh_sim_set_plane",
     "kwh_sim_element_traces", "kwh_sim_autotune", "kwh_sim_set_checkpointing", "kwh_sim_vjp", "kwh_sim_destroy",
     "kwh_batch_create", "kwh_batch_run", "kwh_batch_sim", "kwh_batch_read_output", "kwh_batch_destroy",
     "kwh_time_reversal", "kwh_sim_set_nonfinite_check", "kwh_sim_read_output_noalloc",
@@ -103,6 +105,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.kwh_sim_set_kernel_timing_only.argtypes = [vp, C.c_char_p]
     lib.kwh_sim_kernel_stats.argtypes = [vp, C.POINTER(KernelStat), C.c_int32, C.POINTER(C.c_int32)]
     lib.kwh_sim_set_graph.argtypes = [vp, C.c_int]
+    lib.kwh_sim_set_plane.argtypes = [vp, C.c_int]
     lib.kwh_sim_element_traces.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
                                            C.c_double, vp, C.c_size_t]
     lib.kwh_sim_autotune.argtypes = [vp, C.c_char_p, C.c_size_t]

@@ -475,6 +475,11 @@ class Simulation:
         """Replay the time step from captured hipGraphs (bitwise equal to eager launches)."""
         L.check(self.lib.kwh_sim_set_graph(self.handle, 1 if on else 0))
 
+    def set_plane(self, on: bool):
+        """Plane mode (kwh_sim_set_plane): the y and z transforms of each field in one kernel per
+        x plane; 3-D grids with a 96x96 or 128x128 (y, z) plane, lossless or Stokes media."""
+        L.check(self.lib.kwh_sim_set_plane(self.handle, 1 if on else 0))
+
     def element_traces(self, field: int, element, delay, n_elements: int, scale: float) -> np.ndarray:
         """Receive combination on the device (combineSensorData, KWaveTransducer.swift:263-289):
         out[e, t] = scale * sum over sensor points k of element e of series[k, t + delay[k]]."""
@@ -568,10 +573,12 @@ class SimulationBatch:
     per-scan-line transmits, for instance. `sims[i]` reads outputs like a Simulation.
 
     run(fused=True): one launch per pass of the time step covers all sims of one grid shape, the
-    shapes running concurrently on their own streams; with use_graph each shape's batched step
-    is captured once and replayed.
+    shapes running one after another on the batch's stream; with use_graph each shape's batched
+    step is captured once and replayed.
     run(fused=False): each sim on its own stream, replaying its own step graphs (use_graph) or
-    enqueuing its steps eagerly. All modes give results bitwise equal to single runs."""
+    enqueuing its steps eagerly.
+    run(plane=True): the sims that plane mode covers (Simulation.set_plane) run in it.
+    All modes give results bitwise equal to single runs in the same plane mode."""
 
     def __init__(self, cases, device: int = 0):
         self.lib = L.load()
@@ -596,8 +603,8 @@ class SimulationBatch:
             L.check(self.lib.kwh_batch_sim(self.handle, i, C.byref(h)))
             self.sims.append(Simulation._view(self.lib, h, grid, dt, rec, ns))
 
-    def run(self, use_graph: bool = True, fused: bool = True):
-        mode = (L.BATCH_GRAPH if use_graph else 0) | (L.BATCH_FUSED if fused else 0)
+    def run(self, use_graph: bool = True, fused: bool = True, plane: bool = False):
+        mode = (L.BATCH_GRAPH if use_graph else 0) | (L.BATCH_FUSED if fused else 0) | (L.BATCH_PLANE if plane else 0)
         L.check(self.lib.kwh_batch_run(self.handle, mode))
 
     def close(self):
