@@ -46,11 +46,14 @@ template <typename T, int LY, int LZ>
 struct PlaneShape {
     static constexpr int M = LZ / 2;                  // complex slots per row
     static constexpr int MZH = M + 1;                 // half-spectrum row length in HBM
-    static constexpr int PITCH = M + 1;               // LDS row pitch (odd: rows land on other banks)
-    static constexpr int TWY = LY * PITCH;            // LDS offset of the y twiddles (LY entries)
-    static constexpr int TWZ = TWY + LY;              // LDS offset of the z twiddles (LZ entries)
-    static constexpr size_t kLds = (size_t)(TWZ + LZ) * sizeof(cx<T>);
-    static_assert(LZ % 4 == 0 && plane_r1(LY) > 0 && plane_r1(M) > 0, "unsupported plane shape");
+    static constexpr int PITCH = MZH;                 // LDS row pitch = HBM row: planes copy verbatim
+    static constexpr int R1Y = plane_r1(LY), R2Y = LY / R1Y;
+    static constexpr int R1M = plane_r1(M), R2M = M / R1M;
+    static constexpr int TWY = LY * PITCH;            // W_LY^(n1 k2) laid out [n1][k2] (LY entries)
+    static constexpr int TWM = TWY + LY;              // W_M^(n1 k2) laid out [n1][k2] (M entries)
+    static constexpr int TWS = TWM + M;               // W_LZ^k, k <= M / 2 (M entries reserved)
+    static constexpr size_t kLds = (size_t)(TWS + M) * sizeof(cx<T>);
+    static_assert(LZ % 4 == 0 && R1Y > 0 && R1M > 0, "unsupported plane shape");
 };
 
 // Occupancy floor (waves per SIMD) of a plane kernel with NT lanes: as many workgroups per CU as
@@ -64,67 +67,82 @@ constexpr int plane_waves() {
     return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
-// Twiddle tables W_LY^m and W_LZ^m (forward) into LDS. Ordered before first use by the barrier
-// that follows the first plane load.
+// Twiddle tables into LDS (from the per-axis tables tw[a][m] = W_La^m). Laid out [n1][k2] so that
+// a task's twiddles are base + immediate addresses. Ordered before first use by the barrier that
+// follows the first plane load.
 template <typename T, int LY, int LZ, int NT>
 __device__ __forceinline__ void plane_twiddles(cx<T>* lds, const AxisTabs<T>& tab) {
     using P = PlaneShape<T, LY, LZ>;
-    for (int i = threadIdx.x; i < LY + LZ; i += NT)
-        lds[P::TWY + i] = i < LY ? tab.tw[1][i] : tab.tw[2][i - LY];
+    for (int i = threadIdx.x; i < LY + 2 * P::M; i += NT) {
+        cx<T> w;
+        if (i < LY) {
+            w = tab.tw[1][((i / P::R2Y) * (i % P::R2Y)) % LY];
+        } else if (i < LY + P::M) {
+            const int j = i - LY;
+            w = tab.tw[2][(2 * (j / P::R2M) * (j % P::R2M)) % LZ];
+        } else {
+            w = tab.tw[2][i - LY - P::M];
+        }
+        lds[P::TWY + i] = w;
+    }
 }
 
 // In-place DFTs of NTR transforms of length L = R1 * R2 held in LDS: element n of transform t
-// at buf[t * TS + n * ES]; consecutive lanes take consecutive transforms. tw[e * TWS] = W_L^e
-// (conjugated for the inverse). Ends with a barrier.
-template <typename T, int L, int NTR, int TS, int ES, int NT, bool INV, int TWS>
+// at buf[t * TS + n * ES]. Lanes are transform-fastest (lane = t + NTR * sub-task); NT is a
+// multiple of NTR, so a lane keeps its transform across iterations. tw[n1 * R2 + k2] =
+// W_L^(n1 k2) (conjugated for the inverse). Ends with a barrier.
+template <typename T, int L, int NTR, int TS, int ES, int NT, bool INV>
 __device__ __forceinline__ void plane_dft(cx<T>* buf, const cx<T>* tw) {
     constexpr int R1 = plane_r1(L), R2 = L / R1;
     static_assert(R1 > 0 && R1 * R2 == L, "no split for this length");
-    int tid = plane_tid();
-    constexpr int NA = NTR * R1, IA = (NA + NT - 1) / NT;
+    static_assert(NT % NTR == 0, "workgroup size must be a multiple of the transform count");
+    constexpr int SPI = NT / NTR;   // sub-tasks per iteration
+    {
+        const int tid = plane_tid();
+        const int t = tid % NTR, s0 = tid / NTR;
+        constexpr int IA = (R1 + SPI - 1) / SPI;
 #pragma unroll
-    for (int it = 0; it < IA; ++it) {
-        const int q = tid + it * NT;
-        if (NA % NT == 0 || q < NA) {
-            const int t = q % NTR, n1 = q / NTR;
-            cx<T>* b = buf + t * TS + n1 * ES;
-            cx<T> v[R2];
+        for (int it = 0; it < IA; ++it) {
+            const int n1 = s0 + it * SPI;
+            if (R1 % SPI == 0 || n1 < R1) {
+                cx<T>* b = buf + t * TS + n1 * ES;
+                const cx<T>* w = tw + n1 * R2;
+                cx<T> v[R2];
 #pragma unroll
-            for (int n2 = 0; n2 < R2; ++n2) v[n2] = b[n2 * R1 * ES];
-            dftN<T, R2, INV>(v);
+                for (int n2 = 0; n2 < R2; ++n2) v[n2] = b[n2 * R1 * ES];
+                dftN<T, R2, INV>(v);
 #pragma unroll
-            for (int k2 = 1; k2 < R2; ++k2) {
-                const cx<T> w = tw[n1 * k2 * TWS];
-                v[k2] = cmul(v[k2], INV ? conjv(w) : w);
+                for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], INV ? conjv(w[k2]) : w[k2]);
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) b[k2 * R1 * ES] = v[k2];
             }
-#pragma unroll
-            for (int k2 = 0; k2 < R2; ++k2) b[k2 * R1 * ES] = v[k2];
         }
     }
     __syncthreads();
-    tid = plane_tid();
-    constexpr int NB = NTR * R2, IB = (NB + NT - 1) / NT;
-    cx<T> s[IB][R1];
+    {
+        const int tid = plane_tid();
+        const int t = tid % NTR, s0 = tid / NTR;
+        constexpr int IB = (R2 + SPI - 1) / SPI;
+        cx<T> s[IB][R1];
 #pragma unroll
-    for (int it = 0; it < IB; ++it) {
-        const int q = tid + it * NT;
-        if (NB % NT == 0 || q < NB) {
-            const int t = q % NTR, k2 = q / NTR;
-            const cx<T>* b = buf + t * TS + k2 * R1 * ES;
+        for (int it = 0; it < IB; ++it) {
+            const int k2 = s0 + it * SPI;
+            if (R2 % SPI == 0 || k2 < R2) {
+                const cx<T>* b = buf + t * TS + k2 * R1 * ES;
 #pragma unroll
-            for (int n1 = 0; n1 < R1; ++n1) s[it][n1] = b[n1 * ES];
-            dftN<T, R1, INV>(s[it]);
+                for (int n1 = 0; n1 < R1; ++n1) s[it][n1] = b[n1 * ES];
+                dftN<T, R1, INV>(s[it]);
+            }
         }
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int it = 0; it < IB; ++it) {
-        const int q = tid + it * NT;
-        if (NB % NT == 0 || q < NB) {
-            const int t = q % NTR, k2 = q / NTR;
-            cx<T>* b = buf + t * TS + k2 * ES;
+        for (int it = 0; it < IB; ++it) {
+            const int k2 = s0 + it * SPI;
+            if (R2 % SPI == 0 || k2 < R2) {
+                cx<T>* b = buf + t * TS + k2 * ES;
 #pragma unroll
-            for (int k1 = 0; k1 < R1; ++k1) b[k1 * R2 * ES] = s[it][k1];
+                for (int k1 = 0; k1 < R1; ++k1) b[k1 * R2 * ES] = s[it][k1];
+            }
         }
     }
     __syncthreads();
@@ -133,12 +151,26 @@ __device__ __forceinline__ void plane_dft(cx<T>* buf, const cx<T>* tw) {
 template <typename T, int LY, int LZ, int NT, bool INV>
 __device__ __forceinline__ void plane_cols(cx<T>* pl) {
     using P = PlaneShape<T, LY, LZ>;
-    plane_dft<T, LY, P::M, 1, P::PITCH, NT, INV, 1>(pl, pl + P::TWY);
+    plane_dft<T, LY, P::M, 1, P::PITCH, NT, INV>(pl, pl + P::TWY);
 }
 template <typename T, int LY, int LZ, int NT, bool INV>
 __device__ __forceinline__ void plane_rows(cx<T>* pl) {
     using P = PlaneShape<T, LY, LZ>;
-    plane_dft<T, P::M, LY, P::PITCH, 1, NT, INV, 2>(pl, pl + P::TWZ);   // W_M^e = W_LZ^(2e)
+    plane_dft<T, P::M, LY, P::PITCH, 1, NT, INV>(pl, pl + P::TWM);
+}
+
+// Task (r, k), k = 0 .. M/2, of the row splits: lanes row-fastest (NT a multiple of LY).
+template <int LY, int M, int NT, class F>
+__device__ __forceinline__ void plane_split_tasks(F&& f) {
+    static_assert(NT % LY == 0, "workgroup size must be a multiple of the rows");
+    constexpr int SPI = NT / LY, H = M / 2 + 1, IQ = (H + SPI - 1) / SPI;
+    const int tid = plane_tid();
+    const int r = tid % LY, k0 = tid / LY;
+#pragma unroll
+    for (int it = 0; it < IQ; ++it) {
+        const int k = k0 + it * SPI;
+        if (H % SPI == 0 || k < H) f(r, k);
+    }
 }
 
 // Inverse real transform of every row, before plane_rows<INV = true>: the packed half spectrum X
@@ -148,27 +180,21 @@ __device__ __forceinline__ void plane_rows(cx<T>* pl) {
 template <typename T, int LY, int LZ, int NT>
 __device__ __forceinline__ void plane_c2r_split(cx<T>* pl) {
     using P = PlaneShape<T, LY, LZ>;
-    constexpr int M = P::M, H = M / 2 + 1, NQ = LY * H, IQ = (NQ + NT - 1) / NT;
-    const cx<T>* twz = pl + P::TWZ;
-    const int tid = plane_tid();
-#pragma unroll
-    for (int it = 0; it < IQ; ++it) {
-        const int q = tid + it * NT;
-        if (NQ % NT == 0 || q < NQ) {
-            const int r = q % LY, k = q / LY;
-            cx<T>* row = pl + r * P::PITCH;
-            if (k == 0) {
-                const cx<T> a = row[0];
-                row[0] = cx<T>{a.x + a.y, a.x - a.y};
-            } else {
-                const cx<T> a = row[k], b = row[M - k];
-                const cx<T> e2{a.x + b.x, a.y - b.y};
-                const cx<T> o2 = cmul(cx<T>{a.x - b.x, a.y + b.y}, conjv(twz[k]));
-                row[k] = cx<T>{e2.x - o2.y, e2.y + o2.x};
-                if (k != M - k) row[M - k] = cx<T>{e2.x + o2.y, o2.x - e2.y};
-            }
+    constexpr int M = P::M;
+    const cx<T>* tws = pl + P::TWS;
+    plane_split_tasks<LY, M, NT>([&](int r, int k) {
+        cx<T>* row = pl + r * P::PITCH;
+        if (k == 0) {
+            const cx<T> a = row[0];
+            row[0] = cx<T>{a.x + a.y, a.x - a.y};
+        } else {
+            const cx<T> a = row[k], b = row[M - k];
+            const cx<T> e2{a.x + b.x, a.y - b.y};
+            const cx<T> o2 = cmul(cx<T>{a.x - b.x, a.y + b.y}, conjv(tws[k]));
+            row[k] = cx<T>{e2.x - o2.y, e2.y + o2.x};
+            if (k != M - k) row[M - k] = cx<T>{e2.x + o2.y, o2.x - e2.y};
         }
-    }
+    });
     __syncthreads();
 }
 
@@ -178,78 +204,78 @@ __device__ __forceinline__ void plane_c2r_split(cx<T>* pl) {
 template <typename T, int LY, int LZ, int NT>
 __device__ __forceinline__ void plane_r2c_split(cx<T>* pl) {
     using P = PlaneShape<T, LY, LZ>;
-    constexpr int M = P::M, H = M / 2 + 1, NQ = LY * H, IQ = (NQ + NT - 1) / NT;
-    const cx<T>* twz = pl + P::TWZ;
-    const int tid = plane_tid();
-#pragma unroll
-    for (int it = 0; it < IQ; ++it) {
-        const int q = tid + it * NT;
-        if (NQ % NT == 0 || q < NQ) {
-            const int r = q % LY, k = q / LY;
-            cx<T>* row = pl + r * P::PITCH;
-            if (k == 0) {
-                const cx<T> a = row[0];
-                row[0] = cx<T>{a.x + a.y, a.x - a.y};
-            } else {
-                const cx<T> a = row[k], b = row[M - k];
-                const cx<T> e{T(0.5) * (a.x + b.x), T(0.5) * (a.y - b.y)};
-                const cx<T> o{T(0.5) * (a.y + b.y), T(0.5) * (b.x - a.x)};
-                const cx<T> wo = cmul(twz[k], o);
-                row[k] = e + wo;
-                if (k != M - k) row[M - k] = conjv(e - wo);
-            }
+    constexpr int M = P::M;
+    const cx<T>* tws = pl + P::TWS;
+    plane_split_tasks<LY, M, NT>([&](int r, int k) {
+        cx<T>* row = pl + r * P::PITCH;
+        if (k == 0) {
+            const cx<T> a = row[0];
+            row[0] = cx<T>{a.x + a.y, a.x - a.y};
+        } else {
+            const cx<T> a = row[k], b = row[M - k];
+            const cx<T> e{T(0.5) * (a.x + b.x), T(0.5) * (a.y - b.y)};
+            const cx<T> o{T(0.5) * (a.y + b.y), T(0.5) * (b.x - a.x)};
+            const cx<T> wo = cmul(tws[k], o);
+            row[k] = e + wo;
+            if (k != M - k) row[M - k] = conjv(e - wo);
         }
-    }
+    });
     __syncthreads();
 }
 
-// HBM half-spectrum plane [LY][M + 1] -> LDS slots (columns 0 and M packed into slot 0).
-// The caller places a barrier before the slots are read.
+// Two complex values: the unit of the verbatim plane copies (16 B in float32).
+template <typename T> struct alignas(4 * sizeof(T)) CPair { cx<T> a, b; };
+
+// HBM half-spectrum plane [LY][M + 1] -> LDS (verbatim, the LDS rows have the HBM row length),
+// then slot 0 of every row packs columns 0 and M as G0 + i GN. The caller places a barrier
+// before the slots are read.
 template <typename T, int LY, int LZ, int NT>
 __device__ __forceinline__ void plane_load_spec(cx<T>* pl, const cx<T>* __restrict__ src) {
     using P = PlaneShape<T, LY, LZ>;
-    constexpr int M = P::M, NE = LY * P::MZH, IE = (NE + NT - 1) / NT;
-    const int tid = plane_tid();
-#pragma unroll 4
-    for (int it = 0; it < IE; ++it) {
-        const int i = tid + it * NT;
-        if (i < NE) {
-            const int r = i / P::MZH, k = i - r * P::MZH;
-            if (k < M) {
-                cx<T> v = src[i];
-                if (k == 0) {
-                    const cx<T> b = src[i + M];
-                    v = cx<T>{v.x - b.y, v.y + b.x};
-                }
-                pl[r * P::PITCH + k] = v;
-            }
+    constexpr int NP = LY * P::MZH / 2, IP = (NP + NT - 1) / NT;
+    static_assert((LY * P::MZH) % 2 == 0, "plane copies move complex pairs");
+    {
+        const int tid = plane_tid();
+        const CPair<T>* s = reinterpret_cast<const CPair<T>*>(src);
+        CPair<T>* d = reinterpret_cast<CPair<T>*>(pl);
+#pragma unroll
+        for (int it = 0; it < IP; ++it) {
+            const int i = tid + it * NT;
+            if (NP % NT == 0 || i < NP) d[i] = s[i];
         }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < LY; r += NT) {
+        const cx<T> g0 = pl[r * P::PITCH], gn = pl[r * P::PITCH + P::M];
+        pl[r * P::PITCH] = cx<T>{g0.x - gn.y, g0.y + gn.x};
     }
 }
 
-// LDS slots -> HBM half-spectrum plane, unpacking slot 0 into columns 0 and M:
-// G0 = (C[ky] + conj C[-ky]) / 2, GN = (C[ky] - conj C[-ky]) / 2i.
+// Slot 0 of the rows -> columns 0 and M, G0 = (C[ky] + conj C[-ky]) / 2 and
+// GN = (C[ky] - conj C[-ky]) / 2i (rows ky and LY - ky handled together), then the plane to HBM
+// (verbatim). The caller places a barrier before the plane is written again.
 template <typename T, int LY, int LZ, int NT>
-__device__ __forceinline__ void plane_store_spec(const cx<T>* pl, cx<T>* __restrict__ dst) {
+__device__ __forceinline__ void plane_store_spec(cx<T>* pl, cx<T>* __restrict__ dst) {
     using P = PlaneShape<T, LY, LZ>;
-    constexpr int M = P::M, NE = LY * P::MZH, IE = (NE + NT - 1) / NT;
-    const int tid = plane_tid();
-#pragma unroll 4
-    for (int it = 0; it < IE; ++it) {
-        const int i = tid + it * NT;
-        if (i < NE) {
-            const int r = i / P::MZH, k = i - r * P::MZH;
-            cx<T> v;
-            if (k == 0 || k == M) {
-                const cx<T> c = pl[r * P::PITCH];
-                const cx<T> m = pl[(r == 0 ? 0 : LY - r) * P::PITCH];
-                v = k == 0 ? cx<T>{T(0.5) * (c.x + m.x), T(0.5) * (c.y - m.y)}
-                           : cx<T>{T(0.5) * (c.y + m.y), T(0.5) * (m.x - c.x)};
-            } else {
-                v = pl[r * P::PITCH + k];
-            }
-            dst[i] = v;
+    constexpr int M = P::M, NP = LY * P::MZH / 2, IP = (NP + NT - 1) / NT;
+    for (int r = threadIdx.x; r <= LY / 2; r += NT) {
+        const int rm = r == 0 ? 0 : LY - r;
+        const cx<T> c = pl[r * P::PITCH], m = pl[rm * P::PITCH];
+        pl[r * P::PITCH] = cx<T>{T(0.5) * (c.x + m.x), T(0.5) * (c.y - m.y)};
+        pl[r * P::PITCH + M] = cx<T>{T(0.5) * (c.y + m.y), T(0.5) * (m.x - c.x)};
+        if (rm != r) {
+            pl[rm * P::PITCH] = cx<T>{T(0.5) * (m.x + c.x), T(0.5) * (m.y - c.y)};
+            pl[rm * P::PITCH + M] = cx<T>{T(0.5) * (m.y + c.y), T(0.5) * (c.x - m.x)};
         }
+    }
+    __syncthreads();
+    const int tid = plane_tid();
+    const CPair<T>* s = reinterpret_cast<const CPair<T>*>(pl);
+    CPair<T>* d = reinterpret_cast<CPair<T>*>(dst);
+#pragma unroll
+    for (int it = 0; it < IP; ++it) {
+        const int i = tid + it * NT;
+        if (NP % NT == 0 || i < NP) d[i] = s[i];
     }
 }
 
@@ -293,12 +319,12 @@ template <typename T, int LY, int LZ, int NT, int G = 2, class F>
 __device__ __forceinline__ void plane_elements(F&& f) {
     using P = PlaneShape<T, LY, LZ>;
     constexpr int NS = LY * P::M;
-    static_assert(NS % NT == 0, "workgroup size must divide the plane");
+    static_assert(NS % NT == 0 && NT % P::M == 0, "workgroup size must divide the plane and be a multiple of a row");
     const int tid = plane_tid();
+    const int m = tid % P::M, r0 = tid / P::M;
 #pragma unroll
     for (int it = 0; it < NS / NT; ++it) {
-        const int i = tid + it * NT;
-        const int r = i / P::M, m = i - r * P::M;
+        const int r = r0 + it * (NT / P::M);
         f(it, r, m, r * LZ + 2 * m);
         if ((it + 1) % G == 0) __builtin_amdgcn_sched_barrier(0);
     }
@@ -456,7 +482,8 @@ __device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs
     using P = PlaneShape<T, LY, LZ>;
     static_assert(MODE == 0 || MODE == 2, "plane density kernel: lossless or Stokes");
     constexpr int NV = LY * P::M / NT;
-    constexpr bool HOLD = !NL && MODE == 0;     // rho0 kept in registers across the axes
+    // rho0 kept in registers across the axes when the occupancy floor leaves room (>= 128 registers)
+    constexpr bool HOLD = !NL && MODE == 0 && plane_waves<T, LY, LZ, NT>() <= 4;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* pl = reinterpret_cast<cx<T>*>(smem);
     plane_twiddles<T, LY, LZ, NT>(pl, tab);
