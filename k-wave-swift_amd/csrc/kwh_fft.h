@@ -45,6 +45,41 @@ struct FftPlan {
     int radix[kMaxStages];
 };
 
+// Loads and stores with the nontemporal hint, for whole-line streaming copies (each lane moves 16
+// contiguous bytes, every line read or written once per pass). Measured on a 1 GiB device copy:
+// 5.46-5.77 TB/s plain, 5.93-6.31 TB/s with nontemporal loads and stores. Not for the register
+// kernels' 8-byte lane accesses, which the hint made 1.7x slower at 256^3 (their lines are completed
+// by other waves).
+#ifndef KWH_STREAM_HINTS
+#define KWH_STREAM_HINTS 1
+#endif
+template <typename U> __device__ __forceinline__ U gld(const U* p) {
+    if constexpr (KWH_STREAM_HINTS) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <typename U> __device__ __forceinline__ void gst(U* p, U v) {
+    if constexpr (KWH_STREAM_HINTS) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+template <typename T> __device__ __forceinline__ cx<T> gld(const cx<T>* p) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    if constexpr (KWH_STREAM_HINTS) {
+        const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+        return cx<T>{v.x, v.y};
+    } else {
+        return *p;
+    }
+}
+template <typename T> __device__ __forceinline__ void gst(cx<T>* p, cx<T> v) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    if constexpr (KWH_STREAM_HINTS) {
+        const v2 w = {v.x, v.y};
+        __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+    } else {
+        *p = v;
+    }
+}
+
 // cos / sin of 2*pi*m/16, m = 0..15 (exact decimal expansions of the standard constants).
 #define KWH_C16_1 0.92387953251128675613
 #define KWH_S16_1 0.38268343236508977173

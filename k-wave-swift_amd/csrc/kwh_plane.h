@@ -223,8 +223,18 @@ __device__ __forceinline__ void plane_r2c_split(cx<T>* pl) {
     __syncthreads();
 }
 
-// Two complex values: the unit of the verbatim plane copies (16 B in float32).
+// Two complex values: the unit of the verbatim plane copies (16 B in float32), moved with the
+// nontemporal hint (gld / gst, kwh_fft.h).
 template <typename T> struct alignas(4 * sizeof(T)) CPair { cx<T> a, b; };
+template <typename T> __device__ __forceinline__ CPair<T> gldp(const CPair<T>* p) {
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    const v4 v = gld(reinterpret_cast<const v4*>(p));
+    return CPair<T>{{v.x, v.y}, {v.z, v.w}};
+}
+template <typename T> __device__ __forceinline__ void gstp(CPair<T>* p, const CPair<T>& v) {
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    gst(reinterpret_cast<v4*>(p), v4{v.a.x, v.a.y, v.b.x, v.b.y});
+}
 
 // HBM half-spectrum plane [LY][M + 1] -> LDS (verbatim, the LDS rows have the HBM row length),
 // then slot 0 of every row packs columns 0 and M as G0 + i GN. The caller places a barrier
@@ -235,13 +245,18 @@ __device__ __forceinline__ void plane_load_spec(cx<T>* pl, const cx<T>* __restri
     constexpr int NP = LY * P::MZH / 2, IP = (NP + NT - 1) / NT;
     static_assert((LY * P::MZH) % 2 == 0, "plane copies move complex pairs");
     {
+        // every load issued before the first LDS write (clamped index instead of a branch, so the
+        // loads are not serialised behind per-iteration waits)
         const int tid = plane_tid();
         const CPair<T>* s = reinterpret_cast<const CPair<T>*>(src);
         CPair<T>* d = reinterpret_cast<CPair<T>*>(pl);
+        CPair<T> v[IP];
+#pragma unroll
+        for (int it = 0; it < IP; ++it) v[it] = gldp(s + min(tid + it * NT, NP - 1));
 #pragma unroll
         for (int it = 0; it < IP; ++it) {
             const int i = tid + it * NT;
-            if (NP % NT == 0 || i < NP) d[i] = s[i];
+            if (NP % NT == 0 || i < NP) d[i] = v[it];
         }
     }
     __syncthreads();
@@ -272,10 +287,13 @@ __device__ __forceinline__ void plane_store_spec(cx<T>* pl, cx<T>* __restrict__ 
     const int tid = plane_tid();
     const CPair<T>* s = reinterpret_cast<const CPair<T>*>(pl);
     CPair<T>* d = reinterpret_cast<CPair<T>*>(dst);
+    CPair<T> v[IP];
+#pragma unroll
+    for (int it = 0; it < IP; ++it) v[it] = s[min(tid + it * NT, NP - 1)];
 #pragma unroll
     for (int it = 0; it < IP; ++it) {
         const int i = tid + it * NT;
-        if (NP % NT == 0 || i < NP) d[i] = s[i];
+        if (NP % NT == 0 || i < NP) gstp(d + i, v[it]);
     }
 }
 
@@ -328,6 +346,39 @@ __device__ __forceinline__ void plane_elements(F&& f) {
         f(it, r, m, r * LZ + 2 * m);
         if ((it + 1) % G == 0) __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// Element loop in groups of G iterations whose global loads (load(it, r, m, c) -> V) are all issued
+// before the group's updates and stores (body(it, r, m, c, V)): a store to one array may alias a later
+// load of another as far as the compiler knows, so interleaved code waits out a full memory latency
+// per iteration.
+template <typename T, int LY, int LZ, int NT, int G, class LD, class F>
+__device__ __forceinline__ void plane_elements_lf(LD&& load, F&& body) {
+    using P = PlaneShape<T, LY, LZ>;
+    constexpr int NS = LY * P::M, NI = NS / NT;
+    static_assert(NS % NT == 0 && NT % P::M == 0 && NI % G == 0, "workgroup size must divide the plane");
+    const int tid = plane_tid();
+    const int m = tid % P::M, r0 = tid / P::M;
+    using V = decltype(load(0, 0, 0, 0));
+#pragma unroll
+    for (int g0 = 0; g0 < NI; g0 += G) {
+        V v[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int r = r0 + (g0 + k) * (NT / P::M);
+            v[k] = load(g0 + k, r, m, r * LZ + 2 * m);
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int r = r0 + (g0 + k) * (NT / P::M);
+            body(g0 + k, r, m, r * LZ + 2 * m, v[k]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+template <int LY, int M, int NT> constexpr int plane_group() {
+    constexpr int NI = LY * M / NT;
+    return NI % 4 == 0 ? 4 : (NI % 2 == 0 ? 2 : 1);
 }
 
 // Sparse sources of rows [x LY, (x+1) LY) into the real view (dirichlet or additive).
@@ -433,10 +484,16 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
         const T* pm = tab.pmlsg[ax];
         const bool sp = sparse_in_plane(a.sp[ax].rowptr, x, LY);
         const T qx = pm[ax == 0 ? x : 0];
-        plane_elements<T, LY, LZ, NT>([&](int, int r, int m, int c) {
+        struct VL { Pair<T> uu, dt, cr; };
+        plane_elements_lf<T, LY, LZ, NT, plane_group<LY, P::M, NT>()>([&](int, int, int, int c) {
+            VL l;
+            l.uu = ld2(u + c);
+            l.dt = ld2m(dtr, rb + c);
+            if constexpr (CORR) l.cr = ld2m(corr, rb + c);
+            return l;
+        }, [&](int, int r, int m, int c, const VL& l) {
             const cx<T> gv = pl[r * P::PITCH + m];
-            const Pair<T> uu = ld2(u + c);
-            const Pair<T> dt = ld2m(dtr, rb + c);
+            const Pair<T> uu = l.uu, dt = l.dt;
             T q0, q1;
             if constexpr (ax == 0) { q0 = qx; q1 = qx; }
             else if constexpr (ax == 1) { q0 = pm[r]; q1 = q0; }
@@ -444,9 +501,8 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
             T v0 = q0 * (q0 * uu.a - dt.a * gv.x);
             T v1 = q1 * (q1 * uu.b - dt.b * gv.y);
             if constexpr (CORR) {
-                const Pair<T> cr = ld2m(corr, rb + c);
-                v0 += cr.a;
-                v1 += cr.b;
+                v0 += l.cr.a;
+                v1 += l.cr.b;
             }
             pl[r * P::PITCH + m] = cx<T>{v0, v1};
             if (!sp) st2(u + c, v0, v1);
@@ -519,12 +575,17 @@ __device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs
         const T cf = a.corr_factor[ax];
         const bool sp = sparse_in_plane(a.sp[ax].rowptr, x, LY);
         const T qx = pm[ax == 0 ? x : 0];
-        plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int c) {
+        struct DL { Pair<T> r0, ro, cr; };
+        plane_elements_lf<T, LY, LZ, NT, plane_group<LY, P::M, NT>()>([&](int it, int, int, int c) {
+            DL l;
+            if constexpr (HOLD) l.r0 = Pair<T>{r0h[it].x, r0h[it].y};
+            else l.r0 = ld2m(a.rho0, rb + c);
+            l.ro = ld2(rho + c);
+            if constexpr (CORR) l.cr = ld2m(a.corr, rb + c);
+            return l;
+        }, [&](int it, int r, int m, int c, const DL& l) {
             const cx<T> du = pl[r * P::PITCH + m];
-            Pair<T> r0;
-            if constexpr (HOLD) r0 = Pair<T>{r0h[it].x, r0h[it].y};
-            else r0 = ld2m(a.rho0, rb + c);
-            const Pair<T> ro = ld2(rho + c);
+            const Pair<T> r0 = l.r0, ro = l.ro;
             T q0, q1;
             if constexpr (ax == 0) { q0 = qx; q1 = qx; }
             else if constexpr (ax == 1) { q0 = pm[r]; q1 = q0; }
@@ -542,9 +603,8 @@ __device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs
             T v0 = q0 * (q0 * ro.a - m0);
             T v1 = q1 * (q1 * ro.b - m1);
             if constexpr (CORR) {
-                const Pair<T> cr = ld2m(a.corr, rb + c);
-                v0 += cr.a * cf;
-                v1 += cr.b * cf;
+                v0 += l.cr.a * cf;
+                v1 += l.cr.b * cf;
             }
             if (sp) {
                 pl[r * P::PITCH + m] = cx<T>{v0, v1};
@@ -569,24 +629,30 @@ __device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs
     });
     // EOS into the real view, then P = R2C_yz(p)
     T* p = a.p + rb;
-    plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int c) {
+    struct EL { Pair<T> r0, tu, bn, cc; };
+    plane_elements_lf<T, LY, LZ, NT, plane_group<LY, P::M, NT>()>([&](int it, int, int, int c) {
+        EL l;
+        l.r0 = Pair<T>{T(1), T(1)};
+        if constexpr (NL) {
+            if constexpr (HOLD) l.r0 = Pair<T>{r0h[it].x, r0h[it].y};
+            else l.r0 = ld2m(a.rho0, rb + c);
+            l.bn = ld2m(a.bona, rb + c);
+        }
+        if constexpr (MODE == 2) l.tu = ld2m(a.tau, rb + c);
+        l.cc = ld2m(a.c0, rb + c);
+        return l;
+    }, [&](int it, int r, int m, int c, const EL& l) {
         T e0 = rt[it].x, e1 = rt[it].y;
-        Pair<T> r0{T(1), T(1)};
-        if constexpr (NL) {
-            if constexpr (HOLD) r0 = Pair<T>{r0h[it].x, r0h[it].y};
-            else r0 = ld2m(a.rho0, rb + c);
-        }
+        const Pair<T> r0 = l.r0;
         if constexpr (MODE == 2) {
-            const Pair<T> tu = ld2m(a.tau, rb + c);
-            e0 += tu.a * sd[it].x;
-            e1 += tu.b * sd[it].y;
+            e0 += l.tu.a * sd[it].x;
+            e1 += l.tu.b * sd[it].y;
         }
         if constexpr (NL) {
-            const Pair<T> bn = ld2m(a.bona, rb + c);
-            e0 += bn.a * (rt[it].x * rt[it].x) / (T(2) * r0.a);
-            e1 += bn.b * (rt[it].y * rt[it].y) / (T(2) * r0.b);
+            e0 += l.bn.a * (rt[it].x * rt[it].x) / (T(2) * r0.a);
+            e1 += l.bn.b * (rt[it].y * rt[it].y) / (T(2) * r0.b);
         }
-        const Pair<T> cc = ld2m(a.c0, rb + c);
+        const Pair<T> cc = l.cc;
         const T p0 = cc.a * cc.a * e0, p1 = cc.b * cc.b * e1;
         st2(p + c, p0, p1);
         pl[r * P::PITCH + m] = cx<T>{p0, p1};
