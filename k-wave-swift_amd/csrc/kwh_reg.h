@@ -1117,167 +1117,4 @@ __global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_MIDDLE_WAVES)))
     reg_middle_body<T, J, E, NTR>(bidx_here(), a, tab, g);
 }
 
-// ------------------------------------------------------------------------------------------
-// Middle pass with more lanes per column than J | E allows (160 = 8 x 20, 96 = 8 x 12): half the
-// registers per lane of the J = 4 split. The forward leaves lane j holding X[k2 + E k1] for
-// k2 = j + J a (a < A = ceil(E / J), k2 < E) and k1 < J -- the transposed order, in which the
-// operators are applied -- and the inverse runs the two steps in reverse: DFT_J^-1 over k1 per k2,
-// conjugate twiddles, the LDS transpose back, DFT_E^-1 per lane, which returns the natural lane
-// pattern n = j + J m. LDS slots are those of RegCol: (n1, k2) at n1 (E NTR + 1) + k2 NTR + w.
-// ------------------------------------------------------------------------------------------
-template <typename T, int J, int E, int NTR>
-struct RegColT {
-    static constexpr int L = J * E;
-    static constexpr int A = (E + J - 1) / J;
-    __device__ static __forceinline__ void forward(cx<T>* v, cx<T> (&X)[A][J], cx<T>* lds, int j, int w,
-                                                   const LaneTw<T, J, E>& tw) {
-        dftN<T, E, false>(v);
-        cx<T>* wr = lds + (j * E * NTR + w + j);
-#pragma unroll
-        for (int r = 0; r < E; ++r) wr[r * NTR] = v[r];
-        __syncthreads();
-        const cx<T>* rd = lds + (j * NTR + w);
-#pragma unroll
-        for (int a = 0; a < A; ++a) {
-            if ((a + 1) * J <= E || J * a + j < E) {
-                cx<T> s[J];
-#pragma unroll
-                for (int n1 = 0; n1 < J; ++n1) {
-                    cx<T> x = rd[n1 * (E * NTR + 1) + J * a * NTR];
-                    if (n1 > 0) x = cmul(x, tw.w(a, n1));
-                    s[n1] = x;
-                }
-                dftN<T, J, false>(s);
-#pragma unroll
-                for (int k1 = 0; k1 < J; ++k1) X[a][k1] = s[k1];
-            }
-        }
-        __syncthreads();
-    }
-    __device__ static __forceinline__ void inverse(cx<T> (&X)[A][J], cx<T>* v, cx<T>* lds, int j, int w,
-                                                   const LaneTw<T, J, E>& tw) {
-        cx<T>* wr = lds + (j * NTR + w);
-#pragma unroll
-        for (int a = 0; a < A; ++a) {
-            if ((a + 1) * J <= E || J * a + j < E) {
-                dftN<T, J, true>(X[a]);
-#pragma unroll
-                for (int n1 = 0; n1 < J; ++n1) {
-                    cx<T> x = X[a][n1];
-                    if (n1 > 0) x = cmul(x, conjv(tw.w(a, n1)));
-                    wr[n1 * (E * NTR + 1) + J * a * NTR] = x;
-                }
-            }
-        }
-        __syncthreads();
-        const cx<T>* rd = lds + (j * E * NTR + w + j);
-#pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = rd[r * NTR];
-        dftN<T, E, true>(v);
-        __syncthreads();
-    }
-};
-
-template <typename T, int J, int E, int NTR>
-__device__ __forceinline__ void reg_middle_t_body(const BIdx& bi, const MidArgs& a, const AxisTabs<T>& tab,
-                                                  const Geo& g) {
-    using S = RegColT<T, J, E, NTR>;
-    constexpr int A = S::A;
-    extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
-    cx<T>* lds = ldsb + S::L;
-    const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
-    LaneTw<T, J, E> lt;
-    lt.load(ldsb, tab.tw[a.axis], j);
-    const int64_t c = (int64_t)bi.tile() * NTR + w;
-    const bool valid = c < a.ncols;
-    const int64_t cc = valid ? c : 0;
-    const int64_t base = (cc / a.inner) * (int64_t)S::L * a.inner + cc % a.inner + (int64_t)j * a.stride;
-    const int64_t step = (int64_t)J * a.stride;
-    const T scale = T(a.scale);
-    int i0c, i1c, i2c;
-    if (a.axis == 0) { i0c = 0; i1c = (int)(cc / g.mzh); i2c = (int)(cc % g.mzh); }
-    else { i0c = (int)(cc / g.mzh); i1c = 0; i2c = (int)(cc % g.mzh); }
-    const int64_t hstride = a.axis == 0 ? (int64_t)g.m1 * g.mzh : (int64_t)g.mzh;
-    // row k = j + J a + E k1 of the transposed order
-    const int64_t hbase = (a.axis == 0 ? cc : (int64_t)i0c * g.m1 * g.mzh + i2c) + (int64_t)j * hstride;
-    T kap[A][J];
-    const T* kap_src = nullptr;
-    for (int f = 0; f < a.nin; ++f) {
-        cx<T> v[E];
-        {
-            const cx<T>* q = reinterpret_cast<const cx<T>*>(a.in[f]) + base;
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                v[m] = *q;
-                q += step;
-            }
-        }
-        const T* ktab = nullptr;
-        for (int o = 0; o < a.nop; ++o)
-            if (a.op[o].in == f && a.op[o].table) { ktab = reinterpret_cast<const T*>(a.op[o].table); break; }
-        if (ktab && ktab != kap_src) {
-#pragma unroll
-            for (int aa = 0; aa < A; ++aa)
-#pragma unroll
-                for (int k1 = 0; k1 < J; ++k1)
-                    if ((aa + 1) * J <= E || J * aa + j < E)
-                        kap[aa][k1] = ktab[hbase + (int64_t)(J * aa + E * k1) * hstride] * scale;
-            kap_src = ktab;
-        }
-        cx<T> X[A][J];
-        S::forward(v, X, lds, j, w, lt);
-        for (int o = 0; o < a.nop; ++o) {
-            const MidOp op = a.op[o];
-            if (op.in != f) continue;
-            const bool use_kappa = op.table != nullptr;
-            const bool use_vec = op.kind != OP_TAB;
-            const cx<T>* tab1 = op.kind == OP_DDXP_KAPPA || op.kind == OP_DDXP ? tab.ddxp[op.axis]
-                              : op.kind == OP_DDXN_KAPPA ? tab.ddxn[op.axis] : tab.coll[op.axis];
-            const bool vec_on_mid = op.axis == a.axis;
-            const int fixed_idx = op.axis == 0 ? i0c : (op.axis == 1 ? i1c : i2c);
-            const cx<T> fixed_vec = use_vec && !vec_on_mid ? tab1[fixed_idx] : cx<T>{T(1), T(0)};
-            const cx<T>* tj = tab1 + j;
-            cx<T> Y[A][J];
-            // the operator's kind is uniform over the workgroup: one branch-free element loop per kind
-            if (use_vec && vec_on_mid) {
-#pragma unroll
-                for (int aa = 0; aa < A; ++aa)
-#pragma unroll
-                    for (int k1 = 0; k1 < J; ++k1) {
-                        const cx<T> mv = tj[J * aa + E * k1];
-                        Y[aa][k1] = cmul(X[aa][k1], use_kappa ? mv * kap[aa][k1] : mv * scale);
-                    }
-            } else if (use_kappa) {
-#pragma unroll
-                for (int aa = 0; aa < A; ++aa)
-#pragma unroll
-                    for (int k1 = 0; k1 < J; ++k1) Y[aa][k1] = cmul(X[aa][k1], fixed_vec * kap[aa][k1]);
-            } else {
-                const cx<T> fv = fixed_vec * scale;
-#pragma unroll
-                for (int aa = 0; aa < A; ++aa)
-#pragma unroll
-                    for (int k1 = 0; k1 < J; ++k1) Y[aa][k1] = cmul(X[aa][k1], fv);
-            }
-            S::inverse(Y, v, lds, j, w, lt);
-            if (valid) {
-                cx<T>* q = reinterpret_cast<cx<T>*>(op.out) + base;
-#pragma unroll
-                for (int m = 0; m < E; ++m) {
-                    *q = v[m];
-                    q += step;
-                }
-            }
-        }
-    }
-}
-#ifndef KWH_MIDDLE_T_WAVES
-#define KWH_MIDDLE_T_WAVES 2
-#endif
-template <typename T, int J, int E, int NTR>
-__global__ void __launch_bounds__(256, KWH_MIDDLE_T_WAVES) reg_middle_t(MidArgs a, AxisTabs<T> tab, Geo g) {
-    reg_middle_t_body<T, J, E, NTR>(bidx_here(), a, tab, g);
-}
-
 }  // namespace kwh

@@ -35,10 +35,6 @@ struct RegOps {
     // column kernels over tiles of ntr columns (ntr_max or ntr_max / 2), ntr * J lanes
     void (*strided)(const Emit&, dim3, dim3, size_t, const StridedArgs&, const cx<T>*, bool inv, int ntr);
     void (*middle)(const Emit&, dim3, dim3, size_t, const MidArgs&, const AxisTabs<T>&, const Geo&, int ntr);
-    // middle pass with tJ > J lanes per column, tJ not dividing E (reg_middle_t: 160 = 8 x 20,
-    // 96 = 8 x 12), tiles of tNTR columns; null where the length has none
-    int tJ = 0, tNTR = 0;
-    void (*middle_t)(const Emit&, dim3, dim3, size_t, const MidArgs&, const AxisTabs<T>&, const Geo&) = nullptr;
 };
 
 // Launchers for axis length L, or nullptr when no register-resident kernel covers it.

@@ -99,24 +99,4 @@ struct RegImpl {
     }
 };
 
-// The middle pass with J2 lanes per column (J2 not dividing E2) added to the table of length
-// J2 * E2.
-template <typename T, int J2, int E2>
-struct RegMidT {
-    static constexpr int NTR = reg_ntr_max<T, J2, E2>();
-    static void middle(const Emit& em, dim3 gr, dim3 bl, size_t lds, const MidArgs& a, const AxisTabs<T>& tab,
-                       const Geo& g) {
-        em.go<Body<&reg_middle_t_body<T, J2, E2, NTR>, &reg_middle_t<T, J2, E2, NTR>, KWH_MIDDLE_T_WAVES, false>>(
-            gr, bl, lds, a, tab, g);
-    }
-};
-template <class Impl, typename T, int J2, int E2>
-RegOps<T> with_middle_t() {
-    RegOps<T> o = *Impl::ops();
-    o.tJ = J2;
-    o.tNTR = RegMidT<T, J2, E2>::NTR;
-    o.middle_t = &RegMidT<T, J2, E2>::middle;
-    return o;
-}
-
 }  // namespace kwh

@@ -404,8 +404,6 @@ struct SimT final : kwh_sim {
     struct Schedule { int ntr_shift = 0, p_shift = 0; size_t lds = 64 * 1024; int min_blocks = 1024; };
     Schedule sched;
     bool use_sq = true;
-    // middle passes with more lanes per column where the length has them (RegOps::middle_t)
-    bool use_middle_t = true;
 
     // axisymmetric solver (kspaceFirstOrderAS, KSpaceFirstOrderAS.swift:17-309): the fields live
     // on the physical [nx][ny] grid (x axial, y radial); every spectral operation mirrors its
@@ -612,23 +610,6 @@ struct SimT final : kwh_sim {
         }
         const int ntab = (int)tabs_used.size();
         const double bytes = spec_bytes() * (a.nin + a.nop) + (double)g.NH * sizeof(T) * ntab;
-        if (o && o->middle_t && use_middle_t) {
-            for (int f = 0; f < a.nin; ++f) {   // one k-space table per input (loaded once in-kernel)
-                const void* t0 = nullptr;
-                for (auto& op : ops)
-                    if (op.in == f && op.table) {
-                        KWH_REQUIRE(!t0 || t0 == op.table, KWH_EUNSUPPORTED, "middle pass: two tables for one input");
-                        t0 = op.table;
-                    }
-            }
-            const int ntr = o->tNTR, L = o->tJ * (o->J * o->E / o->tJ);
-            const unsigned blocks = (unsigned)((c.ncols + ntr - 1) / ntr);
-            const size_t lds = ((size_t)L * ntr + o->tJ + L) * sizeof(C);
-            timed(kid, bytes, [&] {
-                o->middle_t(em(), dim3(blocks), dim3(ntr * o->tJ), lds, a, tab, g);
-            });
-            return;
-        }
         if (o) {
             for (int f = 0; f < a.nin; ++f) {   // one k-space table per input (loaded once in-kernel)
                 const void* t0 = nullptr;
@@ -754,7 +735,6 @@ struct SimT final : kwh_sim {
         mid_axis = g.m0 > 1 ? 0 : 1;
         plain1 = mid_axis == 0 && g.m1 > 1;
         if (const char* e = std::getenv("KWH_GENERIC_FFT")) use_sq = std::atoi(e) == 0;
-        if (const char* e = std::getenv("KWH_MIDDLE_T")) use_middle_t = std::atoi(e) != 0;
         {
             const int mm[3] = {g.m0, g.m1, g.m2};
             for (int ax = 0; ax < 3; ++ax)
