@@ -299,8 +299,8 @@ int kwh_sim_vjp(kwh_sim* sim, const void* dl_dp, int32_t checkpoint_every, void*
  *   KWH_BATCH_GRAPH    each sim replays its captured step graph on its own stream, the sims
  *                      interleaved step by step;
  *   KWH_BATCH_FUSED    one launch per pass of the step covers every sim whose step has the same
- *                      launch sequence and launch geometry (the sims of one grid shape); the
- *                      shapes of one step run one after another on the batch's stream; with
+ *                      launch sequence and launch geometry (the sims of one grid shape); each
+ *                      grid shape runs on a stream of its own, overlapping the others; with
  *                      KWH_BATCH_GRAPH each shape's batched step is captured once and replayed;
  *   KWH_BATCH_PLANE    the sims that plane mode covers run in it (kwh_sim_set_plane); without
  *                      the flag every sim runs the split passes.

@@ -572,8 +572,8 @@ class SimulationBatch:
     """Independent simulations of uneven sizes advanced together on one device (kwh_batch_*):
     per-scan-line transmits, for instance. `sims[i]` reads outputs like a Simulation.
 
-    run(fused=True): one launch per pass of the time step covers all sims of one grid shape, the
-    shapes running one after another on the batch's stream; with use_graph each shape's batched
+    run(fused=True): one launch per pass of the time step covers all sims of one grid shape, each
+    shape on a stream of its own (overlapping the others); with use_graph each shape's batched
     step is captured once and replayed.
     run(fused=False): each sim on its own stream, replaying its own step graphs (use_graph) or
     enqueuing its steps eagerly.
