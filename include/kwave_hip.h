@@ -241,15 +241,17 @@ int kwh_sim_set_nonfinite_check(kwh_sim* sim, int32_t every);
  * k-space-corrected sources, captured on first use; results are bitwise those of eager runs. */
 int kwh_sim_set_graph(kwh_sim* sim, int enable);
 
-/* Plane mode (1 = on; no counterpart in the reference, whose MLX transforms are whole-array
- * calls, KSpaceFirstOrder.swift:647-686): 3-D grids whose (y, z) plane is 96 x 96 or 128 x 128 run
- * the y and z transforms of each field in one kernel per x plane (the plane held in LDS), so a
- * step reads and writes each field once for the y and z axes together instead of once per axis.
- * Covers lossless and Stokes media, every source kind and pressure recording; returns
- * KWH_EUNSUPPORTED for other configurations (FFT power-law absorption, collocated velocity
- * output, checkpointing / the adjoint) and other plane shapes. May be switched between steps.
- * Results agree with the split passes to rounding (the transforms are factored differently);
- * a sim is deterministic in either mode. */
+/* Plane mode (no counterpart in the reference, whose MLX transforms are whole-array calls,
+ * KSpaceFirstOrder.swift:647-686): 3-D grids whose (y, z) plane is 96 x 96 or 128 x 128 run the y
+ * and z transforms of each field in one kernel per x plane (the plane held in LDS), so a step
+ * reads and writes each field once for the y and z axes together instead of once per axis. It
+ * covers lossless and Stokes media, every source kind and pressure recording, and is ON BY DEFAULT
+ * wherever it applies (environment KWH_PLANE=0 turns the default off); checkpointing and the
+ * adjoint switch the sim to the split passes while they are set. enable = 0 selects the split
+ * passes; enable = 1 returns KWH_EUNSUPPORTED for configurations plane mode does not cover (FFT
+ * power-law absorption, collocated velocity output, checkpointing, other plane shapes). May be
+ * switched between steps. Results agree with the split passes to rounding (the transforms are
+ * factored differently); a sim is deterministic in either mode. */
 int kwh_sim_set_plane(kwh_sim* sim, int enable);
 
 /* Receive combination on the device (combineSensorData, KWaveTransducer.swift:263-289):
@@ -302,8 +304,9 @@ int kwh_sim_vjp(kwh_sim* sim, const void* dl_dp, int32_t checkpoint_every, void*
  *                      launch sequence and launch geometry (the sims of one grid shape); each
  *                      grid shape runs on a stream of its own, overlapping the others; with
  *                      KWH_BATCH_GRAPH each shape's batched step is captured once and replayed;
- *   KWH_BATCH_PLANE    the sims that plane mode covers run in it (kwh_sim_set_plane); without
- *                      the flag every sim runs the split passes.
+ *   KWH_BATCH_PLANE    switch every sim that plane mode covers into it (kwh_sim_set_plane; it is
+ *                      the default where it applies, the flag restores it for sims switched
+ *                      off); without the flag every sim keeps its own mode.
  * Every mode gives results bitwise equal to running the sims one by one in the same plane mode. Outputs are read per sim
  * (kwh_batch_read_output, or the borrowed handle from kwh_batch_sim with the kwh_sim_* readers). */
 #define KWH_BATCH_GRAPH 1

@@ -397,6 +397,7 @@ struct SimT final : kwh_sim {
     // and z passes of a step inside them, so spec[0] carries R2C_yz(p) between steps
     const PlaneOps<T>* px = nullptr;
     bool plane = false;
+    bool plane_wanted = true;   // on by default where plane mode applies (KWH_PLANE=0: off)
     // launch schedule chosen by the autotuner: halve the tile columns of the strided / middle
     // passes (ntr_shift), halve the row pairs per workgroup of the row kernels (p_shift), LDS
     // budget of the generic Stockham passes and the workgroup count below which they take fewer
@@ -740,7 +741,8 @@ struct SimT final : kwh_sim {
             for (int ax = 0; ax < 3; ++ax)
                 rx[ax] = use_sq && mm[ax] > 1 ? reg_ops<T>(mm[ax]) : nullptr;
         }
-        if (dim == 3 && !axisym && mid_axis == 0) {
+        if (const char* e = std::getenv("KWH_PLANE")) plane_wanted = std::atoi(e) != 0;
+        if (dim == 3 && !axisym && mid_axis == 0 && use_sq) {
             px = plane_ops<T>(g.m1, g.m2);
             int lds_max = 0;
             HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -786,6 +788,7 @@ struct SimT final : kwh_sim {
         build_sources(d);
         build_sensor(d);
         allocate_state(d);
+        plane = plane_wanted && plane_supported();   // spec[0] is zero: no conversion
         HIPCHK(hipStreamSynchronize(stream));
     }
 
@@ -1767,6 +1770,12 @@ struct SimT final : kwh_sim {
         KWH_REQUIRE(!on || plane_supported(), KWH_EUNSUPPORTED,
                     "plane kernels need a 3-D grid with a 96x96 or 128x128 (y, z) plane, lossless or Stokes media, "
                     "no collocated velocity output and no checkpointing");
+        plane_wanted = on;
+        apply_plane(on);
+    }
+    // Plane mode follows the request wherever it applies (checkpointing switches it off).
+    void refresh_plane() { apply_plane(plane_wanted && plane_supported()); }
+    void apply_plane(bool on) {
         if (on == plane) return;
         HIPCHK(hipSetDevice(dev));
         sync();
@@ -2265,9 +2274,9 @@ struct SimT final : kwh_sim {
     void set_checkpointing(int every) override {
         KWH_REQUIRE(every >= 0, KWH_EINVAL, "checkpoint interval must be >= 0");
         KWH_REQUIRE(!axisym || every == 0, KWH_EUNSUPPORTED, "the axisymmetric solver has no adjoint");
-        KWH_REQUIRE(!plane || every == 0, KWH_EUNSUPPORTED, "checkpointing runs with the split passes (plane mode off)");
         sync();
         ckpt_every = every;
+        refresh_plane();   // checkpoints and the adjoint run with the split passes
         ckpt_have.clear();
         ckpt.alloc(0);
         if (every > 0) {
@@ -2369,7 +2378,6 @@ struct SimT final : kwh_sim {
         KWH_REQUIRE(dldy && grad_p0 && grad_c0, KWH_EINVAL, "NULL argument");
         KWH_REQUIRE(every > 0, KWH_EINVAL, "checkpoint interval must be > 0");
         KWH_REQUIRE(!axisym, KWH_EUNSUPPORTED, "the axisymmetric solver has no adjoint");
-        KWH_REQUIRE(!plane, KWH_EUNSUPPORTED, "the adjoint runs with the split passes (plane mode off)");
         KWH_REQUIRE(has_p0, KWH_EUNSUPPORTED, "the gradient is defined for the p0 problem");
         KWH_REQUIRE(!absorbing && !nonlinear, KWH_EUNSUPPORTED, "the adjoint covers the lossless linear loop");
         KWH_REQUIRE(!psrc.present && !usrc[0].present && !usrc[1].present && !usrc[2].present, KWH_EUNSUPPORTED,
@@ -3142,8 +3150,9 @@ struct kwh_batch {
     void run(int mode) {
         HIPCHK(hipSetDevice(dev));
         for (kwh_sim* s : sims) s->sync();
-        for (kwh_sim* s : sims)
-            if (s->plane_available() || s->plane_mode()) s->set_plane((mode & KWH_BATCH_PLANE) != 0);
+        if (mode & KWH_BATCH_PLANE)
+            for (kwh_sim* s : sims)
+                if (s->plane_available()) s->set_plane(true);
         if (mode & KWH_BATCH_FUSED) {
             run_fused((mode & KWH_BATCH_GRAPH) != 0);
             return;

@@ -477,7 +477,8 @@ class Simulation:
 
     def set_plane(self, on: bool):
         """Plane mode (kwh_sim_set_plane): the y and z transforms of each field in one kernel per
-        x plane; 3-D grids with a 96x96 or 128x128 (y, z) plane, lossless or Stokes media."""
+        x plane; 3-D grids with a 96x96 or 128x128 (y, z) plane, lossless or Stokes media. On by
+        default wherever it applies; set_plane(False) selects the split passes."""
         L.check(self.lib.kwh_sim_set_plane(self.handle, 1 if on else 0))
 
     def element_traces(self, field: int, element, delay, n_elements: int, scale: float) -> np.ndarray:
@@ -577,7 +578,8 @@ class SimulationBatch:
     step is captured once and replayed.
     run(fused=False): each sim on its own stream, replaying its own step graphs (use_graph) or
     enqueuing its steps eagerly.
-    run(plane=True): the sims that plane mode covers (Simulation.set_plane) run in it.
+    run(plane=True): switch every sim that plane mode covers into it (Simulation.set_plane; the
+    default where it applies); with plane=False every sim keeps its own mode.
     All modes give results bitwise equal to single runs in the same plane mode."""
 
     def __init__(self, cases, device: int = 0):

@@ -41,8 +41,7 @@ def metrics(a, b):
 def run(args, plane, graph=False, chunks=None):
     sim = Simulation(*args)
     try:
-        if plane:
-            sim.set_plane(True)
+        sim.set_plane(bool(plane))   # plane mode is the default where it applies
         if graph:
             sim.set_graph(True)
         if chunks:
