@@ -242,7 +242,7 @@ int kwh_sim_set_nonfinite_check(kwh_sim* sim, int32_t every);
 int kwh_sim_set_graph(kwh_sim* sim, int enable);
 
 /* Plane mode (no counterpart in the reference, whose MLX transforms are whole-array calls,
- * KSpaceFirstOrder.swift:647-686): 3-D grids whose (y, z) plane is 96 x 96 or 128 x 128 run the y
+ * KSpaceFirstOrder.swift:647-686): 3-D grids whose (y, z) plane is 32 x 32, 64 x 64, 96 x 96 or 128 x 128 run the y
  * and z transforms of each field in one kernel per x plane (the plane held in LDS), so a step
  * reads and writes each field once for the y and z axes together instead of once per axis. It
  * covers lossless and Stokes media, every source kind and pressure recording, and is ON BY DEFAULT

@@ -1768,7 +1768,8 @@ struct SimT final : kwh_sim {
     }
     void set_plane(bool on) override {
         KWH_REQUIRE(!on || plane_supported(), KWH_EUNSUPPORTED,
-                    "plane kernels need a 3-D grid with a 96x96 or 128x128 (y, z) plane, lossless or Stokes media, "
+                    "plane kernels need a 3-D grid with a 32x32, 64x64, 96x96 or 128x128 (y, z) plane, lossless or "
+                    "Stokes media, "
                     "no collocated velocity output and no checkpointing");
         plane_wanted = on;
         apply_plane(on);

@@ -94,7 +94,7 @@ def sources_case(shape, dtype, nt=30):
     return grid, med, src, sens, SimulationOptions(pmlSize=4, smoothP0=False, dtype=dtype)
 
 
-SHAPES = [(12, 96, 96), (10, 128, 128)]
+SHAPES = [(12, 96, 96), (10, 128, 128), (16, 64, 64), (12, 32, 32)]
 
 
 @pytest.mark.parametrize("dtype", [F32, F64])
@@ -223,10 +223,10 @@ def test_plane_unsupported_configurations():
                 sim.set_plane(True)
         finally:
             sim.close()
-    g2 = KWaveGrid(64, 1e-4, 64, 1e-4, 64, 1e-4)
+    g2 = KWaveGrid(48, 1e-4, 40, 1e-4, 40, 1e-4)   # no plane kernels for a 40 x 40 plane
     g2.setTime(4, 2e-8)
-    sim = Simulation(g2, KWaveMedium(1500.0, 1000.0), KWaveSource(p0=makeBall(64, 64, 64, radius=3)),
-                     KWaveSensor(mask=np.ones((64, 64, 64), np.float32)), SimulationOptions(pmlSize=2))
+    sim = Simulation(g2, KWaveMedium(1500.0, 1000.0), KWaveSource(p0=makeBall(48, 40, 40, radius=3)),
+                     KWaveSensor(mask=np.ones((48, 40, 40), np.float32)), SimulationOptions(pmlSize=2))
     try:
         with pytest.raises(NotImplementedError):
             sim.set_plane(True)
