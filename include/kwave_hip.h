@@ -245,11 +245,11 @@ int kwh_sim_set_graph(kwh_sim* sim, int enable);
  * KSpaceFirstOrder.swift:647-686): 3-D grids whose (y, z) plane is 32 x 32, 64 x 64, 96 x 96 or 128 x 128 run the y
  * and z transforms of each field in one kernel per x plane (the plane held in LDS), so a step
  * reads and writes each field once for the y and z axes together instead of once per axis. It
- * covers lossless and Stokes media, every source kind and pressure recording, and is ON BY DEFAULT
+ * covers every medium and source kind and pressure recording, and is ON BY DEFAULT
  * wherever it applies (environment KWH_PLANE=0 turns the default off); checkpointing and the
  * adjoint switch the sim to the split passes while they are set. enable = 0 selects the split
- * passes; enable = 1 returns KWH_EUNSUPPORTED for configurations plane mode does not cover (FFT
- * power-law absorption, collocated velocity output, checkpointing, other plane shapes). May be
+ * passes; enable = 1 returns KWH_EUNSUPPORTED for configurations plane mode does not cover
+ * (collocated velocity output, checkpointing, other plane shapes). May be
  * switched between steps. Results agree with the split passes to rounding (the transforms are
  * factored differently); a sim is deterministic in either mode. */
 int kwh_sim_set_plane(kwh_sim* sim, int enable);

@@ -527,16 +527,18 @@ __global__ void __launch_bounds__(NT, (plane_waves<T, LY, LZ, NT>())) plane_velo
     plane_velocity_body<T, LY, LZ, NT, CORR>(bidx_here(), a, g, tab);
 }
 
-// Density update and EOS of plane x (KSpaceFirstOrder.swift:680-702): per axis a,
-// du_a = C2R_yz(D_a); rho_a = pml_a (pml_a rho_a - dt rho0 [B/A factor] du_a) [+ sources];
-// then p = c0^2 (sum_a rho_a [+ tau rho0 sum_a du_a] [+ B/A (sum rho)^2 / (2 rho0)]) (MODE 0:
-// lossless, 2: Stokes) and P = R2C_yz(p) for the next step's x pass. The sums are held in
-// registers (one slot pair per element of the thread's share of the plane).
+// Density update of plane x (KSpaceFirstOrder.swift:680-702): per axis a, du_a = C2R_yz(D_a);
+// rho_a = pml_a (pml_a rho_a - dt rho0 [B/A factor] du_a) [+ sources]. MODE 0 (lossless) / 2
+// (Stokes): then the EOS p = c0^2 (sum_a rho_a [+ tau rho0 sum_a du_a] [+ B/A (sum rho)^2 /
+// (2 rho0)]) and P = R2C_yz(p) for the next step's x pass. MODE 1 (FFT power-law absorption): the
+// inputs of the absorption term instead, rhoT = sum_a rho_a, F1 = R2C_yz(rho0 sum_a du_a) and, with
+// dispersion, F2 = R2C_yz(rhoT) (the EOS follows in plane_eos). The sums are held in registers
+// (one slot pair per element of the thread's share of the plane).
 template <typename T, int LY, int LZ, int NT, bool NL, int MODE, bool CORR>
 __device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g,
                                                    const AxisTabs<T>& tab) {
     using P = PlaneShape<T, LY, LZ>;
-    static_assert(MODE == 0 || MODE == 2, "plane density kernel: lossless or Stokes");
+    static_assert(MODE >= 0 && MODE <= 2, "plane density kernel: lossless, FFT absorption or Stokes");
     constexpr int NV = LY * P::M / NT;
     // rho0 kept in registers across the axes when the occupancy floor leaves room (>= 128 registers)
     constexpr bool HOLD = !NL && MODE == 0 && plane_waves<T, LY, LZ, NT>() <= 4;
@@ -546,7 +548,7 @@ __device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs
     const int x = bi.x;
     const int64_t rb = (int64_t)x * LY * LZ, sb = (int64_t)x * LY * P::MZH;
     cx<T> rt[NV];
-    cx<T> sd[MODE == 2 ? NV : 1];
+    cx<T> sd[MODE != 0 ? NV : 1];
     cx<T> rp[NL ? NV : 1];
     cx<T> r0h[HOLD ? NV : 1];
     if constexpr (NL) {
@@ -591,7 +593,7 @@ __device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs
             else if constexpr (ax == 1) { q0 = pm[r]; q1 = q0; }
             else { const Pair<T> qq = ld2(pm + 2 * m); q0 = qq.a; q1 = qq.b; }
             const T d0 = r0.a * du.x, d1 = r0.b * du.y;
-            if constexpr (MODE == 2) {
+            if constexpr (MODE != 0) {
                 if (ax == 0) sd[it] = cx<T>{d0, d1};
                 else { sd[it].x += d0; sd[it].y += d1; }
             }
@@ -627,6 +629,24 @@ __device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs
             __syncthreads();
         }
     });
+    if constexpr (MODE == 1) {
+        // rhoT, F1 = R2C_yz(rho0 sum du), F2 = R2C_yz(rhoT)
+        T* rT = a.rhoT + rb;
+        plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int c) {
+            st2(rT + c, rt[it].x, rt[it].y);
+            pl[r * P::PITCH + m] = sd[it];
+        });
+        __syncthreads();
+        plane_forward<T, LY, LZ, NT>(pl, a.F1 + sb);
+        if (a.dispersion) {
+            __syncthreads();
+            plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int) { pl[r * P::PITCH + m] = rt[it]; });
+            __syncthreads();
+            plane_forward<T, LY, LZ, NT>(pl, a.F2 + sb);
+        }
+        (void)g;
+        return;
+    }
     // EOS into the real view, then P = R2C_yz(p)
     T* p = a.p + rb;
     struct EL { Pair<T> r0, tu, bn, cc; };
@@ -664,6 +684,99 @@ __device__ __forceinline__ void plane_density_body(const BIdx& bi, const RhoArgs
 template <typename T, int LY, int LZ, int NT, bool NL, int MODE, bool CORR>
 __global__ void __launch_bounds__(NT, (plane_waves<T, LY, LZ, NT>())) plane_density(RhoArgs<T> a, Geo g, AxisTabs<T> tab) {
     plane_density_body<T, LY, LZ, NT, NL, MODE, CORR>(bidx_here(), a, g, tab);
+}
+
+// EOS with FFT power-law absorption on plane x (KSpaceFirstOrder.swift:693-702): A1, A2 are the
+// x-pass outputs |k|^(y-2) F1 and |k|^(y-1) F2; e = rhoT + tau C2R_yz(A1) [- eta C2R_yz(A2)]
+// [+ B/A rhoT^2 / (2 rho0)], p = c0^2 e, P = R2C_yz(p). tau, eta from the cell's sound speed
+// (tau_eta_from_c) or from their maps. The partial sum is held in registers while the second
+// spectrum is transformed. Same arithmetic, in the same order, as reg_z_eos.
+template <typename T, int LY, int LZ, int NT, bool NL>
+__device__ __forceinline__ void plane_eos_body(const BIdx& bi, const EosArgs<T>& a, const Geo& g,
+                                               const AxisTabs<T>& tab) {
+    using P = PlaneShape<T, LY, LZ>;
+    constexpr int NV = LY * P::M / NT;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* pl = reinterpret_cast<cx<T>*>(smem);
+    plane_twiddles<T, LY, LZ, NT>(pl, tab);
+    const int x = bi.x;
+    const int64_t rb = (int64_t)x * LY * LZ, sb = (int64_t)x * LY * P::MZH;
+    const T* rT = a.rhoT + rb;
+    cx<T> e[NV];
+    plane_inverse<T, LY, LZ, NT>(pl, a.A1 + sb);
+    {
+        struct L1 { Pair<T> rt, c, tau; };
+        plane_elements_lf<T, LY, LZ, NT, plane_group<LY, P::M, NT>()>([&](int, int, int, int c) {
+            L1 l;
+            l.rt = ld2(rT + c);
+            l.c = ld2m(a.c0, rb + c);
+            if (!a.tau_eta_from_c) l.tau = ld2m(a.tau, rb + c);
+            return l;
+        }, [&](int it, int r, int m, int, const L1& l) {
+            const cx<T> a1 = pl[r * P::PITCH + m];
+            T tA, tB;
+            if (a.tau_eta_from_c) {
+                tA = a.tk * exp2(a.te * log2(l.c.a));
+                tB = a.tk * exp2(a.te * log2(l.c.b));
+            } else {
+                tA = l.tau.a;
+                tB = l.tau.b;
+            }
+            e[it] = cx<T>{l.rt.a + tA * a1.x, l.rt.b + tB * a1.y};
+        });
+    }
+    __syncthreads();
+    if (a.A2) {
+        plane_inverse<T, LY, LZ, NT>(pl, a.A2 + sb);
+        struct L2 { Pair<T> c, eta; };
+        plane_elements_lf<T, LY, LZ, NT, plane_group<LY, P::M, NT>()>([&](int, int, int, int c) {
+            L2 l;
+            if (a.tau_eta_from_c) l.c = ld2m(a.c0, rb + c);
+            else l.eta = ld2m(a.eta, rb + c);
+            return l;
+        }, [&](int it, int r, int m, int, const L2& l) {
+            const cx<T> a2 = pl[r * P::PITCH + m];
+            T hA, hB;
+            if (a.tau_eta_from_c) {
+                hA = a.ek * exp2(a.ee * log2(l.c.a));
+                hB = a.ek * exp2(a.ee * log2(l.c.b));
+            } else {
+                hA = l.eta.a;
+                hB = l.eta.b;
+            }
+            e[it].x -= hA * a2.x;
+            e[it].y -= hB * a2.y;
+        });
+        __syncthreads();
+    }
+    T* p = a.p + rb;
+    struct L3 { Pair<T> c, rt, bn, r0; };
+    plane_elements_lf<T, LY, LZ, NT, plane_group<LY, P::M, NT>()>([&](int, int, int, int c) {
+        L3 l;
+        l.c = ld2m(a.c0, rb + c);
+        if constexpr (NL) {
+            l.rt = ld2(rT + c);
+            l.bn = ld2m(a.bona, rb + c);
+            l.r0 = ld2m(a.rho0, rb + c);
+        }
+        return l;
+    }, [&](int it, int r, int m, int c, const L3& l) {
+        T ea = e[it].x, eb = e[it].y;
+        if constexpr (NL) {
+            ea += l.bn.a * (l.rt.a * l.rt.a) / (T(2) * l.r0.a);
+            eb += l.bn.b * (l.rt.b * l.rt.b) / (T(2) * l.r0.b);
+        }
+        const T pA = l.c.a * l.c.a * ea, pB = l.c.b * l.c.b * eb;
+        st2(p + c, pA, pB);
+        pl[r * P::PITCH + m] = cx<T>{pA, pB};
+    });
+    __syncthreads();
+    plane_forward<T, LY, LZ, NT>(pl, a.outP + sb);
+    (void)g;
+}
+template <typename T, int LY, int LZ, int NT, bool NL>
+__global__ void __launch_bounds__(NT, (plane_waves<T, LY, LZ, NT>())) plane_eos(EosArgs<T> a, Geo g, AxisTabs<T> tab) {
+    plane_eos_body<T, LY, LZ, NT, NL>(bidx_here(), a, g, tab);
 }
 
 }  // namespace kwh

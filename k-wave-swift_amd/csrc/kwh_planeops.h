@@ -19,9 +19,11 @@ struct PlaneOps {
     void (*fwd_real)(const Emit&, int planes, const T*, cx<T>*, T, const Geo&, const AxisTabs<T>&);
     void (*inv_real)(const Emit&, int planes, const cx<T>*, T*, T, int accumulate, const Geo&, const AxisTabs<T>&);
     void (*velocity)(const Emit&, int planes, const VelArgs<T>&, const Geo&, const AxisTabs<T>&, bool corr);
-    // mode 0: lossless EOS; 2: EOS with Stokes absorption
+    // mode 0: lossless EOS; 1: inputs of the FFT absorption term; 2: EOS with Stokes absorption
     void (*density)(const Emit&, int planes, const RhoArgs<T>&, const Geo&, const AxisTabs<T>&, bool nonlinear,
                     int mode, bool corr);
+    // EOS with FFT power-law absorption
+    void (*eos)(const Emit&, int planes, const EosArgs<T>&, const Geo&, const AxisTabs<T>&);
 };
 
 // Launchers for plane shape m1 x m2, or nullptr when no plane kernel covers it.

@@ -17,6 +17,8 @@ struct PlaneImpl {
     template <bool NL, int MODE, bool CORR>
     using DenB = Body<&plane_density_body<T, LY, LZ, NT, NL, MODE, CORR>, &plane_density<T, LY, LZ, NT, NL, MODE, CORR>,
                       W, false, NT>;
+    template <bool NL>
+    using EosB = Body<&plane_eos_body<T, LY, LZ, NT, NL>, &plane_eos<T, LY, LZ, NT, NL>, W, false, NT>;
 
     static hipError_t set_lds(const void* fn) {
         return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::kLds);
@@ -35,8 +37,12 @@ struct PlaneImpl {
             if (hipError_t e = set_lds(f); e != hipSuccess) return e;
         hipError_t e = set_density<false, 0>();
         if (e == hipSuccess) e = set_density<true, 0>();
+        if (e == hipSuccess) e = set_density<false, 1>();
+        if (e == hipSuccess) e = set_density<true, 1>();
         if (e == hipSuccess) e = set_density<false, 2>();
         if (e == hipSuccess) e = set_density<true, 2>();
+        if (e == hipSuccess) e = set_lds((const void*)&plane_eos<T, LY, LZ, NT, false>);
+        if (e == hipSuccess) e = set_lds((const void*)&plane_eos<T, LY, LZ, NT, true>);
         return e;
     }
 
@@ -64,14 +70,22 @@ struct PlaneImpl {
         if (mode == 2) {
             if (nonlinear) density_m<true, 2>(em, planes, ra, g, tab, corr);
             else density_m<false, 2>(em, planes, ra, g, tab, corr);
+        } else if (mode == 1) {
+            if (nonlinear) density_m<true, 1>(em, planes, ra, g, tab, corr);
+            else density_m<false, 1>(em, planes, ra, g, tab, corr);
         } else {
             if (nonlinear) density_m<true, 0>(em, planes, ra, g, tab, corr);
             else density_m<false, 0>(em, planes, ra, g, tab, corr);
         }
     }
 
+    static void eos(const Emit& em, int planes, const EosArgs<T>& ea, const Geo& g, const AxisTabs<T>& tab) {
+        if (ea.nonlinear) em.go<EosB<true>>(dim3(planes), dim3(NT), S::kLds, ea, g, tab);
+        else em.go<EosB<false>>(dim3(planes), dim3(NT), S::kLds, ea, g, tab);
+    }
+
     static const PlaneOps<T>* ops() {
-        static const PlaneOps<T> o{LY, LZ, NT, S::kLds, &prepare, &fwd_real, &inv_real, &velocity, &density};
+        static const PlaneOps<T> o{LY, LZ, NT, S::kLds, &prepare, &fwd_real, &inv_real, &velocity, &density, &eos};
         return &o;
     }
 };

@@ -269,14 +269,14 @@ struct kwh_sim {
 
 namespace {
 
-constexpr int kNames = 21;
+constexpr int kNames = 22;
 const char* kKernelNames[kNames] = {"strided_fwd", "strided_inv", "middle_grad", "middle_div", "middle_abs",
                                     "z_velocity",  "z_density",   "z_eos",       "z_init",     "z_forward_real",
                                     "z_inverse_real", "middle_table", "gather", "source_scatter", "fill", "other",
                                     "z_colloc_rows", "plane_velocity", "plane_density", "plane_forward_real",
-                                    "plane_inverse_real"};
+                                    "plane_inverse_real", "plane_eos"};
 enum KId { K_SF, K_SI, K_MG, K_MD, K_MA, K_ZV, K_ZD, K_ZE, K_ZI, K_ZFR, K_ZIR, K_MT, K_GA, K_SC, K_FI, K_OT, K_ZCR,
-           K_PV, K_PD, K_PFR, K_PIR };
+           K_PV, K_PD, K_PFR, K_PIR, K_PE };
 
 template <typename T>
 struct SourceDev {
@@ -1734,6 +1734,8 @@ struct SimT final : kwh_sim {
                 ra.corr_factor[a] = T(psrc.factor[a]);
                 ra.sp[a] = sparse_of(psrc, a);
             }
+            const bool fft_abs = absorbing && !stokes;
+            const int mode = fft_abs ? 1 : (stokes ? 2 : 0);
             ra.rho0 = rho0s;
             ra.c0 = c0s;
             ra.bona = bonas;
@@ -1741,35 +1743,69 @@ struct SimT final : kwh_sim {
             ra.corr = cp ? mk_map(cp) : zero_map;
             ra.p = p.as<T>();
             ra.outP = P;
+            ra.rhoT = rhoT.as<T>();
+            ra.F1 = spec[0].as<C>();
+            ra.F2 = spec[1].as<C>();
             ra.dt = T(dt);
-            ra.mode = stokes ? 2 : 0;
+            ra.mode = mode;
             ra.nonlinear = nonlinear;
             ra.stokes = stokes;
+            ra.dispersion = dispersion;
             ra.ctr = (const int*)ctr.p;
-            double bytes = dim * (spec_bytes() + 2 * real_bytes()) + mapb(rho0s.p) + (cp ? real_bytes() : 0) +
-                           real_bytes() + mapb(c0s.p) + spec_bytes() + (stokes ? mapb(taus.p) : 0);
+            double bytes = dim * (spec_bytes() + 2 * real_bytes()) + mapb(rho0s.p) + (cp ? real_bytes() : 0);
+            if (fft_abs) bytes += real_bytes() + spec_bytes() * (dispersion ? 2 : 1);
+            else bytes += real_bytes() + mapb(c0s.p) + spec_bytes() + (stokes ? mapb(taus.p) : 0);
             if (nonlinear) bytes += mapb(bonas.p) + dim * real_bytes();
             timed(K_PD, bytes, [&] {
-                px->density(em(), g.m0, ra, g, tab, nonlinear, stokes ? 2 : 0, cp != nullptr);
+                px->density(em(), g.m0, ra, g, tab, nonlinear, mode, cp != nullptr);
             });
+            if (fft_abs) {
+                // x pass of the absorption inputs (as main_step), then the plane EOS
+                std::vector<const void*> ins = {spec[0].p};
+                std::vector<MidOp> ops = {MidOp{OP_TAB, 0, 0, spec[0].p, nab1.p}};
+                if (dispersion) {
+                    ins.push_back(spec[1].p);
+                    ops.push_back(MidOp{OP_TAB, 0, 1, spec[1].p, nab2.p});
+                }
+                middle(K_MA, ins, ops);
+                EosArgs<T> ea{};
+                ea.A1 = spec[0].as<C>();
+                ea.A2 = dispersion ? spec[1].as<C>() : nullptr;
+                ea.rhoT = rhoT.as<T>();
+                ea.c0 = c0s;
+                ea.rho0 = rho0s;
+                ea.tau = taus;
+                ea.eta = etas;
+                ea.bona = bonas;
+                ea.nonlinear = nonlinear;
+                ea.tau_eta_from_c = tau_eta_from_c ? 1 : 0;
+                ea.tk = T(tk);
+                ea.te = T(te);
+                ea.ek = T(ek);
+                ea.ee = T(ee);
+                ea.p = p.as<T>();
+                ea.outP = P;
+                const double eb = spec_bytes() * (dispersion ? 3 : 2) + 2 * real_bytes() + mapb(c0s.p) +
+                                  mapb(taus.p) + (dispersion ? mapb(etas.p) : 0) +
+                                  (nonlinear ? mapb(bonas.p) + mapb(rho0s.p) + real_bytes() : 0);
+                timed(K_PE, eb, [&] { px->eos(em(), g.m0, ea, g, tab); });
+            }
         }
         P_valid = true;
     }
 
-    // Plane mode covers 3-D grids whose (m1, m2) plane has plane kernels, lossless or Stokes
-    // media (the FFT absorption term has passes of its own), no collocated velocity output
-    // (computed from the z-only spectra of the split passes) and no checkpointing.
+    // Plane mode covers 3-D grids whose (m1, m2) plane has plane kernels, every medium, no
+    // collocated velocity output (computed from the z-only spectra of the split passes) and no
+    // checkpointing.
     bool plane_supported() const {
         if (!px || axisym || ckpt_every > 0) return false;
-        if (absorbing && !stokes) return false;
         for (int a = 0; a < dim; ++a)
             if (need_u[a]) return false;
         return true;
     }
     void set_plane(bool on) override {
         KWH_REQUIRE(!on || plane_supported(), KWH_EUNSUPPORTED,
-                    "plane kernels need a 3-D grid with a 32x32, 64x64, 96x96 or 128x128 (y, z) plane, lossless or "
-                    "Stokes media, "
+                    "plane kernels need a 3-D grid with a 32x32, 64x64, 96x96 or 128x128 (y, z) plane, "
                     "no collocated velocity output and no checkpointing");
         plane_wanted = on;
         apply_plane(on);

@@ -477,7 +477,7 @@ class Simulation:
 
     def set_plane(self, on: bool):
         """Plane mode (kwh_sim_set_plane): the y and z transforms of each field in one kernel per
-        x plane; 3-D grids with a 32x32, 64x64, 96x96 or 128x128 (y, z) plane, lossless or Stokes media. On by
+        x plane; 3-D grids with a 32x32, 64x64, 96x96 or 128x128 (y, z) plane, any medium. On by
         default wherever it applies; set_plane(False) selects the split passes."""
         L.check(self.lib.kwh_sim_set_plane(self.handle, 1 if on else 0))
 
