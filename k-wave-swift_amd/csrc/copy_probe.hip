@@ -1,6 +1,8 @@
-// copy_probe -- a plain device-to-device copy of known size (1 GiB read + 1 GiB written per
-// launch). Its event-timed bandwidth is the practical HBM ceiling the solver kernels are compared
-// with, and rocprofv3's FETCH_SIZE / WRITE_SIZE for it calibrate those counters to bytes.
+// copy_probe -- device-to-device copies of known size (1 GiB read + 1 GiB written per launch).
+// copy_kernel: a plain grid-stride copy; rocprofv3's FETCH_SIZE / WRITE_SIZE for it calibrate those
+// counters to bytes. copy_nt_kernel: one 16-byte element per lane of a grid covering the buffer
+// once, with nontemporal loads and stores -- the fastest copy found (a reference point for
+// streaming kernels, not a ceiling). Both are event-timed; one JSON line each.
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstdlib>
@@ -30,6 +32,20 @@ __global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ a,
     for (; i < n; i += bd) b[i] = a[i];
 }
 
+// One grid pass over the buffer, four 16-byte elements per lane, nontemporal hints.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) copy_nt_kernel(const v4f* __restrict__ a, v4f* __restrict__ b, size_t n) {
+    constexpr int U = 4;
+    const size_t i = ((size_t)blockIdx.x * U) * blockDim.x + threadIdx.x;
+    v4f t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (i + (size_t)u * blockDim.x < n) t[u] = __builtin_nontemporal_load(&a[i + (size_t)u * blockDim.x]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (i + (size_t)u * blockDim.x < n) __builtin_nontemporal_store(t[u], &b[i + (size_t)u * blockDim.x]);
+}
+
 int main(int argc, char** argv) {
     const size_t bytes = (size_t)1 << 30;
     const size_t n = bytes / sizeof(float4);
@@ -54,6 +70,20 @@ int main(int argc, char** argv) {
     float ms = 0;
     CHK(hipEventElapsedTime(&ms, e0, e1));
     std::printf("{\"kernel\": \"copy_kernel\", \"bytes_per_launch\": %zu, \"avg_us\": %.2f, \"GBps\": %.1f}\n",
+                2 * bytes, ms * 1e3 / iters, 2.0 * bytes * iters / (ms * 1e-3) / 1e9);
+    const unsigned nt_blocks = (unsigned)((n + 4 * 256 - 1) / (4 * 256));
+    const v4f* av = reinterpret_cast<const v4f*>(a);
+    v4f* bv = reinterpret_cast<v4f*>(b);
+    for (int w = 0; w < 3; ++w) hipLaunchKernelGGL(copy_nt_kernel, dim3(nt_blocks), dim3(256), 0, 0, av, bv, n);
+    CHK(hipGetLastError());
+    CHK(hipDeviceSynchronize());
+    CHK(hipEventRecord(e0));
+    for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(copy_nt_kernel, dim3(nt_blocks), dim3(256), 0, 0, av, bv, n);
+    CHK(hipGetLastError());
+    CHK(hipEventRecord(e1));
+    CHK(hipEventSynchronize(e1));
+    CHK(hipEventElapsedTime(&ms, e0, e1));
+    std::printf("{\"kernel\": \"copy_nt_kernel\", \"bytes_per_launch\": %zu, \"avg_us\": %.2f, \"GBps\": %.1f}\n",
                 2 * bytes, ms * 1e3 / iters, 2.0 * bytes * iters / (ms * 1e-3) / 1e9);
     CHK(hipFree(a));
     CHK(hipFree(b));
