@@ -14,6 +14,7 @@
 // transform; every lane busy in both steps. For J = E (64, 256) this is two radix-R stages.
 #pragma once
 #include "kwh_kernels.h"
+#include "kwh_xlane.h"
 
 // Occupancy floors (waves per SIMD) of the fused row kernels with E <= 16 in FP32: without them
 // the register allocator spends the whole register file on load hoisting and drops a wave per
@@ -58,6 +59,10 @@
 #endif
 #ifndef KWH_EOS_VERSIONED
 #define KWH_EOS_VERSIONED 1
+#endif
+
+#ifndef KWH_MIDDLE_XLANE_MAXJ
+#define KWH_MIDDLE_XLANE_MAXJ 4
 #endif
 
 namespace kwh {
@@ -1004,15 +1009,29 @@ __global__ void __launch_bounds__(256) reg_strided(StridedArgs a, const cx<T>* _
 
 // Middle pass: forward along the middle axis, operators, inverse, store. Every table-using
 // operator of one input reads the same real k-space table (checked on the host), loaded once.
-template <typename T, int J, int E, int NTR>
+// XL = false: lane (w, j) with w fastest, the inner radix through an LDS transpose (RegCol);
+// spectra between the transforms in the lane-natural pattern k = j + J*m.
+// XL = true: lane (w, j) with j fastest, the inner radix across the lanes of a DPP row (XLane);
+// spectra between the transforms in the block pattern k = m + E*brev(j). LDS: the twiddle table.
+template <typename T, int J, int E, int NTR, bool XL = false>
 __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a, const AxisTabs<T>& tab, const Geo& g) {
     using S = RegCol<T, J, E, NTR>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
     cx<T>* lds = ldsb + S::L;
-    const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
+    const int w = XL ? (int)threadIdx.x / J : (int)threadIdx.x % NTR;
+    const int j = XL ? (int)threadIdx.x % J : (int)threadIdx.x / NTR;
     LaneTw<T, J, E> lt;
-    lt.load(ldsb, tab.tw[a.axis], j);
+    XLane<T, J, E> xl;
+    if constexpr (XL) {
+        xl.init(ldsb, tab.tw[a.axis], j);
+        __syncthreads();
+    } else {
+        lt.load(ldsb, tab.tw[a.axis], j);
+    }
+    // spectrum element held in slot m: k = kb + kst * m
+    const int kb = XL ? xl.block_start() : j;
+    constexpr int kst = XL ? 1 : J;
     const int tile = bi.tile();
     const int64_t c = (int64_t)tile * NTR + w;
     const bool valid = c < a.ncols;
@@ -1024,8 +1043,8 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
     if (a.axis == 0) { i0c = 0; i1c = (int)(cc / g.mzh); i2c = (int)(cc % g.mzh); }
     else { i0c = (int)(cc / g.mzh); i1c = 0; i2c = (int)(cc % g.mzh); }
     const int64_t hstride = a.axis == 0 ? (int64_t)g.m1 * g.mzh : (int64_t)g.mzh;
-    const int64_t hbase = (a.axis == 0 ? cc : (int64_t)i0c * g.m1 * g.mzh + i2c) + (int64_t)j * hstride;
-    const int64_t hstep = (int64_t)J * hstride;
+    const int64_t hbase = (a.axis == 0 ? cc : (int64_t)i0c * g.m1 * g.mzh + i2c) + (int64_t)kb * hstride;
+    const int64_t hstep = (int64_t)kst * hstride;
     T kap[E];
     const T* kap_src = nullptr;   // table currently held in kap (inputs sharing one table load it once)
     for (int f = 0; f < a.nin; ++f) {
@@ -1055,7 +1074,8 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
             }
         };
         if (KWH_MIDDLE_KAPPA_EARLY) load_table();
-        S::template transform<false>(X, lds, j, w, lt);
+        if constexpr (XL) xl.template to_block<false>(X);
+        else S::template transform<false>(X, lds, j, w, lt);
         if (!KWH_MIDDLE_KAPPA_EARLY) load_table();
         for (int o = 0; o < a.nop; ++o) {
             const MidOp op = a.op[o];
@@ -1067,7 +1087,7 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
             const bool vec_on_mid = op.axis == a.axis;
             const int fixed_idx = op.axis == 0 ? i0c : (op.axis == 1 ? i1c : i2c);
             const cx<T> fixed_vec = use_vec && !vec_on_mid ? tab1[fixed_idx] : cx<T>{T(1), T(0)};
-            const cx<T>* tj = tab1 + j;
+            const cx<T>* tj = tab1 + kb;
             cx<T> v[E];
 #if KWH_MIDDLE_VERSIONED
             // The operator's kind is uniform over the workgroup: choose the element loop once, so
@@ -1075,7 +1095,7 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
             if (use_vec && vec_on_mid) {
                 cx<T> mv[E];
 #pragma unroll
-                for (int m = 0; m < E; ++m) mv[m] = tj[J * m];
+                for (int m = 0; m < E; ++m) mv[m] = tj[kst * m];
                 if (use_kappa) {
 #pragma unroll
                     for (int m = 0; m < E; ++m) v[m] = cmul(X[m], mv[m] * kap[m]);
@@ -1094,12 +1114,13 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
 #else
 #pragma unroll
             for (int m = 0; m < E; ++m) {
-                cx<T> mv = use_vec ? (vec_on_mid ? tj[J * m] : fixed_vec) : cx<T>{T(1), T(0)};
+                cx<T> mv = use_vec ? (vec_on_mid ? tj[kst * m] : fixed_vec) : cx<T>{T(1), T(0)};
                 const T s = use_kappa ? kap[m] : scale;
                 v[m] = cmul(X[m], cx<T>{mv.x * s, mv.y * s});
             }
 #endif
-            S::template transform<true>(v, lds, j, w, lt);
+            if constexpr (XL) xl.template to_cyclic<true>(v);
+            else S::template transform<true>(v, lds, j, w, lt);
             if (valid) {
                 cx<T>* q = reinterpret_cast<cx<T>*>(op.out) + base;
 #pragma unroll
@@ -1111,10 +1132,10 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
         }
     }
 }
-template <typename T, int J, int E, int NTR>
+template <typename T, int J, int E, int NTR, bool XL = false>
 __global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_MIDDLE_WAVES)))
     reg_middle(MidArgs a, AxisTabs<T> tab, Geo g) {
-    reg_middle_body<T, J, E, NTR>(bidx_here(), a, tab, g);
+    reg_middle_body<T, J, E, NTR, XL>(bidx_here(), a, tab, g);
 }
 
 }  // namespace kwh

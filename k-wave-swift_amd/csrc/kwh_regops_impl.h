@@ -3,6 +3,7 @@
 #pragma once
 #include "kwh_reg.h"
 #include "kwh_regops.h"
+#include <cstdlib>
 
 namespace kwh {
 
@@ -21,6 +22,18 @@ struct RegImpl {
     using StrB = Body<&reg_strided_body<T, J, E, N, INV>, &reg_strided<T, J, E, N, INV>, 1, false>;
     template <int N>
     using MidB = Body<&reg_middle_body<T, J, E, N>, &reg_middle<T, J, E, N>, reg_waves<T, E>(KWH_MIDDLE_WAVES), false>;
+    template <int N>
+    using MidXB = Body<&reg_middle_body<T, J, E, N, true>, &reg_middle<T, J, E, N, true>,
+                       reg_waves<T, E>(KWH_MIDDLE_WAVES), false>;
+    // Middle pass with the inner radix across lanes (DPP) for J up to KWH_MIDDLE_XLANE_MAXJ;
+    // KWH_MIDDLE_XLANE=0/1 in the environment overrides.
+    static bool middle_xlane() {
+        static const bool on = [] {
+            const char* e = std::getenv("KWH_MIDDLE_XLANE");
+            return e && *e ? std::atoi(e) != 0 : J <= KWH_MIDDLE_XLANE_MAXJ;
+        }();
+        return on;
+    }
 
     static void z_forward_real(const Emit& em, dim3 gr, dim3 bl, size_t lds, const T* src, cx<T>* spec, T scale,
                                const Geo& g, int P, const AxisTabs<T>& tab) {
@@ -88,6 +101,12 @@ struct RegImpl {
     }
     static void middle(const Emit& em, dim3 gr, dim3 bl, size_t lds, const MidArgs& a, const AxisTabs<T>& tab,
                        const Geo& g, int ntr) {
+        if (middle_xlane()) {
+            const size_t lx = (size_t)J * E * sizeof(cx<T>);   // twiddle table only
+            if (ntr == NTR) em.go<MidXB<NTR>>(gr, bl, lx, a, tab, g);
+            else em.go<MidXB<NTR / 2>>(gr, bl, lx, a, tab, g);
+            return;
+        }
         if (ntr == NTR) em.go<MidB<NTR>>(gr, bl, lds, a, tab, g);
         else em.go<MidB<NTR / 2>>(gr, bl, lds, a, tab, g);
     }
