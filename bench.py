@@ -37,6 +37,13 @@ def workload(name, dtype):
                 "grid": [256, 256, 256], "pml": 10, "nt": 1000}
         # byte model per step: s*N*(2*d*T + S), T = 15 transforms, S = 27 other full-grid arrays
         model = {"T": 15, "S": 27, "d": 3}
+    elif name == "slab":
+        # one x slab of a 512^3 grid split over 8 GPUs: the per-GPU shape of the local z / y passes
+        from kwavehip import workloads
+        grid, med, src, sens, opts = workloads.throughput_3d(steps=1000, dtype=dt_, plane=40, shape=(64, 512, 512))
+        info = {"workload": "3D 64x512x512 (one x slab of 512^3 over 8 GPUs) heterogeneous power-law (y=1.1) "
+                            "p0-ball, plane sensor p+uz", "grid": [64, 512, 512], "pml": 10, "nt": 1000}
+        model = {"T": 15, "S": 27, "d": 3}
     elif name == "c0":
         n, dx = 128, 1e-4
         grid = KWaveGrid(n, dx, n, dx)
@@ -343,7 +350,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)   # at least 2: step 0 is never timed
-    ap.add_argument("--workload", default="c2")   # c2 (default), c0, or refbench (the reference's scenarios)
+    ap.add_argument("--workload", default="c2")   # c2 (default), c0, slab, or refbench (the reference's scenarios)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")

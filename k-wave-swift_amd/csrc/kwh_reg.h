@@ -19,7 +19,8 @@
 // Occupancy floors (waves per SIMD) of the fused row kernels with E <= 16 in FP32: without them
 // the register allocator spends the whole register file on load hoisting and drops a wave per
 // SIMD. The density kernels that also accumulate rho0 * div u stay at two. Longer rows (E > 16)
-// hold more registers per lane and carry no floor.
+// hold more registers per lane and carry no floor (at E = 32 a floor of two costs the EOS and the
+// density kernels scratch spills and measured no faster).
 #ifndef KWH_DENSITY_WAVES
 #define KWH_DENSITY_WAVES 3
 #endif
@@ -471,7 +472,13 @@ __global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_VELOCITY_WAVES)))
 // followed by the EOS (MODE 0), the EOS with Stokes absorption (MODE 2) or the inputs of the
 // FFT absorption term (MODE 1: rhoT, F1 = R2C_z(rho0 sum_a du_a), F2 = R2C_z(rhoT)). NL: B/A.
 // The total density and, for MODE 1/2, rho0 * sum_a du_a are accumulated in registers while the
-// axes are processed, so each divergence spectrum is read once.
+// axes are processed, so each divergence spectrum is read once. Length-512 rows in MODE 1 (FP32,
+// E = 32) keep only rho0 * sum_a du_a: rhoT and F2 come from reg_z_rhot, which re-reads the stored
+// rho_a (d reads of a real field) and frees E accumulator registers per lane. Measured on 3-D
+// power-law grids: E = 32 density 571 -> 300 + 144 us (64 x 512 x 512); at E = 24 and 40 the split
+// is no faster (115 -> 79 + 38, 245 -> 189 + 83 us), so those keep the fused kernel.
+template <typename T, int E, int MODE>
+constexpr bool density_rhot_split() { return MODE == 1 && E == 32 && sizeof(T) == 4; }
 template <typename T, int J, int E, bool NL, int MODE, bool CORR>
 constexpr int density_waves() {
     return (NL || CORR) ? 1 : reg_waves<T, E>(MODE == 0 ? KWH_DENSITY_WAVES : 2);
@@ -501,7 +508,8 @@ __device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs
 #pragma unroll
         for (int m = 0; m < E; ++m) r0h[m] = cx<T>{r0pa[J * m], r0pb[J * m]};
     }
-    cx<T> rt[E];
+    constexpr bool RT = !density_rhot_split<T, E, MODE>();
+    cx<T> rt[RT ? E : 1];
     cx<T> sd[ACC ? E : 1];
     cx<T> rp[NL ? E : 1];
     if constexpr (NL) {
@@ -584,8 +592,10 @@ __device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs
         for (int m = 0; m < E; ++m) {
             if (va) wa[J * m] = v[m].x;
             if (vb) wb[J * m] = v[m].y;
-            if (ax == 0) rt[m] = v[m];
-            else { rt[m].x += v[m].x; rt[m].y += v[m].y; }
+            if constexpr (RT) {
+                if (ax == 0) rt[m] = v[m];
+                else { rt[m].x += v[m].x; rt[m].y += v[m].y; }
+            }
         }
     }
     cx<T> v[E];
@@ -621,14 +631,61 @@ __device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cx<T>{va ? sd[m].x : T(0), vb ? sd[m].y : T(0)};
         S::template forward_store<FULL>(a.F1, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+        if constexpr (RT) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                if (va) a.rhoT[oa + J * m] = rt[m].x;
+                if (vb) a.rhoT[ob + J * m] = rt[m].y;
+                rt[m] = cx<T>{va ? rt[m].x : T(0), vb ? rt[m].y : T(0)};
+            }
+            if (a.dispersion) S::template forward_store<FULL>(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, rt);
+        }
+    }
+}
+
+// rhoT = sum_a rho_a of a row pair (summed in axis order, as the density kernel does) and, with
+// dispersion, F2 = R2C_z(rhoT): the part of the FFT-absorption density kernel that long rows leave
+// to a pass of their own (density_rhot_split).
+template <typename T, int J, int E, bool FULL>
+__device__ __forceinline__ void reg_z_rhot_impl(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P,
+                                                const AxisTabs<T>& tab) {
+    using S = RegRow<T, J, E>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
+    RegZPos<J> ps(P, bi);
+    LaneTw<T, J, E> lt;
+    lt.load(ldsb, tab.tw[2], ps.j);
+    cx<T>* lds = ldsb + S::L + ps.w * S::LP;
+    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
+    const int64_t oa = (int64_t)(va ? ps.ra : 0) * S::L + ps.j, ob = (int64_t)(vb ? ps.rb : 0) * S::L + ps.j;
+    cx<T> rt[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) rt[m] = cx<T>{a.rho[0][oa + J * m], a.rho[0][ob + J * m]};
+    for (int ax = 1; ax < g.dim; ++ax) {
+        const T* rho = a.rho[ax];
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-            if (va) a.rhoT[oa + J * m] = rt[m].x;
-            if (vb) a.rhoT[ob + J * m] = rt[m].y;
-            rt[m] = cx<T>{va ? rt[m].x : T(0), vb ? rt[m].y : T(0)};
+            rt[m].x += rho[oa + J * m];
+            rt[m].y += rho[ob + J * m];
         }
-        if (a.dispersion) S::template forward_store<FULL>(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, rt);
     }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        if (va) a.rhoT[oa + J * m] = rt[m].x;
+        if (vb) a.rhoT[ob + J * m] = rt[m].y;
+        rt[m] = cx<T>{va ? rt[m].x : T(0), vb ? rt[m].y : T(0)};
+    }
+    if (a.dispersion) S::template forward_store<FULL>(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, rt);
+}
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_z_rhot_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P,
+                                                const AxisTabs<T>& tab) {
+    if (KWH_ROW_FULL_TILES && reg_tile_full(bi, P, g.nrows)) reg_z_rhot_impl<T, J, E, true>(bi, a, g, P, tab);
+    else reg_z_rhot_impl<T, J, E, false>(bi, a, g, P, tab);
+}
+template <typename T, int J, int E>
+__global__ void __launch_bounds__(256) reg_z_rhot(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    reg_z_rhot_body<T, J, E>(bidx_here(), a, g, P, tab);
 }
 template <typename T, int J, int E, bool NL, int MODE, bool CORR>
 __device__ __forceinline__ void reg_z_density_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P, const AxisTabs<T>& tab) {

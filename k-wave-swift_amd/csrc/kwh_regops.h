@@ -27,6 +27,9 @@ struct RegOps {
     void (*z_density)(const Emit&, dim3, dim3, size_t, const RhoArgs<T>&, const Geo&, int, const AxisTabs<T>&,
                       bool nonlinear, int mode, bool corr);
     void (*z_eos)(const Emit&, dim3, dim3, size_t, const EosArgs<T>&, const Geo&, int, const AxisTabs<T>&);
+    // rhoT = sum_a rho_a and F2 = R2C_z(rhoT), after z_density mode 1 when rhot_split is set
+    bool rhot_split;
+    void (*z_rhot)(const Emit&, dim3, dim3, size_t, const RhoArgs<T>&, const Geo&, int, const AxisTabs<T>&);
     // adjoint row kernels (AdjRowArgs): forward mode 0 / 1, inverse mode 0 / 1 / 2
     void (*adj_fwd)(const Emit&, dim3, dim3, size_t, const AdjRowArgs<T>&, const Geo&, int, const AxisTabs<T>&,
                     int mode);

@@ -79,6 +79,10 @@ struct RegImpl {
         em.go<Body<&reg_z_eos_body<T, J, E>, &reg_z_eos<T, J, E>, eos_waves<T, E>(), false>>(
             gr, bl, lds, ea, g, P, tab);
     }
+    static void z_rhot(const Emit& em, dim3 gr, dim3 bl, size_t lds, const RhoArgs<T>& ra, const Geo& g, int P,
+                       const AxisTabs<T>& tab) {
+        em.go<Body<&reg_z_rhot_body<T, J, E>, &reg_z_rhot<T, J, E>, 1, false>>(gr, bl, lds, ra, g, P, tab);
+    }
     static void adj_fwd(const Emit& em, dim3 gr, dim3 bl, size_t lds, const AdjRowArgs<T>& a, const Geo& g, int P,
                         const AxisTabs<T>& tab, int mode) {
         if (mode == 0) em.go<Body<&reg_adj_fwd_body<T, J, E, 0>, &reg_adj_fwd<T, J, E, 0>, 1, false>>(gr, bl, lds, a, g, P, tab);
@@ -113,7 +117,8 @@ struct RegImpl {
 
     static const RegOps<T>* ops() {
         static const RegOps<T> o{J, E, NTR, &z_forward_real, &z_inverse_real, &z_colloc_rows, &z_velocity,
-                                 &z_density, &z_eos, &adj_fwd, &adj_inv, &strided, &middle};
+                                 &z_density, &z_eos, density_rhot_split<T, E, 1>(), &z_rhot, &adj_fwd, &adj_inv,
+                                 &strided, &middle};
         return &o;
     }
 };

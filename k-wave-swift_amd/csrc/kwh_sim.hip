@@ -269,14 +269,14 @@ struct kwh_sim {
 
 namespace {
 
-constexpr int kNames = 22;
+constexpr int kNames = 23;
 const char* kKernelNames[kNames] = {"strided_fwd", "strided_inv", "middle_grad", "middle_div", "middle_abs",
                                     "z_velocity",  "z_density",   "z_eos",       "z_init",     "z_forward_real",
                                     "z_inverse_real", "middle_table", "gather", "source_scatter", "fill", "other",
                                     "z_colloc_rows", "plane_velocity", "plane_density", "plane_forward_real",
-                                    "plane_inverse_real", "plane_eos"};
+                                    "plane_inverse_real", "plane_eos", "z_rhot"};
 enum KId { K_SF, K_SI, K_MG, K_MD, K_MA, K_ZV, K_ZD, K_ZE, K_ZI, K_ZFR, K_ZIR, K_MT, K_GA, K_SC, K_FI, K_OT, K_ZCR,
-           K_PV, K_PD, K_PFR, K_PIR, K_PE };
+           K_PV, K_PD, K_PFR, K_PIR, K_PE, K_ZRT };
 
 template <typename T>
 struct SourceDev {
@@ -1630,10 +1630,17 @@ struct SimT final : kwh_sim {
             if (rx[2]) {
                 const RegZ z = regz();
                 const int mode = fft_abs ? 1 : (stokes ? 2 : 0);
-                timed(K_ZD, bytes, [&] {
+                // long rows: rhoT and F2 in a pass of their own (reg_z_rhot), after the density kernel
+                const bool split = mode == 1 && rx[2]->rhot_split;
+                const double rt_bytes = real_bytes() + (dispersion ? spec_bytes() : 0);
+                timed(K_ZD, split ? bytes - rt_bytes : bytes, [&] {
                     rx[2]->z_density(em(), dim3(z.blocks), dim3(z.threads), z.lds, ra, g, z.P, tab, nonlinear, mode,
                                      cp != nullptr);
                 });
+                if (split)
+                    timed(K_ZRT, dim * real_bytes() + rt_bytes, [&] {
+                        rx[2]->z_rhot(em(), dim3(z.blocks), dim3(z.threads), z.lds, ra, g, z.P, tab);
+                    });
             } else {
                 ZCfg zc = zcfg(nbuf);
                 timed(K_ZD, bytes, [&] {

@@ -41,19 +41,20 @@ def heterogeneous_nonlinear_2d(n=512, steps=2000, dtype=DTypePrecision.float32):
     return grid, med, src, full, KWaveSensor(mask=line, record=RecordField.p), opts
 
 
-def throughput_3d(n=256, steps=1000, dtype=DTypePrecision.float32, plane=168):
-    """3D heterogeneous power-law run: n^3 (256), dx = 1e-4 m, PML 10, smoothP0; c and rho from
-    random fields (seeds 3 / 4, correlation 20 cells) in [1400, 1600] / [900, 1100]; 0.5
+def throughput_3d(n=256, steps=1000, dtype=DTypePrecision.float32, plane=168, shape=None):
+    """3D heterogeneous power-law run: n^3 (256) or `shape`, dx = 1e-4 m, PML 10, smoothP0; c and
+    rho from random fields (seeds 3 / 4, correlation 20 cells) in [1400, 1600] / [900, 1100]; 0.5
     dB/(MHz^y cm), y = 1.1, with dispersion, linear; p0 = ball of radius 8 cells at the centre,
     1 Pa; CFL 0.3; a sensor plane x = `plane` recording p and uz."""
     dx = 1e-4
-    c = random_field((n, n, n), 3, 20.0, 1400.0, 1600.0)
-    rho = random_field((n, n, n), 4, 20.0, 900.0, 1100.0)
+    shape = tuple(shape) if shape is not None else (n, n, n)
+    c = random_field(shape, 3, 20.0, 1400.0, 1600.0)
+    rho = random_field(shape, 4, 20.0, 900.0, 1100.0)
     med = KWaveMedium(c, rho, alphaCoeff=0.5, alphaPower=1.1, alphaMode=AbsorptionMode.powerLaw)
-    grid = KWaveGrid(n, dx, n, dx, n, dx)
+    grid = KWaveGrid(shape[0], dx, shape[1], dx, shape[2], dx)
     grid.setTime(steps, 0.3 * dx / float(c.max()))
-    src = KWaveSource(p0=makeBall(n, n, n, radius=8))
-    mask = np.zeros((n, n, n), np.float32)
+    src = KWaveSource(p0=makeBall(*shape, radius=8))
+    mask = np.zeros(shape, np.float32)
     mask[plane, :, :] = 1
     sens = KWaveSensor(mask=mask, record=RecordField.p | RecordField.uz)
     return grid, med, src, sens, SimulationOptions(pmlSize=10, smoothP0=True, dtype=dtype)

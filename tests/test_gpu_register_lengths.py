@@ -67,6 +67,22 @@ def test_2d_register_density_variants(absorbing, mode, nonlinear):
           KWaveSensor(), ["pFinal"], pmlSize=10)
 
 
+@pytest.mark.parametrize("mode,nonlinear", [(AbsorptionMode.powerLaw, False), (AbsorptionMode.noDispersion, False),
+                                            (AbsorptionMode.powerLaw, True), (AbsorptionMode.noDispersion, True)])
+def test_2d_512_rows_fft_absorption(mode, nonlinear):
+    """Length-512 rows with FFT power-law absorption: the density kernel leaves rhoT and, with
+    dispersion, R2C_z(rhoT) to a pass of their own (reg_z_rhot); linear and B/A, with and without
+    dispersion, against the oracle (pFinal and the p series of a line sensor)."""
+    shape = (48, 512)
+    med = _medium(shape, seed=75, absorbing=True, nonlinear=nonlinear, mode=mode, y=1.3)
+    grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4)
+    grid.setTime(100, 0.3 * 1e-4 / float(np.max(med.soundSpeed)))
+    sens = np.zeros(shape, np.float32)
+    sens[30, :] = 1
+    check(f"2d_512_rows_{mode.value}_{nonlinear}", grid, med, KWaveSource(p0=makeDisc(*shape, radius=6)),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal), ["p", "pFinal"], pmlSize=10)
+
+
 @pytest.mark.parametrize("shape", [(32, 96, 160), (160, 32, 96), (96, 128, 32), (128, 192, 32)])
 def test_3d_register_lengths(shape):
     """3D heterogeneous power-law medium with B/A, p0 ball plus a velocity source patch; plane
