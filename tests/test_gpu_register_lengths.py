@@ -83,6 +83,21 @@ def test_2d_512_rows_fft_absorption(mode, nonlinear):
           KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal), ["p", "pFinal"], pmlSize=10)
 
 
+def test_3d_512_rows_fft_absorption():
+    """3D grid with length-512 rows and FFT power-law absorption (the summed-density pass over three
+    axes), heterogeneous medium with B/A, against the oracle."""
+    shape = (12, 16, 512)
+    c = random_field(shape, 77, 4.0, 1400.0, 1600.0)
+    rho = random_field(shape, 78, 4.0, 900.0, 1100.0)
+    med = KWaveMedium(c, rho, alphaCoeff=0.6, alphaPower=1.2, alphaMode=AbsorptionMode.powerLaw, bOnA=5.0)
+    grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4, shape[2], 1e-4)
+    grid.setTime(40, 0.3 * 1e-4 / float(c.max()))
+    sens = np.zeros(shape, np.float32)
+    sens[6, 8, :] = 1
+    check("3d_512_rows_powerlaw", grid, med, KWaveSource(p0=makeBall(*shape, radius=3)),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal), ["p", "pFinal"], pmlSize=4)
+
+
 @pytest.mark.parametrize("shape", [(32, 96, 160), (160, 32, 96), (96, 128, 32), (128, 192, 32)])
 def test_3d_register_lengths(shape):
     """3D heterogeneous power-law medium with B/A, p0 ball plus a velocity source patch; plane
