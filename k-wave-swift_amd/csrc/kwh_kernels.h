@@ -62,6 +62,12 @@ struct BIdx {
     int x, nx, y;
     int remap;
     __device__ __forceinline__ int tile() const { return remap ? xcd_tile(x, nx) : x; }
+    // XCD-aware order with each XCD's run of tiles walked from its end (single launches only)
+    __device__ __forceinline__ int tile_rev() const {
+        if (!remap) return x;
+        const int t = xcd_tile(x, nx), per = nx / 8;
+        return (nx < 16 || t >= per * 8) ? t : (t / per) * per + (per - 1 - t % per);
+    }
 };
 __device__ __forceinline__ BIdx bidx_here() { return BIdx{(int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y, 1}; }
 
@@ -83,6 +89,7 @@ struct StridedArgs {
     int64_t ncols;
     int lg_ntr;
     int pitch;
+    int walk_rev;    // register strided pass: walk each XCD's run of tiles from its end
 };
 
 template <typename T>

@@ -1033,7 +1033,7 @@ __device__ __forceinline__ void reg_strided_body(const BIdx& bi, const StridedAr
     const int w = threadIdx.x % NTR, j = threadIdx.x / NTR;
     LaneTw<T, J, E> lt;
     lt.load(ldsb, tw, j);
-    const int tile = bi.tile();
+    const int tile = a.walk_rev ? bi.tile_rev() : bi.tile();
     const int64_t c = (int64_t)tile * NTR + w;
     const bool valid = c < a.ncols;
     const int64_t cc = valid ? c : 0;
