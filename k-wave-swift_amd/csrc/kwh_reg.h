@@ -62,8 +62,10 @@
 #define KWH_EOS_VERSIONED 1
 #endif
 
+// Largest J whose middle pass uses the cross-lane (DPP) inner radix by default: none, as it
+// measured slower than the LDS transpose at every length (KWH_MIDDLE_XLANE=1 opts in).
 #ifndef KWH_MIDDLE_XLANE_MAXJ
-#define KWH_MIDDLE_XLANE_MAXJ 4
+#define KWH_MIDDLE_XLANE_MAXJ 0
 #endif
 
 namespace kwh {
