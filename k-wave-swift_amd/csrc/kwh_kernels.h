@@ -153,6 +153,9 @@ struct MidArgs {
     int64_t stride, inner, ncols;
     int lg_ntr, pitch;
     double scale;  // 1/N normalisation folded into the operator
+    // input 0 is zero outside the middle-axis slabs [slab_lo, slab_lo + slab_n) (slab_n = 0: dense
+    // input): the register middle pass then forms its forward transform directly from those slabs
+    int slab_lo, slab_n;
 };
 
 template <typename T>

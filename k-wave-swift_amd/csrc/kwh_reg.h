@@ -1108,7 +1108,7 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
     const T* kap_src = nullptr;   // table currently held in kap (inputs sharing one table load it once)
     for (int f = 0; f < a.nin; ++f) {
         cx<T> X[E];
-        {
+        if (!(f == 0 && a.slab_n > 0)) {
             const cx<T>* q = reinterpret_cast<const cx<T>*>(a.in[f]) + base;
 #pragma unroll
             for (int m = 0; m < E; ++m) {
@@ -1132,10 +1132,33 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
                 kap_src = ktab;
             }
         };
-        if (KWH_MIDDLE_KAPPA_EARLY) load_table();
-        if constexpr (XL) xl.template to_block<false>(X);
-        else S::template transform<false>(X, lds, j, w, lt);
-        if (!KWH_MIDDLE_KAPPA_EARLY) load_table();
+        if (f == 0 && a.slab_n > 0) {
+            // input nonzero only in slabs x0 = slab_lo + s: X[k] = sum_s x[x0] W_L^(x0 k), formed
+            // directly (slab_n loads and complex products per element instead of E loads and a
+            // transform); k = kb + kst m is the element slot m holds after a forward transform
+            load_table();
+            const cx<T>* col = reinterpret_cast<const cx<T>*>(a.in[0]) + (base - (int64_t)j * a.stride);
+            const cx<T>* twl = tab.tw[a.axis];
+#pragma unroll
+            for (int m = 0; m < E; ++m) X[m] = cx<T>{T(0), T(0)};
+            for (int sl = 0; sl < a.slab_n; ++sl) {
+                const int x0 = a.slab_lo + sl;
+                // W_L^(x0 k) = W_L^(x0 kb) W_L^(x0 kst m): a per-lane factor times one that is
+                // uniform over the wave (scalar loads)
+                const cx<T> xs = cmul(col[(int64_t)x0 * a.stride], twl[(x0 * kb) % S::L]);
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const cx<T> w = twl[(x0 * kst * m) % S::L];
+                    X[m].x += xs.x * w.x - xs.y * w.y;
+                    X[m].y += xs.x * w.y + xs.y * w.x;
+                }
+            }
+        } else {
+            if (KWH_MIDDLE_KAPPA_EARLY) load_table();
+            if constexpr (XL) xl.template to_block<false>(X);
+            else S::template transform<false>(X, lds, j, w, lt);
+            if (!KWH_MIDDLE_KAPPA_EARLY) load_table();
+        }
         for (int o = 0; o < a.nop; ++o) {
             const MidOp op = a.op[o];
             if (op.in != f) continue;

@@ -597,7 +597,10 @@ struct SimT final : kwh_sim {
         });
     }
 
-    void middle(int kid, std::vector<const void*> ins, std::vector<MidOp> ops) {
+    // slab_n > 0: input 0 is zero outside the middle-axis slabs [slab_lo, slab_lo + slab_n); the
+    // register pass then forms its forward transform from those slabs (up to kSlabDirect of them)
+    static constexpr int kSlabDirect = 4;
+    void middle(int kid, std::vector<const void*> ins, std::vector<MidOp> ops, int slab_lo_ = 0, int slab_n_ = 0) {
         SCfg c = scfg(mid_axis, 3);
         MidArgs a{};
         a.nin = (int)ins.size();
@@ -619,7 +622,12 @@ struct SimT final : kwh_sim {
                 tabs_used.push_back(o.table);
         }
         const int ntab = (int)tabs_used.size();
-        const double bytes = spec_bytes() * (a.nin + a.nop) + (double)g.NH * sizeof(T) * ntab;
+        double bytes = spec_bytes() * (a.nin + a.nop) + (double)g.NH * sizeof(T) * ntab;
+        if (o && slab_n_ > 0 && slab_n_ <= kSlabDirect) {
+            a.slab_lo = slab_lo_;
+            a.slab_n = slab_n_;
+            bytes -= spec_bytes() * (1.0 - (double)slab_n_ / a.L);
+        }
         if (o) {
             for (int f = 0; f < a.nin; ++f) {   // one k-space table per input (loaded once in-kernel)
                 const void* t0 = nullptr;
@@ -1460,7 +1468,7 @@ struct SimT final : kwh_sim {
                 SlabScope sc(this, slab_n);
                 forward_to_middle(sr, ss, T(1));
             }
-            middle(K_MT, {src_spec.p}, {MidOp{OP_TAB, 0, 0, scr_spec.p, skappa.p}});
+            middle(K_MT, {src_spec.p}, {MidOp{OP_TAB, 0, 0, scr_spec.p, skappa.p}}, slab_lo, slab_n);
         } else {
             forward_to_middle(scr_real.as<T>(), scr_spec.as<C>(), T(1));
             middle(K_MT, {scr_spec.p}, {MidOp{OP_TAB, 0, 0, scr_spec.p, skappa.p}});
