@@ -90,6 +90,13 @@ struct StridedArgs {
     int lg_ntr;
     int pitch;
     int walk_rev;    // register strided pass: walk each XCD's run of tiles from its end
+    // register inverse pass with two outputs from field dual - 1 (dual = 0: none): that field becomes
+    // IFFT(d1[n] X) in place (n: index along the pass axis) and dual_out gets IFFT(d2[col] X)
+    // (col: the column's index within `inner`); both tables are cx<T>
+    int dual;
+    void* dual_out;
+    const void* d1;
+    const void* d2;
 };
 
 template <typename T>

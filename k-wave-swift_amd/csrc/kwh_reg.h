@@ -1026,7 +1026,7 @@ constexpr int reg_ntr_max() {
     return n;
 }
 
-template <typename T, int J, int E, int NTR, bool INV>
+template <typename T, int J, int E, int NTR, bool INV, bool DUAL = false>
 __device__ __forceinline__ void reg_strided_body(const BIdx& bi, const StridedArgs& a, const cx<T>* __restrict__ tw) {
     using S = RegCol<T, J, E, NTR>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1051,6 +1051,27 @@ __device__ __forceinline__ void reg_strided_body(const BIdx& bi, const StridedAr
             q += step;
         }
     }
+    if constexpr (DUAL) {
+        if (bi.y == a.dual - 1) {   // uniform over the workgroup
+            cx<T> v2[E];
+            const cx<T> d2 = reinterpret_cast<const cx<T>*>(a.d2)[cc % a.inner];
+            const cx<T>* d1 = reinterpret_cast<const cx<T>*>(a.d1) + j;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                v2[m] = cmul(v[m], d2);
+                v[m] = cmul(v[m], d1[J * m]);
+            }
+            S::template transform<INV>(v2, lds, j, w, lt);
+            if (valid) {
+                cx<T>* q = reinterpret_cast<cx<T>*>(a.dual_out) + base + (int64_t)j * a.stride;
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    *q = v2[m];
+                    q += step;
+                }
+            }
+        }
+    }
     S::template transform<INV>(v, lds, j, w, lt);
     if (valid) {
         cx<T>* q = data;
@@ -1061,9 +1082,9 @@ __device__ __forceinline__ void reg_strided_body(const BIdx& bi, const StridedAr
         }
     }
 }
-template <typename T, int J, int E, int NTR, bool INV>
+template <typename T, int J, int E, int NTR, bool INV, bool DUAL = false>
 __global__ void __launch_bounds__(256) reg_strided(StridedArgs a, const cx<T>* __restrict__ tw) {
-    reg_strided_body<T, J, E, NTR, INV>(bidx_here(), a, tw);
+    reg_strided_body<T, J, E, NTR, INV, DUAL>(bidx_here(), a, tw);
 }
 
 // Middle pass: forward along the middle axis, operators, inverse, store. Every table-using

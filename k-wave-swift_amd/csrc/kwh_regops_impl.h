@@ -21,6 +21,8 @@ struct RegImpl {
     template <int N, bool INV>
     using StrB = Body<&reg_strided_body<T, J, E, N, INV>, &reg_strided<T, J, E, N, INV>, 1, false>;
     template <int N>
+    using StrDualB = Body<&reg_strided_body<T, J, E, N, true, true>, &reg_strided<T, J, E, N, true, true>, 1, false>;
+    template <int N>
     using MidB = Body<&reg_middle_body<T, J, E, N>, &reg_middle<T, J, E, N>, reg_waves<T, E>(KWH_MIDDLE_WAVES), false>;
     template <int N>
     using MidXB = Body<&reg_middle_body<T, J, E, N, true>, &reg_middle<T, J, E, N, true>,
@@ -95,6 +97,11 @@ struct RegImpl {
     }
     static void strided(const Emit& em, dim3 gr, dim3 bl, size_t lds, const StridedArgs& a, const cx<T>* tw, bool inv,
                         int ntr) {
+        if (inv && a.dual > 0) {
+            if (ntr == NTR) em.go<StrDualB<NTR>>(gr, bl, lds, a, tw);
+            else em.go<StrDualB<NTR / 2>>(gr, bl, lds, a, tw);
+            return;
+        }
         if (ntr == NTR) {
             if (inv) em.go<StrB<NTR, true>>(gr, bl, lds, a, tw);
             else em.go<StrB<NTR, false>>(gr, bl, lds, a, tw);

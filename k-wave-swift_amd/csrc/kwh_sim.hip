@@ -410,6 +410,7 @@ struct SimT final : kwh_sim {
         // around them walk theirs from the start, so each pass begins with the x planes the one
         // before touched last (still in the memory-side cache); 256^3 y passes 58.9 -> 55.5 us
         int strided_rev = 1;
+        int grad_dual = 1;   // KWH_GRAD_DUAL=0: three x-pass outputs for the gradient
     };
     Schedule sched;
     bool use_sq = true;
@@ -562,7 +563,10 @@ struct SimT final : kwh_sim {
         const int L = o->J * o->E;
         return ((size_t)L * ntr + o->J + L) * sizeof(C);   // slab + pad + twiddle table
     }
-    void strided_v(int axis, bool inv, const std::vector<void*>& fields) {
+    // dual_field >= 0 (register inverse passes only): that field F also yields dual_out =
+    // IFFT(d2[column] F) and itself becomes IFFT(d1[n] F) (StridedArgs::dual)
+    void strided_v(int axis, bool inv, const std::vector<void*>& fields, int dual_field = -1, void* dual_out = nullptr,
+                   const void* d1 = nullptr, const void* d2 = nullptr) {
         SCfg c = scfg(axis, 2);
         if (c.L <= 1) return;
         StridedArgs a{};
@@ -574,7 +578,15 @@ struct SimT final : kwh_sim {
         a.stride = c.stride;
         a.inner = c.inner;
         a.ncols = c.ncols;
-        const double bytes = 2.0 * spec_bytes() * nf;
+        double bytes = 2.0 * spec_bytes() * nf;
+        if (dual_field >= 0) {
+            KWH_REQUIRE(inv && rx[axis] && dual_field < nf, KWH_EUNSUPPORTED, "two-output pass needs a register inverse");
+            a.dual = dual_field + 1;
+            a.dual_out = dual_out;
+            a.d1 = d1;
+            a.d2 = d2;
+            bytes += spec_bytes();
+        }
         a.walk_rev = sched.strided_rev;
         if (const RegOps<T>* o = rx[axis]) {
             const int ntr = reg_ntr(o);
@@ -734,6 +746,7 @@ struct SimT final : kwh_sim {
         if (const char* e = std::getenv("KWH_NTR_SHIFT")) sched.ntr_shift = std::atoi(e) != 0 ? 1 : 0;
         if (const char* e = std::getenv("KWH_P_SHIFT")) sched.p_shift = std::max(0, std::min(2, std::atoi(e)));
         if (const char* e = std::getenv("KWH_STRIDED_REV")) sched.strided_rev = std::atoi(e) != 0 ? 1 : 0;
+        if (const char* e = std::getenv("KWH_GRAD_DUAL")) sched.grad_dual = std::atoi(e) != 0 ? 1 : 0;
         scal_pool.alloc((size_t)64 * kScalarRep * sizeof(T));
         zero_map = mk_scalar(T(0));
         taus = etas = bonas = zero_map;
@@ -1514,6 +1527,10 @@ struct SimT final : kwh_sim {
         P_valid = true;
     }
 
+    // 3-D split passes with register kernels on the x and y axes: the gradient's y and z
+    // derivatives are applied in the y inverse (strided_v two-output form)
+    bool grad_dual() const { return dim == 3 && plain1 && mid_axis == 0 && rx[0] && rx[1] && !axisym && sched.grad_dual; }
+
     void main_step(int t) {
         // k-space-corrected additive sources for this step
         const T* cu[3] = {nullptr, nullptr, nullptr};
@@ -1535,12 +1552,18 @@ struct SimT final : kwh_sim {
         C* P = spec[0].as<C>();
         // velocity: grad p (velStep :669-678)
         if (plain1) strided(1, false, {P});
-        {
+        if (grad_dual()) {
+            // the x pass writes Gx = IFFT_x(d_x kappa X) and Q = IFFT_x(kappa X); the y inverse forms
+            // G_y = IFFT_y(d_y Q) and G_z = IFFT_y(d_z Q) from one read of Q (d_y, d_z do not vary
+            // along x): one spectrum written and read less than three x-pass outputs
+            middle(K_MG, {P}, {MidOp{OP_DDXP_KAPPA, 0, 0, spec[0].p, kappa.p}, MidOp{OP_TAB, 0, 0, spec[1].p, kappa.p}});
+            strided_v(1, true, {spec[0].p, spec[1].p}, 1, spec[2].p, tab.ddxp[1], tab.ddxp[2]);
+        } else {
             std::vector<MidOp> ops;
             for (int a = 0; a < dim; ++a) ops.push_back(MidOp{OP_DDXP_KAPPA, a + g.off, 0, spec[a].p, kappa.p});
             middle(K_MG, {P}, ops);
+            if (plain1) strided(1, true, {spec[0].p, spec[1].p, spec[2].p});
         }
-        if (plain1) strided(1, true, {spec[0].p, spec[1].p, spec[2].p});
         {
             VelArgs<T> va{};
             for (int a = 0; a < dim; ++a) {
