@@ -475,6 +475,10 @@ __global__ void __launch_bounds__(256) z_inverse_real(const cx<T>* spec, T* __re
 // written over the gradient spectrum (KSpaceFirstOrder.swift:669-678, sources :705-711).
 template <typename T> struct VelArgs {
     cx<T>* G[3];           // in: gradient spectra after the outer inverse passes; out: R2C_z(u_a)
+    // plane kernels: when gdual is set, axis a reads Gin[a] instead of G[a], and axes 1 and 2 apply
+    // d_y (per row) and d_z (per column) of the +dx/2 gradient to the loaded y-z spectrum
+    const cx<T>* Gin[3];
+    int gdual;
     T* u[3];
     SMap<T> dtr[3];        // dt / rho0 staggered along a
     SMap<T> corr[3];       // additive k-space-corrected source contribution (zero scalar when absent)

@@ -240,7 +240,8 @@ template <typename T> __device__ __forceinline__ void gstp(CPair<T>* p, const CP
 // then slot 0 of every row packs columns 0 and M as G0 + i GN. The caller places a barrier
 // before the slots are read.
 template <typename T, int LY, int LZ, int NT>
-__device__ __forceinline__ void plane_load_spec(cx<T>* pl, const cx<T>* __restrict__ src) {
+__device__ __forceinline__ void plane_load_spec(cx<T>* pl, const cx<T>* __restrict__ src,
+                                                const cx<T>* __restrict__ op = nullptr, bool op_rows = false) {
     using P = PlaneShape<T, LY, LZ>;
     constexpr int NP = LY * P::MZH / 2, IP = (NP + NT - 1) / NT;
     static_assert((LY * P::MZH) % 2 == 0, "plane copies move complex pairs");
@@ -260,6 +261,14 @@ __device__ __forceinline__ void plane_load_spec(cx<T>* pl, const cx<T>* __restri
         }
     }
     __syncthreads();
+    if (op) {   // uniform: an operator per row (ky) or per column (kz) of the unpacked spectrum
+        for (int i = threadIdx.x; i < LY * P::MZH; i += NT) {
+            const int r = i / P::MZH, k = i - r * P::MZH;
+            cx<T>& e = pl[r * P::PITCH + k];
+            e = cmul(e, op[op_rows ? r : k]);
+        }
+        __syncthreads();
+    }
     for (int r = threadIdx.x; r < LY; r += NT) {
         const cx<T> g0 = pl[r * P::PITCH], gn = pl[r * P::PITCH + P::M];
         pl[r * P::PITCH] = cx<T>{g0.x - gn.y, g0.y + gn.x};
@@ -299,8 +308,9 @@ __device__ __forceinline__ void plane_store_spec(cx<T>* pl, cx<T>* __restrict__ 
 
 // Inverse transform of the spectrum plane into the real view (ends with a barrier).
 template <typename T, int LY, int LZ, int NT>
-__device__ __forceinline__ void plane_inverse(cx<T>* pl, const cx<T>* __restrict__ src) {
-    plane_load_spec<T, LY, LZ, NT>(pl, src);
+__device__ __forceinline__ void plane_inverse(cx<T>* pl, const cx<T>* __restrict__ src,
+                                              const cx<T>* __restrict__ op = nullptr, bool op_rows = false) {
+    plane_load_spec<T, LY, LZ, NT>(pl, src, op, op_rows);
     __syncthreads();
     plane_cols<T, LY, LZ, NT, true>(pl);
     plane_c2r_split<T, LY, LZ, NT>(pl);
@@ -478,7 +488,10 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
     const int64_t rb = (int64_t)x * LY * LZ, sb = (int64_t)x * LY * P::MZH;
     sfor<0, 3>([&](auto cax) {
         constexpr int ax = decltype(cax)::value;
-        plane_inverse<T, LY, LZ, NT>(pl, a.G[ax] + sb);
+        if (a.gdual && ax > 0)
+            plane_inverse<T, LY, LZ, NT>(pl, a.Gin[ax] + sb, tab.ddxp[ax], ax == 1);
+        else
+            plane_inverse<T, LY, LZ, NT>(pl, (a.gdual ? a.Gin[ax] : a.G[ax]) + sb);
         T* u = a.u[ax] + rb;
         const SMap<T> dtr = a.dtr[ax], corr = a.corr[ax];
         const T* pm = tab.pmlsg[ax];

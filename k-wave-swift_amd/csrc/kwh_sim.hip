@@ -1744,7 +1744,14 @@ struct SimT final : kwh_sim {
     // (inverse, update, EOS, R2C_yz(p)). Same operations and order as main_step's split passes.
     void main_step_plane(const T* const cu[3], const T* cp) {
         C* P = spec[0].as<C>();
-        {
+        // two-output gradient: the x pass writes Gx (spec 0) and Q = IFFT_x(kappa P) (spec 1); the
+        // plane velocity kernel applies d_y / d_z to Q on load and stores R2C_yz(u_y) to spec 2 and
+        // R2C_yz(u_z) to spec 1 (after its own read of Q), so axis a's spectra live in sb[a]
+        const bool gd = sched.grad_dual && dim == 3;
+        const int sb[3] = {0, gd ? 2 : 1, gd ? 1 : 2};
+        if (gd) {
+            middle(K_MG, {P}, {MidOp{OP_DDXP_KAPPA, 0, 0, spec[0].p, kappa.p}, MidOp{OP_TAB, 0, 0, spec[1].p, kappa.p}});
+        } else {
             std::vector<MidOp> ops;
             for (int a = 0; a < dim; ++a) ops.push_back(MidOp{OP_DDXP_KAPPA, a + g.off, 0, spec[a].p, kappa.p});
             middle(K_MG, {P}, ops);
@@ -1753,8 +1760,10 @@ struct SimT final : kwh_sim {
             VelArgs<T> va{};
             bool anycorr = false;
             double bytes = 0;
+            va.gdual = gd ? 1 : 0;
             for (int a = 0; a < dim; ++a) {
-                va.G[a] = spec[a].as<C>();
+                va.G[a] = spec[sb[a]].as<C>();
+                va.Gin[a] = spec[gd && a > 0 ? 1 : a].as<C>();
                 va.u[a] = u[a].as<T>();
                 va.dtr[a] = dtrs[a];
                 va.corr[a] = cu[a] ? mk_map(cu[a]) : zero_map;
@@ -1769,15 +1778,15 @@ struct SimT final : kwh_sim {
             std::vector<const void*> ins;
             std::vector<MidOp> ops;
             for (int a = 0; a < dim; ++a) {
-                ins.push_back(spec[a].p);
-                ops.push_back(MidOp{OP_DDXN_KAPPA, a + g.off, a, spec[a].p, kappa.p});
+                ins.push_back(spec[sb[a]].p);
+                ops.push_back(MidOp{OP_DDXN_KAPPA, a + g.off, a, spec[sb[a]].p, kappa.p});
             }
             middle(K_MD, ins, ops);
         }
         {
             RhoArgs<T> ra{};
             for (int a = 0; a < dim; ++a) {
-                ra.D[a] = spec[a].as<C>();
+                ra.D[a] = spec[sb[a]].as<C>();
                 ra.rho[a] = rho[a].as<T>();
                 ra.corr_factor[a] = T(psrc.factor[a]);
                 ra.sp[a] = sparse_of(psrc, a);
