@@ -479,6 +479,10 @@ template <typename T> struct VelArgs {
     // d_y (per row) and d_z (per column) of the +dx/2 gradient to the loaded y-z spectrum
     const cx<T>* Gin[3];
     int gdual;
+    // plane kernels: the k-space-corrected source of axis corr_axis as its y-z spectrum (after the
+    // x pass), inverted in the velocity kernel itself instead of read as a real field (null: none)
+    const cx<T>* corr_spec;
+    int corr_axis;
     T* u[3];
     SMap<T> dtr[3];        // dt / rho0 staggered along a
     SMap<T> corr[3];       // additive k-space-corrected source contribution (zero scalar when absent)

@@ -486,6 +486,17 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
     plane_twiddles<T, LY, LZ, NT>(pl, tab);
     const int x = bi.x;
     const int64_t rb = (int64_t)x * LY * LZ, sb = (int64_t)x * LY * P::MZH;
+    // corrected source given as a spectrum: its plane inverted first, the real values kept in
+    // registers in the element order of the update loop
+    constexpr int NI = LY * P::M / NT;
+    cx<T> crr[CORR ? NI : 1];
+    if constexpr (CORR) {
+        if (a.corr_spec) {   // uniform over the launch
+            plane_inverse<T, LY, LZ, NT>(pl, a.corr_spec + sb);
+            plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int) { crr[it] = pl[r * P::PITCH + m]; });
+            __syncthreads();
+        }
+    }
     sfor<0, 3>([&](auto cax) {
         constexpr int ax = decltype(cax)::value;
         if (a.gdual && ax > 0)
@@ -504,7 +515,7 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
             l.dt = ld2m(dtr, rb + c);
             if constexpr (CORR) l.cr = ld2m(corr, rb + c);
             return l;
-        }, [&](int, int r, int m, int c, const VL& l) {
+        }, [&](int it, int r, int m, int c, const VL& l) {
             const cx<T> gv = pl[r * P::PITCH + m];
             const Pair<T> uu = l.uu, dt = l.dt;
             T q0, q1;
@@ -516,6 +527,10 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
             if constexpr (CORR) {
                 v0 += l.cr.a;
                 v1 += l.cr.b;
+                if (a.corr_spec && a.corr_axis == ax) {
+                    v0 += crr[it].x;
+                    v1 += crr[it].y;
+                }
             }
             pl[r * P::PITCH + m] = cx<T>{v0, v1};
             if (!sp) st2(u + c, v0, v1);

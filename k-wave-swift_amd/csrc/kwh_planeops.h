@@ -14,6 +14,10 @@ template <typename T>
 struct PlaneOps {
     int LY, LZ, NT;      // plane shape, workgroup size
     size_t lds;          // dynamic LDS per workgroup
+    // the velocity kernel inverts a corrected source's spectrum itself (VelArgs::corr_spec): faster
+    // than a separate plane inverse at 128 x 128 (batched velocity +47 us vs the inverse's 72 us),
+    // slower at 96 x 96 (+75 vs 60 us)
+    bool corr_fuse;
     // Raise the kernels' dynamic-LDS limit above the 64 KiB default (once, outside graph capture).
     hipError_t (*prepare)();
     void (*fwd_real)(const Emit&, int planes, const T*, cx<T>*, T, const Geo&, const AxisTabs<T>&);

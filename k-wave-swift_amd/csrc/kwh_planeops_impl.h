@@ -85,7 +85,8 @@ struct PlaneImpl {
     }
 
     static const PlaneOps<T>* ops() {
-        static const PlaneOps<T> o{LY, LZ, NT, S::kLds, &prepare, &fwd_real, &inv_real, &velocity, &density, &eos};
+        static const PlaneOps<T> o{LY, LZ, NT, S::kLds, LY * LZ >= 128 * 128, &prepare, &fwd_real, &inv_real,
+                                   &velocity, &density, &eos};
         return &o;
     }
 };

@@ -411,6 +411,7 @@ struct SimT final : kwh_sim {
         // before touched last (still in the memory-side cache); 256^3 y passes 58.9 -> 55.5 us
         int strided_rev = 1;
         int grad_dual = 1;   // KWH_GRAD_DUAL=0: three x-pass outputs for the gradient
+        int corr_fuse = 1;   // KWH_CORR_FUSE=0: the plane source correction as a real field
     };
     Schedule sched;
     bool use_sq = true;
@@ -747,6 +748,7 @@ struct SimT final : kwh_sim {
         if (const char* e = std::getenv("KWH_P_SHIFT")) sched.p_shift = std::max(0, std::min(2, std::atoi(e)));
         if (const char* e = std::getenv("KWH_STRIDED_REV")) sched.strided_rev = std::atoi(e) != 0 ? 1 : 0;
         if (const char* e = std::getenv("KWH_GRAD_DUAL")) sched.grad_dual = std::atoi(e) != 0 ? 1 : 0;
+        if (const char* e = std::getenv("KWH_CORR_FUSE")) sched.corr_fuse = std::atoi(e) != 0 ? 1 : 0;
         scal_pool.alloc((size_t)64 * kScalarRep * sizeof(T));
         zero_map = mk_scalar(T(0));
         taus = etas = bonas = zero_map;
@@ -1463,6 +1465,8 @@ struct SimT final : kwh_sim {
         SlabScope(const SlabScope&) = delete;
         SlabScope& operator=(const SlabScope&) = delete;
     };
+    // dst == nullptr: stop after the x pass and leave the correction's y-z spectrum in scr_spec
+    // (plane mode: the velocity plane kernel inverts it itself)
     void compute_correction(SourceDev<T>& s, int t, T* dst, double factor) {
         // Re ifft(source_kappa * fft(scatter(v))) (SourceOp.apply :167-171)
         const int64_t plane = slab_n ? g.N / (mid_axis == 0 ? g.m0 : g.m1) : 0;
@@ -1486,7 +1490,7 @@ struct SimT final : kwh_sim {
             forward_to_middle(scr_real.as<T>(), scr_spec.as<C>(), T(1));
             middle(K_MT, {scr_spec.p}, {MidOp{OP_TAB, 0, 0, scr_spec.p, skappa.p}});
         }
-        inverse_from_middle(scr_spec.as<C>(), dst, T(1), false);
+        if (dst) inverse_from_middle(scr_spec.as<C>(), dst, T(1), false);
     }
 
     void init_step() {
@@ -1534,10 +1538,23 @@ struct SimT final : kwh_sim {
     void main_step(int t) {
         // k-space-corrected additive sources for this step
         const T* cu[3] = {nullptr, nullptr, nullptr};
+        auto u_active = [&](int a) { return usrc[a].present && usrc[a].mode == KWH_SRC_ADDITIVE && t < usrc[a].len; };
+        int nu = 0;
+        for (int a = 0; a < dim; ++a) nu += u_active(a) ? 1 : 0;
+        const bool p_active = psrc.present && psrc.mode == KWH_SRC_ADDITIVE && t < psrc.len;
+        // plane mode with one corrected velocity source and no pressure correction: the velocity
+        // plane kernel inverts the correction's spectrum itself (no dense real field)
+        const bool fuse = plane && px->corr_fuse && sched.corr_fuse && nu == 1 && !p_active;
+        int corr_axis = -1;
         for (int a = 0; a < dim; ++a)
-            if (usrc[a].present && usrc[a].mode == KWH_SRC_ADDITIVE && t < usrc[a].len) {
-                compute_correction(usrc[a], t, corr_u[a].as<T>(), usrc[a].factor[a]);
-                cu[a] = corr_u[a].as<T>();
+            if (u_active(a)) {
+                if (fuse) {
+                    compute_correction(usrc[a], t, nullptr, usrc[a].factor[a]);
+                    corr_axis = a;
+                } else {
+                    compute_correction(usrc[a], t, corr_u[a].as<T>(), usrc[a].factor[a]);
+                    cu[a] = corr_u[a].as<T>();
+                }
             }
         const T* cp = nullptr;
         if (psrc.present && psrc.mode == KWH_SRC_ADDITIVE && t < psrc.len) {
@@ -1546,7 +1563,7 @@ struct SimT final : kwh_sim {
             cp = corr_p.as<T>();
         }
         if (plane) {
-            main_step_plane(cu, cp);
+            main_step_plane(cu, cp, corr_axis);
             return;
         }
         C* P = spec[0].as<C>();
@@ -1742,7 +1759,7 @@ struct SimT final : kwh_sim {
     // The step in plane mode: x pass of P (already R2C_yz(p)), plane velocity kernel (y and z
     // inverse, update, y and z forward per axis), x pass of the divergence, plane density kernel
     // (inverse, update, EOS, R2C_yz(p)). Same operations and order as main_step's split passes.
-    void main_step_plane(const T* const cu[3], const T* cp) {
+    void main_step_plane(const T* const cu[3], const T* cp, int corr_axis) {
         C* P = spec[0].as<C>();
         // two-output gradient: the x pass writes Gx (spec 0) and Q = IFFT_x(kappa P) (spec 1); the
         // plane velocity kernel applies d_y / d_z to Q on load and stores R2C_yz(u_y) to spec 2 and
@@ -1761,6 +1778,12 @@ struct SimT final : kwh_sim {
             bool anycorr = false;
             double bytes = 0;
             va.gdual = gd ? 1 : 0;
+            if (corr_axis >= 0) {
+                va.corr_spec = scr_spec.as<C>();
+                va.corr_axis = corr_axis;
+                anycorr = true;
+                bytes += spec_bytes();
+            }
             for (int a = 0; a < dim; ++a) {
                 va.G[a] = spec[sb[a]].as<C>();
                 va.Gin[a] = spec[gd && a > 0 ? 1 : a].as<C>();
