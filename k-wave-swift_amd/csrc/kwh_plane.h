@@ -477,7 +477,9 @@ __global__ void __launch_bounds__(NT, (plane_waves<T, LY, LZ, NT>())) plane_inv_
 // Velocity update of plane x, per axis a (KSpaceFirstOrder.swift:669-678, sources :705-711):
 // g_a = C2R_yz(G_a); u_a = pml_sg_a (pml_sg_a u_a - dt/rho_a g_a) [+ corrected source]
 // [+ sparse sources]; G_a = R2C_yz(u_a) for the divergence's x pass.
-template <typename T, int LY, int LZ, int NT, bool CORR>
+// CORR: 0 no corrected source, 1 the corrected source as a real field (VelArgs::corr), 2 as the y-z
+// spectrum of axis corr_axis (VelArgs::corr_spec), inverted here and held in registers.
+template <typename T, int LY, int LZ, int NT, int CORR>
 __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArgs<T>& a, const Geo& g,
                                                     const AxisTabs<T>& tab) {
     using P = PlaneShape<T, LY, LZ>;
@@ -489,13 +491,11 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
     // corrected source given as a spectrum: its plane inverted first, the real values kept in
     // registers in the element order of the update loop
     constexpr int NI = LY * P::M / NT;
-    cx<T> crr[CORR ? NI : 1];
-    if constexpr (CORR) {
-        if (a.corr_spec) {   // uniform over the launch
-            plane_inverse<T, LY, LZ, NT>(pl, a.corr_spec + sb);
-            plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int) { crr[it] = pl[r * P::PITCH + m]; });
-            __syncthreads();
-        }
+    cx<T> crr[CORR == 2 ? NI : 1];
+    if constexpr (CORR == 2) {
+        plane_inverse<T, LY, LZ, NT>(pl, a.corr_spec + sb);
+        plane_elements<T, LY, LZ, NT>([&](int it, int r, int m, int) { crr[it] = pl[r * P::PITCH + m]; });
+        __syncthreads();
     }
     sfor<0, 3>([&](auto cax) {
         constexpr int ax = decltype(cax)::value;
@@ -513,7 +513,7 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
             VL l;
             l.uu = ld2(u + c);
             l.dt = ld2m(dtr, rb + c);
-            if constexpr (CORR) l.cr = ld2m(corr, rb + c);
+            if constexpr (CORR == 1) l.cr = ld2m(corr, rb + c);
             return l;
         }, [&](int it, int r, int m, int c, const VL& l) {
             const cx<T> gv = pl[r * P::PITCH + m];
@@ -524,10 +524,11 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
             else { const Pair<T> qq = ld2(pm + 2 * m); q0 = qq.a; q1 = qq.b; }
             T v0 = q0 * (q0 * uu.a - dt.a * gv.x);
             T v1 = q1 * (q1 * uu.b - dt.b * gv.y);
-            if constexpr (CORR) {
+            if constexpr (CORR == 1) {
                 v0 += l.cr.a;
                 v1 += l.cr.b;
-                if (a.corr_spec && a.corr_axis == ax) {
+            } else if constexpr (CORR == 2) {
+                if (a.corr_axis == ax) {
                     v0 += crr[it].x;
                     v1 += crr[it].y;
                 }
@@ -550,7 +551,7 @@ __device__ __forceinline__ void plane_velocity_body(const BIdx& bi, const VelArg
     });
     (void)g;
 }
-template <typename T, int LY, int LZ, int NT, bool CORR>
+template <typename T, int LY, int LZ, int NT, int CORR>
 __global__ void __launch_bounds__(NT, (plane_waves<T, LY, LZ, NT>())) plane_velocity(VelArgs<T> a, Geo g, AxisTabs<T> tab) {
     plane_velocity_body<T, LY, LZ, NT, CORR>(bidx_here(), a, g, tab);
 }

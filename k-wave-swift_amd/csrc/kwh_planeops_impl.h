@@ -12,8 +12,10 @@ struct PlaneImpl {
     static constexpr int W = plane_waves<T, LY, LZ, NT>();
     using FwdB = Body<&plane_fwd_real_body<T, LY, LZ, NT>, &plane_fwd_real<T, LY, LZ, NT>, W, false, NT>;
     using InvB = Body<&plane_inv_real_body<T, LY, LZ, NT>, &plane_inv_real<T, LY, LZ, NT>, W, false, NT>;
-    template <bool CORR>
+    template <int CORR>
     using VelB = Body<&plane_velocity_body<T, LY, LZ, NT, CORR>, &plane_velocity<T, LY, LZ, NT, CORR>, W, false, NT>;
+    // the velocity kernel that inverts a corrected source's spectrum itself: 128 x 128 float32 only
+    static constexpr bool kCorrFuse = LY * LZ >= 128 * 128 && sizeof(T) == 4;
     template <bool NL, int MODE, bool CORR>
     using DenB = Body<&plane_density_body<T, LY, LZ, NT, NL, MODE, CORR>, &plane_density<T, LY, LZ, NT, NL, MODE, CORR>,
                       W, false, NT>;
@@ -31,10 +33,13 @@ struct PlaneImpl {
     static hipError_t prepare() {
         if (S::kLds <= 65536) return hipSuccess;
         const void* fns[] = {(const void*)&plane_fwd_real<T, LY, LZ, NT>, (const void*)&plane_inv_real<T, LY, LZ, NT>,
-                             (const void*)&plane_velocity<T, LY, LZ, NT, false>,
-                             (const void*)&plane_velocity<T, LY, LZ, NT, true>};
+                             (const void*)&plane_velocity<T, LY, LZ, NT, 0>,
+                             (const void*)&plane_velocity<T, LY, LZ, NT, 1>};
         for (const void* f : fns)
             if (hipError_t e = set_lds(f); e != hipSuccess) return e;
+        if constexpr (kCorrFuse) {
+            if (hipError_t e = set_lds((const void*)&plane_velocity<T, LY, LZ, NT, 2>); e != hipSuccess) return e;
+        }
         hipError_t e = set_density<false, 0>();
         if (e == hipSuccess) e = set_density<true, 0>();
         if (e == hipSuccess) e = set_density<false, 1>();
@@ -56,8 +61,14 @@ struct PlaneImpl {
     }
     static void velocity(const Emit& em, int planes, const VelArgs<T>& va, const Geo& g, const AxisTabs<T>& tab,
                          bool corr) {
-        if (corr) em.go<VelB<true>>(dim3(planes), dim3(NT), S::kLds, va, g, tab);
-        else em.go<VelB<false>>(dim3(planes), dim3(NT), S::kLds, va, g, tab);
+        if constexpr (kCorrFuse) {
+            if (va.corr_spec) {
+                em.go<VelB<2>>(dim3(planes), dim3(NT), S::kLds, va, g, tab);
+                return;
+            }
+        }
+        if (corr) em.go<VelB<1>>(dim3(planes), dim3(NT), S::kLds, va, g, tab);
+        else em.go<VelB<0>>(dim3(planes), dim3(NT), S::kLds, va, g, tab);
     }
     template <bool NL, int MODE>
     static void density_m(const Emit& em, int planes, const RhoArgs<T>& ra, const Geo& g, const AxisTabs<T>& tab,
@@ -85,7 +96,7 @@ struct PlaneImpl {
     }
 
     static const PlaneOps<T>* ops() {
-        static const PlaneOps<T> o{LY, LZ, NT, S::kLds, LY * LZ >= 128 * 128, &prepare, &fwd_real, &inv_real,
+        static const PlaneOps<T> o{LY, LZ, NT, S::kLds, kCorrFuse, &prepare, &fwd_real, &inv_real,
                                    &velocity, &density, &eos};
         return &o;
     }
