@@ -379,7 +379,7 @@ __device__ __forceinline__ int axis_pos(const Geo& g, int row, int n, int ax) {
 // kScalarRep, the longest register-resident row) are the map's row elements or the scalar, so the
 // register-resident kernels address every element as base + immediate. `p` / `s` keep the
 // host-side view (p == nullptr: scalar).
-constexpr int kScalarRep = 512;
+constexpr int kScalarRep = 1024;
 template <typename T> struct SMap {
     const T* p;
     T s;

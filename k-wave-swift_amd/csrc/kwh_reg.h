@@ -1,6 +1,6 @@
 // kwh_reg.h -- register-resident kernels for axis lengths L = J*E with J | E
 // (J, E) = (4, 8): 32, (8, 8): 64, (4, 24): 96, (8, 16): 128, (4, 40): 160, (8, 24): 192,
-// (16, 16): 256, (16, 32): 512.
+// (16, 16): 256, (16, 32): 512, (32, 32): 1024.
 //
 // A length-L transform is a two-step Cooley-Tukey split L = J * E. Lane j (0..J-1) of a
 // transform holds elements n = j + J*m (m = 0..E-1) in registers:
@@ -1102,15 +1102,17 @@ __device__ __forceinline__ void reg_middle_body(const BIdx& bi, const MidArgs& a
     const int w = XL ? (int)threadIdx.x / J : (int)threadIdx.x % NTR;
     const int j = XL ? (int)threadIdx.x % J : (int)threadIdx.x / NTR;
     LaneTw<T, J, E> lt;
-    XLane<T, J, E> xl;
+    struct NoXLane {};
+    std::conditional_t<XL, XLane<T, J, E>, NoXLane> xl;   // the cross-lane form exists for J <= 16 only
+    // spectrum element held in slot m: k = kb + kst * m
+    int kb = j;
     if constexpr (XL) {
         xl.init(ldsb, tab.tw[a.axis], j);
         __syncthreads();
+        kb = xl.block_start();
     } else {
         lt.load(ldsb, tab.tw[a.axis], j);
     }
-    // spectrum element held in slot m: k = kb + kst * m
-    const int kb = XL ? xl.block_start() : j;
     constexpr int kst = XL ? 1 : J;
     const int tile = bi.tile();
     const int64_t c = (int64_t)tile * NTR + w;

@@ -11,6 +11,7 @@ const RegOps<float>* reg_ops_f32_160();
 const RegOps<float>* reg_ops_f32_192();
 const RegOps<float>* reg_ops_f32_256();
 const RegOps<float>* reg_ops_f32_512();
+const RegOps<float>* reg_ops_f32_1024();
 const RegOps<double>* reg_ops_f64_64();
 const RegOps<double>* reg_ops_f64_256();
 
@@ -24,6 +25,7 @@ const RegOps<float>* reg_ops_f32(int L) {
         case 192: return reg_ops_f32_192();
         case 256: return reg_ops_f32_256();
         case 512: return reg_ops_f32_512();
+        case 1024: return reg_ops_f32_1024();
         default: return nullptr;
     }
 }

@@ -9,6 +9,7 @@ namespace kwh {
 
 template <typename T, int J, int E>
 struct RegImpl {
+    static_assert(J * E <= kScalarRep, "a scalar map's buffer must cover a row (SMap::lane)");
     static constexpr int NTR = reg_ntr_max<T, J, E>();
     static_assert(NTR >= 2, "column tile too narrow");
 
@@ -112,11 +113,13 @@ struct RegImpl {
     }
     static void middle(const Emit& em, dim3 gr, dim3 bl, size_t lds, const MidArgs& a, const AxisTabs<T>& tab,
                        const Geo& g, int ntr) {
-        if (middle_xlane()) {
-            const size_t lx = (size_t)J * E * sizeof(cx<T>);   // twiddle table only
-            if (ntr == NTR) em.go<MidXB<NTR>>(gr, bl, lx, a, tab, g);
-            else em.go<MidXB<NTR / 2>>(gr, bl, lx, a, tab, g);
-            return;
+        if constexpr (J <= 16) {   // the cross-lane network covers J = 4, 8, 16 (one DPP row)
+            if (middle_xlane()) {
+                const size_t lx = (size_t)J * E * sizeof(cx<T>);   // twiddle table only
+                if (ntr == NTR) em.go<MidXB<NTR>>(gr, bl, lx, a, tab, g);
+                else em.go<MidXB<NTR / 2>>(gr, bl, lx, a, tab, g);
+                return;
+            }
         }
         if (ntr == NTR) em.go<MidB<NTR>>(gr, bl, lds, a, tab, g);
         else em.go<MidB<NTR / 2>>(gr, bl, lds, a, tab, g);

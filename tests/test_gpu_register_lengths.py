@@ -1,5 +1,6 @@
 """GPU parity of the register-resident kernels for the axis lengths L = J*E beyond the square
-ones: 32 (4 x 8), 96 (4 x 24), 128 (8 x 16), 160 (4 x 40), 192 (8 x 24), 512 (16 x 32). Each
+ones: 32 (4 x 8), 96 (4 x 24), 128 (8 x 16), 160 (4 x 40), 192 (8 x 24), 512 (16 x 32),
+1024 (32 x 32). Each
 length is run as the contiguous axis (row kernels), the outermost axis (middle pass) and, in 3D,
 the strided axis, against the float64 oracle on identical float32 inputs; plus agreement with
 the generic Stockham kernels on the same problem and bitwise neutrality of the launch schedule.
@@ -28,7 +29,8 @@ def _medium(shape, seed, absorbing=True, nonlinear=True, mode=AbsorptionMode.pow
                        bOnA=6.0 if nonlinear else None)
 
 
-@pytest.mark.parametrize("shape", [(96, 160), (160, 96), (128, 192), (192, 128), (512, 32), (32, 512)])
+@pytest.mark.parametrize("shape", [(96, 160), (160, 96), (128, 192), (192, 128), (512, 32), (32, 512), (1024, 48),
+                                   (48, 1024)])
 def test_2d_register_lengths(shape):
     """Heterogeneous absorbing nonlinear medium, p0 disc plus an additive (k-space corrected)
     velocity line source; line sensor p / ux / uy and pFinal."""
@@ -81,6 +83,21 @@ def test_2d_512_rows_fft_absorption(mode, nonlinear):
     sens[30, :] = 1
     check(f"2d_512_rows_{mode.value}_{nonlinear}", grid, med, KWaveSource(p0=makeDisc(*shape, radius=6)),
           KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal), ["p", "pFinal"], pmlSize=10)
+
+
+def test_3d_1024_rows_fft_absorption():
+    """3D grid with length-1024 rows (32 lanes x 32 elements per transform) and FFT power-law
+    absorption with B/A, against the oracle."""
+    shape = (8, 12, 1024)
+    c = random_field(shape, 79, 4.0, 1400.0, 1600.0)
+    rho = random_field(shape, 80, 4.0, 900.0, 1100.0)
+    med = KWaveMedium(c, rho, alphaCoeff=0.6, alphaPower=1.2, alphaMode=AbsorptionMode.powerLaw, bOnA=5.0)
+    grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4, shape[2], 1e-4)
+    grid.setTime(40, 0.3 * 1e-4 / float(c.max()))
+    sens = np.zeros(shape, np.float32)
+    sens[4, 6, :] = 1
+    check("3d_1024_rows_powerlaw", grid, med, KWaveSource(p0=makeBall(*shape, radius=3)),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal), ["p", "pFinal"], pmlSize=3)
 
 
 def test_3d_512_rows_fft_absorption():
