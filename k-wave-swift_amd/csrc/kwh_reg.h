@@ -82,8 +82,10 @@ template <typename T, int J, int E>
 struct LaneTw {
     static constexpr int L = J * E;
     const cx<T>* p;
+    // tw: W_(S*L)^i; S = 2 for the half-row transforms, which read W_L from the row's W_2L table
+    template <int S = 1>
     __device__ __forceinline__ void load(cx<T>* tab_lds, const cx<T>* __restrict__ tw, int j) {
-        for (int i = threadIdx.x; i < L; i += blockDim.x) tab_lds[i] = tw[(i % E) * (i / E)];
+        for (int i = threadIdx.x; i < L; i += blockDim.x) tab_lds[i] = tw[S * ((i % E) * (i / E))];
         p = tab_lds + j;
     }
     __device__ __forceinline__ cx<T> w(int a, int n1) const { return p[J * a + E * n1]; }
@@ -195,6 +197,139 @@ struct RegRow {
     }
 };
 
+// ------------------------------------------------------------------------------------------
+// One real row of length 2L per length-L complex transform (even/odd split): z[n] = x[2n] +
+// i x[2n+1], Z = DFT_L(z); E[k] = (Z[k] + conj Z[L-k]) / 2 and O[k] = (Z[k] - conj Z[L-k]) / 2i
+// are the DFTs of the even and odd samples, and X[k] = E[k] + W_2L^k O[k] (k = 0..L; E and O are
+// L-periodic). Against a row pair as one length-2L transform this holds half the elements per
+// lane: length-512 rows run as 256-point transforms (16 x 16) instead of 512-point ones (16 x 32).
+// tw2: W_2L^i (i < 2L), the row's own twiddle table.
+// ------------------------------------------------------------------------------------------
+template <typename T, int J, int E>
+struct RegHalf {
+    using R = RegRow<T, J, E>;
+    static constexpr int L = J * E;
+
+    // Z'[k] = X[k] + conj X[L-k] + i W_2L^-k (X[k] - conj X[L-k]) = 2 (E[k] + i O[k]), k = j + J m;
+    // its unnormalised inverse is 2L (x[2n] + i x[2n+1]), the unnormalised C2R of X with the DC
+    // and Nyquist bins taken as real.
+    template <bool FULL = false>
+    __device__ static __forceinline__ void load_spectrum(const cx<T>* __restrict__ spec, int row, int nrows, int mzh,
+                                                         int j, const cx<T>* __restrict__ tw2, cx<T>* v) {
+        const bool ok = FULL || row < nrows;
+        const T mk = ok ? T(1) : T(0);
+        const cx<T>* s = spec + (int64_t)(ok ? row : 0) * mzh;
+        const cx<T>* lo = s + j;
+        const cx<T>* hi = s + (L - j);
+        const cx<T>* w = tw2 + j;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const cx<T> A = lo[J * m], B = hi[-J * m], c = w[J * m];
+            const T sx = A.x + B.x, sy = A.y - B.y;       // A + conj B
+            const T dx = A.x - B.x, dy = A.y + B.y;       // A - conj B
+            const T fx = dx * c.x + dy * c.y, fy = dy * c.x - dx * c.y;   // (A - conj B) conj(c)
+            v[m] = cx<T>{(sx - fy) * mk, (sy + fx) * mk};
+        }
+        if (j == 0) {
+            const T x0 = s[0].x * mk, xn = s[L].x * mk;
+            v[0] = cx<T>{x0 + xn, x0 - xn};
+        }
+    }
+
+    // Forward of v = z (lane-natural), then X[k] for k = j + J m and, from lane 0, X[L].
+    template <bool FULL = false>
+    __device__ static __forceinline__ void forward_store(cx<T>* spec, int row, int nrows, int mzh, cx<T>* lds, int j,
+                                                         const LaneTw<T, J, E>& tw, const cx<T>* __restrict__ tw2,
+                                                         cx<T>* v) {
+        R::template transform<false>(v, lds, j, tw);
+        cx<T>* wr = lds + j;
+#pragma unroll
+        for (int m = 0; m < E; ++m) wr[(J + 1) * m] = v[m];
+        __syncthreads();
+        const bool ok = FULL || row < nrows;
+        // slot of Z[L - k], k = j + J m > 0: (L - j + E - [j > 0]) - (J + 1) m
+        const cx<T>* pm = lds + (L - j + E - (j > 0 ? 1 : 0));
+        cx<T>* o = spec + (int64_t)(ok ? row : 0) * mzh + j;
+        const cx<T>* w = tw2 + j;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int k = j + J * m;
+            const cx<T> A = v[m];
+            const cx<T> B = k == 0 ? A : pm[-(J + 1) * m];
+            const T ex = T(0.5) * (A.x + B.x), ey = T(0.5) * (A.y - B.y);   // E[k]
+            const T ox = T(0.5) * (A.y + B.y), oy = T(0.5) * (B.x - A.x);   // O[k]
+            const cx<T> c = w[J * m];
+            if (ok) o[J * m] = cx<T>{ex + c.x * ox - c.y * oy, ey + c.x * oy + c.y * ox};
+        }
+        if (j == 0 && ok) o[L] = cx<T>{v[0].x - v[0].y, T(0)};   // X[L] = E[0] - O[0]
+        __syncthreads();
+    }
+};
+
+// Rows of one lane group in the fused row kernels (velocity, density, EOS).
+//   Row-pair form (HALF = false): rows 2q and 2q + 1 are the real and imaginary parts of one
+//   length-L transform; element m of lane j is sample n = j + J m of each row.
+//   Half form: one row q of length 2L; element m of lane j holds samples 2n (component a) and
+//   2n + 1 (component b), n = j + J m.
+// P lane groups per workgroup; rows outside the grid are clamped to row 0 and masked (va / vb).
+template <int J, int E, bool HALF>
+struct ZRows {
+    static constexpr int L = J * E;
+    static constexpr int RL = HALF ? 2 * L : L;   // samples per row
+    static constexpr int SJ = HALF ? 2 * J : J;   // sample stride between a lane's elements
+    int w, j;
+    int qa, qb;          // rows of components a and b
+    bool va, vb;
+    int64_t oa, ob;      // sample offsets of element 0 (rows clamped)
+    int na, nb;          // position of element 0 along the row
+    int row0, nrow;      // first row and number of rows of the workgroup's tile
+    template <bool FULL>
+    __device__ __forceinline__ void init(int P, const BIdx& bi, int nrows) {
+        const int tid = threadIdx.x;
+        w = tid / J;
+        j = tid - w * J;
+        const int tile = bi.tile();
+        if constexpr (HALF) {
+            qa = qb = tile * P + w;
+            row0 = tile * P;
+            nrow = P;
+            na = 2 * j;
+            nb = 2 * j + 1;
+        } else {
+            qa = 2 * (tile * P + w);
+            qb = qa + 1;
+            row0 = 2 * tile * P;
+            nrow = 2 * P;
+            na = nb = j;
+        }
+        va = FULL || qa < nrows;
+        vb = FULL || qb < nrows;
+        oa = (int64_t)(va ? qa : 0) * RL + na;
+        ob = (int64_t)(vb ? qb : 0) * RL + nb;
+    }
+};
+template <bool HALF>
+__device__ __forceinline__ bool zrows_tile_full(const BIdx& bi, int P, int nrows) {
+    return ((int64_t)bi.tile() + 1) * P * (HALF ? 1 : 2) <= nrows;
+}
+__device__ __forceinline__ bool rows_have_sparse(const int* rowptr, const Geo& g, int row0, int nrow) {
+    if (!rowptr) return false;
+    return rowptr[min(g.nrows, row0 + nrow)] > rowptr[row0];
+}
+template <typename T, int J, int E, bool HALF, bool FULL>
+__device__ __forceinline__ void zrows_load(const cx<T>* spec, const ZRows<J, E, HALF>& ps, const Geo& g,
+                                           const AxisTabs<T>& tab, cx<T>* v) {
+    if constexpr (HALF) RegHalf<T, J, E>::template load_spectrum<FULL>(spec, ps.qa, g.nrows, g.mzh, ps.j, tab.tw[2], v);
+    else RegRow<T, J, E>::template load_spectrum<FULL>(spec, ps.qa, ps.qb, g.nrows, g.mzh, ps.j, v);
+}
+template <typename T, int J, int E, bool HALF, bool FULL>
+__device__ __forceinline__ void zrows_store(cx<T>* spec, const ZRows<J, E, HALF>& ps, const Geo& g, cx<T>* lds,
+                                            const LaneTw<T, J, E>& lt, const AxisTabs<T>& tab, cx<T>* v) {
+    if constexpr (HALF)
+        RegHalf<T, J, E>::template forward_store<FULL>(spec, ps.qa, g.nrows, g.mzh, lds, ps.j, lt, tab.tw[2], v);
+    else RegRow<T, J, E>::template forward_store<FULL>(spec, ps.qa, ps.qb, g.nrows, g.mzh, lds, ps.j, lt, v);
+}
+
 // Row pair handled by a lane group: P transforms per workgroup, J lanes each.
 template <int J>
 struct RegZPos {
@@ -221,7 +356,8 @@ __device__ __forceinline__ bool reg_tile_full(const BIdx& bi, int P, int nrows) 
 #endif
 
 // Sparse sources on register-resident rows: go through LDS (natural order) when the tile has any.
-template <typename T, int J, int E>
+// Half-form rows (HALF): row lr of the tile is lane group lr, sample n in slot n / 2, component n % 2.
+template <typename T, int J, int E, bool HALF = false>
 __device__ void reg_apply_sparse(const SparseSrc<T>& s, const Geo& g, int row0, int nrow_tile, cx<T>* lds_all,
                                  int w, int j, int t, cx<T>* v) {
     using S = RegRow<T, J, E>;
@@ -238,7 +374,8 @@ __device__ void reg_apply_sparse(const SparseSrc<T>& s, const Geo& g, int row0, 
             const int n = (int)(cell - (int64_t)row * g.m2);
             const int lr = row - row0;
             const T val = s.signal[(int64_t)(s.rows == 1 ? 0 : e) * s.len + t] * s.base[e] * s.factor;
-            T& slot = comp(lds_all[(lr >> 1) * S::LP + S::pidx(n)], lr & 1);
+            T& slot = HALF ? comp(lds_all[lr * S::LP + S::pidx(n >> 1)], n & 1)
+                           : comp(lds_all[(lr >> 1) * S::LP + S::pidx(n)], lr & 1);
             slot = s.mode == 0 ? val : slot + val;
         }
     }
@@ -381,20 +518,24 @@ constexpr int kVelGroup = KWH_VEL_GROUP;   // velocity update (keeps it within t
 
 
 // Velocity update, per axis: C2R_z(G_a) -> u_a = pml_sg (pml_sg u_a - dt/rho_a g) [+ sources] -> R2C_z.
-template <typename T, int J, int E, bool CORR, bool FULL>
+// HALF: one row of length 2L per lane group (ZRows).
+template <typename T, int J, int E, bool CORR, bool FULL, bool HALF = false>
 __device__ __forceinline__ void reg_z_velocity_impl(const BIdx& bi, const VelArgs<T>& a, const Geo& g, int P,
                                                     const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
+    using Z = ZRows<J, E, HALF>;
+    constexpr int SJ = Z::SJ;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
     cx<T>* lds_all = ldsb + S::L;
-    RegZPos<J> ps(P, bi);
+    Z ps;
+    ps.template init<FULL>(P, bi, g.nrows);
     LaneTw<T, J, E> lt;
-    lt.load(ldsb, tab.tw[2], ps.j);
+    lt.template load<HALF ? 2 : 1>(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
-    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
-    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
-    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
+    const bool va = ps.va, vb = ps.vb;
+    const int ra = va ? ps.qa : 0, rb = vb ? ps.qb : 0;
+    const int64_t oa = ps.oa, ob = ps.ob;
     for (int ax = 0; ax < g.dim; ++ax) {
         const int ia = ax + g.off;
         cx<T> v[E];
@@ -404,31 +545,36 @@ __device__ __forceinline__ void reg_z_velocity_impl(const BIdx& bi, const VelArg
         const T* pm = tab.pmlsg[ia];
         const T* ua = u + oa;
         const T* ub = u + ob;
-        const T* dta = dtr.lane(oa, ps.j);
-        const T* dtb = dtr.lane(ob, ps.j);
-        const T* cra = corr.lane(oa, ps.j);
-        const T* crb = corr.lane(ob, ps.j);
-        S::template load_spectrum<FULL>(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        const T* dta = dtr.lane(oa, ps.na);
+        const T* dtb = dtr.lane(ob, ps.nb);
+        const T* cra = corr.lane(oa, ps.na);
+        const T* crb = corr.lane(ob, ps.nb);
+        zrows_load<T, J, E, HALF, FULL>(a.G[ax], ps, g, tab, v);
         S::template transform<true>(v, lds, ps.j, lt);
         KWH_SCHED_FENCE();
         auto upd = [&](int m, T qa, T qb) {
-            T xa = qa * (qa * ua[J * m] - dta[J * m] * v[m].x);
-            T xb = qb * (qb * ub[J * m] - dtb[J * m] * v[m].y);
+            T xa = qa * (qa * ua[SJ * m] - dta[SJ * m] * v[m].x);
+            T xb = qb * (qb * ub[SJ * m] - dtb[SJ * m] * v[m].y);
             if constexpr (CORR) {
-                xa += cra[J * m];
-                xb += crb[J * m];
+                xa += cra[SJ * m];
+                xb += crb[SJ * m];
             }
             v[m].x = va ? xa : T(0);
             v[m].y = vb ? xb : T(0);
         };
-        if (ia == 2) {   // staggered PML along z varies with n and is shared by both rows
-            const T* pz = pm + ps.j;
+        if (ia == 2) {   // staggered PML along z varies along the row (shared by a row pair)
+            const T* pza = pm + ps.na;
+            const T* pzb = pm + ps.nb;
 #pragma unroll
             for (int g0 = 0; g0 < E; g0 += kVelGroup) {
 #pragma unroll
                 for (int m = g0; m < g0 + kVelGroup && m < E; ++m) {
-                    const T q = pz[J * m];
-                    upd(m, q, q);
+                    if constexpr (HALF) {
+                        upd(m, pza[SJ * m], pzb[SJ * m]);
+                    } else {
+                        const T q = pza[SJ * m];
+                        upd(m, q, q);
+                    }
                 }
                 KWH_SCHED_FENCE();
             }
@@ -442,18 +588,18 @@ __device__ __forceinline__ void reg_z_velocity_impl(const BIdx& bi, const VelArg
             }
         }
 
-        if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{(int)(bi.tile() * P), P, 0, 0}))
-            reg_apply_sparse<T, J, E>(a.sp[ax], g, 2 * bi.tile() * P, 2 * P, lds_all, ps.w, ps.j, a.ctr[0], v);
+        if (rows_have_sparse(a.sp[ax].rowptr, g, ps.row0, ps.nrow))
+            reg_apply_sparse<T, J, E, HALF>(a.sp[ax], g, ps.row0, ps.nrow, lds_all, ps.w, ps.j, a.ctr[0], v);
         {
             T* wa = u + oa;
             T* wb = u + ob;
 #pragma unroll
             for (int m = 0; m < E; ++m) {
-                if (va) wa[J * m] = v[m].x;
-                if (vb) wb[J * m] = v[m].y;
+                if (va) wa[SJ * m] = v[m].x;
+                if (vb) wb[SJ * m] = v[m].y;
             }
         }
-        S::template forward_store<FULL>(a.G[ax], ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+        zrows_store<T, J, E, HALF, FULL>(a.G[ax], ps, g, lds, lt, tab, v);
     }
 }
 template <typename T, int J, int E, bool CORR>
@@ -462,6 +608,19 @@ __device__ __forceinline__ void reg_z_velocity_body(const BIdx& bi, const VelArg
         reg_z_velocity_impl<T, J, E, CORR, true>(bi, a, g, P, tab);
     else
         reg_z_velocity_impl<T, J, E, CORR, false>(bi, a, g, P, tab);
+}
+template <typename T, int J, int E, bool CORR>
+__device__ __forceinline__ void reg_z_velocity_half_body(const BIdx& bi, const VelArgs<T>& a, const Geo& g, int P,
+                                                         const AxisTabs<T>& tab) {
+    if (KWH_ROW_FULL_TILES && zrows_tile_full<true>(bi, P, g.nrows))
+        reg_z_velocity_impl<T, J, E, CORR, true, true>(bi, a, g, P, tab);
+    else
+        reg_z_velocity_impl<T, J, E, CORR, false, true>(bi, a, g, P, tab);
+}
+template <typename T, int J, int E, bool CORR>
+__global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_VELOCITY_WAVES)))
+    reg_z_velocity_half(VelArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    reg_z_velocity_half_body<T, J, E, CORR>(bidx_here(), a, g, P, tab);
 }
 
 template <typename T, int J, int E, bool CORR>
@@ -474,41 +633,45 @@ __global__ void __launch_bounds__(256, (reg_waves<T, E>(KWH_VELOCITY_WAVES)))
 // followed by the EOS (MODE 0), the EOS with Stokes absorption (MODE 2) or the inputs of the
 // FFT absorption term (MODE 1: rhoT, F1 = R2C_z(rho0 sum_a du_a), F2 = R2C_z(rhoT)). NL: B/A.
 // The total density and, for MODE 1/2, rho0 * sum_a du_a are accumulated in registers while the
-// axes are processed, so each divergence spectrum is read once. Length-512 rows in MODE 1 (FP32,
-// E = 32) keep only rho0 * sum_a du_a: rhoT and F2 come from reg_z_rhot, which re-reads the stored
-// rho_a (d reads of a real field) and frees E accumulator registers per lane. Measured on 3-D
-// power-law grids: E = 32 density 571 -> 300 + 144 us (64 x 512 x 512); at E = 24 and 40 the split
-// is no faster (115 -> 79 + 38, 245 -> 189 + 83 us), so those keep the fused kernel.
+// axes are processed, so each divergence spectrum is read once. Length-512 rows run in the half
+// form (RegHalf, E = 16 per lane) and keep everything in the kernel. As row pairs (E = 32) in MODE 1
+// they keep only rho0 * sum_a du_a: rhoT and F2 then come from reg_z_rhot, which re-reads the
+// stored rho_a (d reads of a real field) and frees E accumulator registers per lane. Measured on
+// 3-D power-law grids: E = 32 density 571 -> 300 + 144 us (64 x 512 x 512), half form 190 us; at
+// E = 24 and 40 the split is no faster (115 -> 79 + 38, 245 -> 189 + 83 us), so those keep the
+// fused kernel.
 template <typename T, int E, int MODE>
 constexpr bool density_rhot_split() { return MODE == 1 && E == 32 && sizeof(T) == 4; }
 template <typename T, int J, int E, bool NL, int MODE, bool CORR>
 constexpr int density_waves() {
     return (NL || CORR) ? 1 : reg_waves<T, E>(MODE == 0 ? KWH_DENSITY_WAVES : 2);
 }
-template <typename T, int J, int E, bool NL, int MODE, bool CORR, bool FULL>
+template <typename T, int J, int E, bool NL, int MODE, bool CORR, bool FULL, bool HALF = false>
 __device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P,
                                                    const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
+    using Z = ZRows<J, E, HALF>;
+    constexpr int SJ = Z::SJ;
     constexpr bool ACC = MODE != 0;   // accumulate rho0 * sum_a du_a
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
     cx<T>* lds_all = ldsb + S::L;
-    RegZPos<J> ps(P, bi);
+    Z ps;
+    ps.template init<FULL>(P, bi, g.nrows);
     LaneTw<T, J, E> lt;
-    lt.load(ldsb, tab.tw[2], ps.j);
+    lt.template load<HALF ? 2 : 1>(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = lds_all + ps.w * S::LP;
-    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
-    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
-    const int row0 = 2 * bi.tile() * P;
-    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
-    const T* r0pa = a.rho0.lane(oa, ps.j);
-    const T* r0pb = a.rho0.lane(ob, ps.j);
+    const bool va = ps.va, vb = ps.vb;
+    const int ra = va ? ps.qa : 0, rb = vb ? ps.qb : 0;
+    const int64_t oa = ps.oa, ob = ps.ob;
+    const T* r0pa = a.rho0.lane(oa, ps.na);
+    const T* r0pb = a.rho0.lane(ob, ps.nb);
     // rho0 of the row pair held in registers across the axes (one HBM read instead of one per axis)
     constexpr bool HOLD = KWH_DENSITY_HOLD_RHO0 && E <= 16 && sizeof(T) == 4;
     cx<T> r0h[HOLD ? E : 1];
     if constexpr (HOLD) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) r0h[m] = cx<T>{r0pa[J * m], r0pb[J * m]};
+        for (int m = 0; m < E; ++m) r0h[m] = cx<T>{r0pa[SJ * m], r0pb[SJ * m]};
     }
     constexpr bool RT = !density_rhot_split<T, E, MODE>();
     cx<T> rt[RT ? E : 1];
@@ -519,8 +682,8 @@ __device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs
         for (int m = 0; m < E; ++m) {
             T sa = 0, sb = 0;
             for (int ax = 0; ax < g.dim; ++ax) {
-                sa += a.rho[ax][oa + J * m];
-                sb += a.rho[ax][ob + J * m];
+                sa += a.rho[ax][oa + SJ * m];
+                sb += a.rho[ax][ob + SJ * m];
             }
             rp[m] = cx<T>{sa, sb};
         }
@@ -533,21 +696,21 @@ __device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs
         T* rho = a.rho[ax];
         const T* rha = rho + oa;
         const T* rhb = rho + ob;
-        S::template load_spectrum<FULL>(a.D[ax], ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+        zrows_load<T, J, E, HALF, FULL>(a.D[ax], ps, g, tab, v);
         cx<T> rpre[PRE ? E : 1];
         if constexpr (PRE) {
 #pragma unroll
-            for (int m = 0; m < E; ++m) rpre[m] = cx<T>{rha[J * m], rhb[J * m]};
+            for (int m = 0; m < E; ++m) rpre[m] = cx<T>{rha[SJ * m], rhb[SJ * m]};
         }
         S::template transform<true>(v, lds, ps.j, lt);
         KWH_SCHED_FENCE();
         const T* pm = tab.pml[ia];
         const T cf = a.corr_factor[ax];
-        const T* cra = a.corr.lane(oa, ps.j);
-        const T* crb = a.corr.lane(ob, ps.j);
+        const T* cra = a.corr.lane(oa, ps.na);
+        const T* crb = a.corr.lane(ob, ps.nb);
         auto upd = [&](int m, T qa, T qb) {
-            const T r0a = HOLD ? r0h[m].x : r0pa[J * m];
-            const T r0b = HOLD ? r0h[m].y : r0pb[J * m];
+            const T r0a = HOLD ? r0h[m].x : r0pa[SJ * m];
+            const T r0b = HOLD ? r0h[m].y : r0pb[SJ * m];
             const T da = r0a * v[m].x, db = r0b * v[m].y;
             if constexpr (ACC) {
                 if (ax == 0) sd[m] = cx<T>{da, db};
@@ -558,22 +721,27 @@ __device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs
                 ma *= (T(2) * rp[m].x + r0a) / r0a;
                 mb *= (T(2) * rp[m].y + r0b) / r0b;
             }
-            T xa = qa * (qa * (PRE ? rpre[m].x : rha[J * m]) - ma);
-            T xb = qb * (qb * (PRE ? rpre[m].y : rhb[J * m]) - mb);
+            T xa = qa * (qa * (PRE ? rpre[m].x : rha[SJ * m]) - ma);
+            T xb = qb * (qb * (PRE ? rpre[m].y : rhb[SJ * m]) - mb);
             if constexpr (CORR) {
-                xa += cra[J * m] * cf;
-                xb += crb[J * m] * cf;
+                xa += cra[SJ * m] * cf;
+                xb += crb[SJ * m] * cf;
             }
             v[m] = cx<T>{xa, xb};
         };
         if (ia == 2) {
-            const T* pz = pm + ps.j;
+            const T* pza = pm + ps.na;
+            const T* pzb = pm + ps.nb;
 #pragma unroll
             for (int g0 = 0; g0 < E; g0 += kGroup) {
 #pragma unroll
                 for (int m = g0; m < g0 + kGroup && m < E; ++m) {
-                    const T q = pz[J * m];
-                    upd(m, q, q);
+                    if constexpr (HALF) {
+                        upd(m, pza[SJ * m], pzb[SJ * m]);
+                    } else {
+                        const T q = pza[SJ * m];
+                        upd(m, q, q);
+                    }
                 }
                 KWH_SCHED_FENCE();
             }
@@ -586,14 +754,14 @@ __device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs
                 KWH_SCHED_FENCE();
             }
         }
-        if (sparse_in_tile(a.sp[ax].rowptr, g, ZTile{row0 / 2, P, 0, 0}))
-            reg_apply_sparse<T, J, E>(a.sp[ax], g, row0, 2 * P, lds_all, ps.w, ps.j, a.ctr[0], v);
+        if (rows_have_sparse(a.sp[ax].rowptr, g, ps.row0, ps.nrow))
+            reg_apply_sparse<T, J, E, HALF>(a.sp[ax], g, ps.row0, ps.nrow, lds_all, ps.w, ps.j, a.ctr[0], v);
         T* wa = rho + oa;
         T* wb = rho + ob;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-            if (va) wa[J * m] = v[m].x;
-            if (vb) wb[J * m] = v[m].y;
+            if (va) wa[SJ * m] = v[m].x;
+            if (vb) wb[SJ * m] = v[m].y;
             if constexpr (RT) {
                 if (ax == 0) rt[m] = v[m];
                 else { rt[m].x += v[m].x; rt[m].y += v[m].y; }
@@ -602,45 +770,45 @@ __device__ __forceinline__ void reg_z_density_impl(const BIdx& bi, const RhoArgs
     }
     cx<T> v[E];
     if constexpr (MODE != 1) {
-        const T* tpa = a.tau.lane(oa, ps.j);
-        const T* tpb = a.tau.lane(ob, ps.j);
-        const T* bpa = a.bona.lane(oa, ps.j);
-        const T* bpb = a.bona.lane(ob, ps.j);
-        const T* cpa = a.c0.lane(oa, ps.j);
-        const T* cpb = a.c0.lane(ob, ps.j);
+        const T* tpa = a.tau.lane(oa, ps.na);
+        const T* tpb = a.tau.lane(ob, ps.nb);
+        const T* bpa = a.bona.lane(oa, ps.na);
+        const T* bpb = a.bona.lane(ob, ps.nb);
+        const T* cpa = a.c0.lane(oa, ps.na);
+        const T* cpb = a.c0.lane(ob, ps.nb);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             T ea = rt[m].x, eb = rt[m].y;
-            const T r0a = HOLD ? r0h[m].x : r0pa[J * m];
-            const T r0b = HOLD ? r0h[m].y : r0pb[J * m];
+            const T r0a = HOLD ? r0h[m].x : r0pa[SJ * m];
+            const T r0b = HOLD ? r0h[m].y : r0pb[SJ * m];
             if constexpr (MODE == 2) {
-                ea += tpa[J * m] * sd[m].x;
-                eb += tpb[J * m] * sd[m].y;
+                ea += tpa[SJ * m] * sd[m].x;
+                eb += tpb[SJ * m] * sd[m].y;
             }
             if constexpr (NL) {
-                ea += bpa[J * m] * (rt[m].x * rt[m].x) / (T(2) * r0a);
-                eb += bpb[J * m] * (rt[m].y * rt[m].y) / (T(2) * r0b);
+                ea += bpa[SJ * m] * (rt[m].x * rt[m].x) / (T(2) * r0a);
+                eb += bpb[SJ * m] * (rt[m].y * rt[m].y) / (T(2) * r0b);
             }
-            const T cA = cpa[J * m];
-            const T cB = cpb[J * m];
+            const T cA = cpa[SJ * m];
+            const T cB = cpb[SJ * m];
             const T pA = cA * cA * ea, pB = cB * cB * eb;
-            if (va) a.p[oa + J * m] = pA;
-            if (vb) a.p[ob + J * m] = pB;
+            if (va) a.p[oa + SJ * m] = pA;
+            if (vb) a.p[ob + SJ * m] = pB;
             v[m] = cx<T>{va ? pA : T(0), vb ? pB : T(0)};
         }
-        S::template forward_store<FULL>(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+        zrows_store<T, J, E, HALF, FULL>(a.outP, ps, g, lds, lt, tab, v);
     } else {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cx<T>{va ? sd[m].x : T(0), vb ? sd[m].y : T(0)};
-        S::template forward_store<FULL>(a.F1, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, v);
+        zrows_store<T, J, E, HALF, FULL>(a.F1, ps, g, lds, lt, tab, v);
         if constexpr (RT) {
 #pragma unroll
             for (int m = 0; m < E; ++m) {
-                if (va) a.rhoT[oa + J * m] = rt[m].x;
-                if (vb) a.rhoT[ob + J * m] = rt[m].y;
+                if (va) a.rhoT[oa + SJ * m] = rt[m].x;
+                if (vb) a.rhoT[ob + SJ * m] = rt[m].y;
                 rt[m] = cx<T>{va ? rt[m].x : T(0), vb ? rt[m].y : T(0)};
             }
-            if (a.dispersion) S::template forward_store<FULL>(a.F2, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, rt);
+            if (a.dispersion) zrows_store<T, J, E, HALF, FULL>(a.F2, ps, g, lds, lt, tab, rt);
         }
     }
 }
@@ -702,6 +870,19 @@ __global__ void __launch_bounds__(256, (density_waves<T, J, E, NL, MODE, CORR>()
     reg_z_density(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
     reg_z_density_body<T, J, E, NL, MODE, CORR>(bidx_here(), a, g, P, tab);
 }
+template <typename T, int J, int E, bool NL, int MODE, bool CORR>
+__device__ __forceinline__ void reg_z_density_half_body(const BIdx& bi, const RhoArgs<T>& a, const Geo& g, int P,
+                                                        const AxisTabs<T>& tab) {
+    if (KWH_ROW_FULL_TILES && zrows_tile_full<true>(bi, P, g.nrows))
+        reg_z_density_impl<T, J, E, NL, MODE, CORR, true, true>(bi, a, g, P, tab);
+    else
+        reg_z_density_impl<T, J, E, NL, MODE, CORR, false, true>(bi, a, g, P, tab);
+}
+template <typename T, int J, int E, bool NL, int MODE, bool CORR>
+__global__ void __launch_bounds__(256, (density_waves<T, J, E, NL, MODE, CORR>()))
+    reg_z_density_half(RhoArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    reg_z_density_half_body<T, J, E, NL, MODE, CORR>(bidx_here(), a, g, P, tab);
+}
 
 // EOS with FFT absorption: p = c^2 (rhoT + tau C2R_z(A1) - eta C2R_z(A2) [+ B/A rhoT^2 / (2 rho0)]),
 // P = R2C_z(p). Both spectra are loaded before the first transform (one memory phase).
@@ -709,44 +890,46 @@ template <typename T, int E>
 constexpr int eos_waves() {
     return reg_waves<T, E>(KWH_EOS_PRELOAD && E <= 16 && sizeof(T) == 4 ? KWH_EOS_PRELOAD_WAVES : KWH_EOS_WAVES);
 }
-template <typename T, int J, int E, bool FULL>
+template <typename T, int J, int E, bool FULL, bool HALF = false>
 __device__ __forceinline__ void reg_z_eos_impl(const BIdx& bi, const EosArgs<T>& a, const Geo& g, int P,
                                                const AxisTabs<T>& tab) {
     using S = RegRow<T, J, E>;
+    using Z = ZRows<J, E, HALF>;
+    constexpr int SJ = Z::SJ;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
-    RegZPos<J> ps(P, bi);
+    Z ps;
+    ps.template init<FULL>(P, bi, g.nrows);
     LaneTw<T, J, E> lt;
-    lt.load(ldsb, tab.tw[2], ps.j);
+    lt.template load<HALF ? 2 : 1>(ldsb, tab.tw[2], ps.j);
     cx<T>* lds = ldsb + S::L + ps.w * S::LP;
-    const bool va = FULL || ps.ra < g.nrows, vb = FULL || ps.rb < g.nrows;
-    const int ra = va ? ps.ra : 0, rb = vb ? ps.rb : 0;
+    const bool va = ps.va, vb = ps.vb;
     cx<T> e[E], v[E];
-    const int64_t oa = (int64_t)ra * S::L + ps.j, ob = (int64_t)rb * S::L + ps.j;
-    const T* cpa = a.c0.lane(oa, ps.j);
-    const T* cpb = a.c0.lane(ob, ps.j);
-    S::template load_spectrum<FULL>(a.A1, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, e);
-    if (a.A2) S::template load_spectrum<FULL>(a.A2, ps.ra, ps.rb, g.nrows, g.mzh, ps.j, v);
+    const int64_t oa = ps.oa, ob = ps.ob;
+    const T* cpa = a.c0.lane(oa, ps.na);
+    const T* cpb = a.c0.lane(ob, ps.nb);
+    zrows_load<T, J, E, HALF, FULL>(a.A1, ps, g, tab, e);
+    if (a.A2) zrows_load<T, J, E, HALF, FULL>(a.A2, ps, g, tab, v);
     constexpr bool PRE = KWH_EOS_PRELOAD && E <= 16 && sizeof(T) == 4;
     cx<T> rtp[PRE ? E : 1], cp[PRE ? E : 1];   // sum rho and c0 of the row pair, loaded with the spectra
     if constexpr (PRE) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-            rtp[m] = cx<T>{a.rhoT[oa + J * m], a.rhoT[ob + J * m]};
-            cp[m] = cx<T>{cpa[J * m], cpb[J * m]};
+            rtp[m] = cx<T>{a.rhoT[oa + SJ * m], a.rhoT[ob + SJ * m]};
+            cp[m] = cx<T>{cpa[SJ * m], cpb[SJ * m]};
         }
     }
     S::template transform<true>(e, lds, ps.j, lt);
     if (a.A2) S::template transform<true>(v, lds, ps.j, lt);
     KWH_SCHED_FENCE();
-    const T* tpa = a.tau.lane(oa, ps.j);
-    const T* tpb = a.tau.lane(ob, ps.j);
-    const T* epa = a.eta.lane(oa, ps.j);
-    const T* epb = a.eta.lane(ob, ps.j);
-    const T* bpa = a.bona.lane(oa, ps.j);
-    const T* bpb = a.bona.lane(ob, ps.j);
-    const T* rpa = a.rho0.lane(oa, ps.j);
-    const T* rpb = a.rho0.lane(ob, ps.j);
+    const T* tpa = a.tau.lane(oa, ps.na);
+    const T* tpb = a.tau.lane(ob, ps.nb);
+    const T* epa = a.eta.lane(oa, ps.na);
+    const T* epb = a.eta.lane(ob, ps.nb);
+    const T* bpa = a.bona.lane(oa, ps.na);
+    const T* bpb = a.bona.lane(ob, ps.nb);
+    const T* rpa = a.rho0.lane(oa, ps.na);
+    const T* rpb = a.rho0.lane(ob, ps.nb);
     // The element loop, instantiated per combination of the three run-time switches (uniform over
     // the launch) so that its body carries no per-element branch.
     auto elements = [&](auto from_c, auto has_a2, auto nl) {
@@ -755,9 +938,9 @@ __device__ __forceinline__ void reg_z_eos_impl(const BIdx& bi, const EosArgs<T>&
         for (int g0 = 0; g0 < E; g0 += kGroup) {
 #pragma unroll
             for (int m = g0; m < g0 + kGroup && m < E; ++m) {
-                const int64_t ca = oa + J * m, cb = ob + J * m;
+                const int64_t ca = oa + SJ * m, cb = ob + SJ * m;
                 const T rta = PRE ? rtp[m].x : a.rhoT[ca], rtb = PRE ? rtp[m].y : a.rhoT[cb];
-                const T cA = PRE ? cp[m].x : cpa[J * m], cB = PRE ? cp[m].y : cpb[J * m];
+                const T cA = PRE ? cp[m].x : cpa[SJ * m], cB = PRE ? cp[m].y : cpb[SJ * m];
                 T tauA, etaA = T(0), tauB, etaB = T(0);
                 if constexpr (FROMC) {
                     const T la = log2(cA), lb = log2(cB);
@@ -768,11 +951,11 @@ __device__ __forceinline__ void reg_z_eos_impl(const BIdx& bi, const EosArgs<T>&
                         etaB = a.ek * exp2(a.ee * lb);
                     }
                 } else {
-                    tauA = tpa[J * m];
-                    tauB = tpb[J * m];
+                    tauA = tpa[SJ * m];
+                    tauB = tpb[SJ * m];
                     if constexpr (A2) {
-                        etaA = epa[J * m];
-                        etaB = epb[J * m];
+                        etaA = epa[SJ * m];
+                        etaB = epb[SJ * m];
                     }
                 }
                 T ea = rta + tauA * e[m].x;
@@ -782,8 +965,8 @@ __device__ __forceinline__ void reg_z_eos_impl(const BIdx& bi, const EosArgs<T>&
                     eb -= etaB * v[m].y;
                 }
                 if constexpr (NLIN) {
-                    ea += bpa[J * m] * (rta * rta) / (T(2) * rpa[J * m]);
-                    eb += bpb[J * m] * (rtb * rtb) / (T(2) * rpb[J * m]);
+                    ea += bpa[SJ * m] * (rta * rta) / (T(2) * rpa[SJ * m]);
+                    eb += bpb[SJ * m] * (rtb * rtb) / (T(2) * rpb[SJ * m]);
                 }
                 const T pA = cA * cA * ea, pB = cB * cB * eb;
                 if (va) a.p[ca] = pA;
@@ -811,9 +994,9 @@ __device__ __forceinline__ void reg_z_eos_impl(const BIdx& bi, const EosArgs<T>&
         for (int g0 = 0; g0 < E; g0 += kGroup) {
 #pragma unroll
             for (int m = g0; m < g0 + kGroup && m < E; ++m) {
-                const int64_t ca = oa + J * m, cb = ob + J * m;
+                const int64_t ca = oa + SJ * m, cb = ob + SJ * m;
                 const T rta = PRE ? rtp[m].x : a.rhoT[ca], rtb = PRE ? rtp[m].y : a.rhoT[cb];
-                const T cA = PRE ? cp[m].x : cpa[J * m], cB = PRE ? cp[m].y : cpb[J * m];
+                const T cA = PRE ? cp[m].x : cpa[SJ * m], cB = PRE ? cp[m].y : cpb[SJ * m];
                 T tauA, etaA, tauB, etaB;
                 if (a.tau_eta_from_c) {
                     const T la = log2(cA), lb = log2(cB);
@@ -822,10 +1005,10 @@ __device__ __forceinline__ void reg_z_eos_impl(const BIdx& bi, const EosArgs<T>&
                     etaA = a.ek * exp2(a.ee * la);
                     etaB = a.ek * exp2(a.ee * lb);
                 } else {
-                    tauA = tpa[J * m];
-                    tauB = tpb[J * m];
-                    etaA = epa[J * m];
-                    etaB = epb[J * m];
+                    tauA = tpa[SJ * m];
+                    tauB = tpb[SJ * m];
+                    etaA = epa[SJ * m];
+                    etaB = epb[SJ * m];
                 }
                 T ea = rta + tauA * e[m].x;
                 T eb = rtb + tauB * e[m].y;
@@ -834,8 +1017,8 @@ __device__ __forceinline__ void reg_z_eos_impl(const BIdx& bi, const EosArgs<T>&
                     eb -= etaB * v[m].y;
                 }
                 if (a.nonlinear) {
-                    ea += bpa[J * m] * (rta * rta) / (T(2) * rpa[J * m]);
-                    eb += bpb[J * m] * (rtb * rtb) / (T(2) * rpb[J * m]);
+                    ea += bpa[SJ * m] * (rta * rta) / (T(2) * rpa[SJ * m]);
+                    eb += bpb[SJ * m] * (rtb * rtb) / (T(2) * rpb[SJ * m]);
                 }
                 const T pA = cA * cA * ea, pB = cB * cB * eb;
                 if (va) a.p[ca] = pA;
@@ -846,7 +1029,7 @@ __device__ __forceinline__ void reg_z_eos_impl(const BIdx& bi, const EosArgs<T>&
         }
     }
 #endif
-    S::template forward_store<FULL>(a.outP, ps.ra, ps.rb, g.nrows, g.mzh, lds, ps.j, lt, e);
+    zrows_store<T, J, E, HALF, FULL>(a.outP, ps, g, lds, lt, tab, e);
 }
 template <typename T, int J, int E>
 __device__ __forceinline__ void reg_z_eos_body(const BIdx& bi, const EosArgs<T>& a, const Geo& g, int P, const AxisTabs<T>& tab) {
@@ -860,6 +1043,19 @@ template <typename T, int J, int E>
 __global__ void __launch_bounds__(256, (eos_waves<T, E>()))
     reg_z_eos(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
     reg_z_eos_body<T, J, E>(bidx_here(), a, g, P, tab);
+}
+template <typename T, int J, int E>
+__device__ __forceinline__ void reg_z_eos_half_body(const BIdx& bi, const EosArgs<T>& a, const Geo& g, int P,
+                                                    const AxisTabs<T>& tab) {
+    if (KWH_ROW_FULL_TILES && zrows_tile_full<true>(bi, P, g.nrows))
+        reg_z_eos_impl<T, J, E, true, true>(bi, a, g, P, tab);
+    else
+        reg_z_eos_impl<T, J, E, false, true>(bi, a, g, P, tab);
+}
+template <typename T, int J, int E>
+__global__ void __launch_bounds__(256, (eos_waves<T, E>()))
+    reg_z_eos_half(EosArgs<T> a, Geo g, int P, AxisTabs<T> tab) {
+    reg_z_eos_half_body<T, J, E>(bidx_here(), a, g, P, tab);
 }
 
 // Adjoint forward row kernel (AdjRowArgs, kwh_kernels.h): the diagonal update of the reverse sweep

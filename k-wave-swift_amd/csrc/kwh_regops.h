@@ -38,6 +38,15 @@ struct RegOps {
     // column kernels over tiles of ntr columns (ntr_max or ntr_max / 2), ntr * J lanes
     void (*strided)(const Emit&, dim3, dim3, size_t, const StridedArgs&, const cx<T>*, bool inv, int ntr);
     void (*middle)(const Emit&, dim3, dim3, size_t, const MidArgs&, const AxisTabs<T>&, const Geo&, int ntr);
+    // Rows of this length as one real row per length-L/2 complex transform (even/odd split, half_J
+    // lanes x half_E elements, P rows per workgroup) for the velocity, density and EOS kernels;
+    // half_J = 0: none. The density kernel then also forms rhoT and F2 (no z_rhot pass).
+    int half_J, half_E;
+    void (*z_velocity_h)(const Emit&, dim3, dim3, size_t, const VelArgs<T>&, const Geo&, int, const AxisTabs<T>&,
+                         bool corr);
+    void (*z_density_h)(const Emit&, dim3, dim3, size_t, const RhoArgs<T>&, const Geo&, int, const AxisTabs<T>&,
+                        bool nonlinear, int mode, bool corr);
+    void (*z_eos_h)(const Emit&, dim3, dim3, size_t, const EosArgs<T>&, const Geo&, int, const AxisTabs<T>&);
 };
 
 // Launchers for axis length L, or nullptr when no register-resident kernel covers it.

@@ -7,6 +7,46 @@
 
 namespace kwh {
 
+// Launchers of the half-form row kernels (one real row of length 2 J E per lane group).
+template <typename T, int J, int E>
+struct RegHalfImpl {
+    template <bool CORR>
+    using VelB = Body<&reg_z_velocity_half_body<T, J, E, CORR>, &reg_z_velocity_half<T, J, E, CORR>,
+                      reg_waves<T, E>(KWH_VELOCITY_WAVES), false>;
+    template <bool NL, int MODE, bool CORR>
+    using DenB = Body<&reg_z_density_half_body<T, J, E, NL, MODE, CORR>, &reg_z_density_half<T, J, E, NL, MODE, CORR>,
+                      density_waves<T, J, E, NL, MODE, CORR>(), false>;
+    static void z_velocity(const Emit& em, dim3 gr, dim3 bl, size_t lds, const VelArgs<T>& va, const Geo& g, int P,
+                           const AxisTabs<T>& tab, bool corr) {
+        if (corr) em.go<VelB<true>>(gr, bl, lds, va, g, P, tab);
+        else em.go<VelB<false>>(gr, bl, lds, va, g, P, tab);
+    }
+    template <bool NL>
+    static void density_nl(const Emit& em, dim3 gr, dim3 bl, size_t lds, const RhoArgs<T>& ra, const Geo& g, int P,
+                           const AxisTabs<T>& tab, int mode, bool corr) {
+        if (mode == 1) {
+            if (corr) em.go<DenB<NL, 1, true>>(gr, bl, lds, ra, g, P, tab);
+            else em.go<DenB<NL, 1, false>>(gr, bl, lds, ra, g, P, tab);
+        } else if (mode == 2) {
+            if (corr) em.go<DenB<NL, 2, true>>(gr, bl, lds, ra, g, P, tab);
+            else em.go<DenB<NL, 2, false>>(gr, bl, lds, ra, g, P, tab);
+        } else {
+            if (corr) em.go<DenB<NL, 0, true>>(gr, bl, lds, ra, g, P, tab);
+            else em.go<DenB<NL, 0, false>>(gr, bl, lds, ra, g, P, tab);
+        }
+    }
+    static void z_density(const Emit& em, dim3 gr, dim3 bl, size_t lds, const RhoArgs<T>& ra, const Geo& g, int P,
+                          const AxisTabs<T>& tab, bool nonlinear, int mode, bool corr) {
+        if (nonlinear) density_nl<true>(em, gr, bl, lds, ra, g, P, tab, mode, corr);
+        else density_nl<false>(em, gr, bl, lds, ra, g, P, tab, mode, corr);
+    }
+    static void z_eos(const Emit& em, dim3 gr, dim3 bl, size_t lds, const EosArgs<T>& ea, const Geo& g, int P,
+                      const AxisTabs<T>& tab) {
+        em.go<Body<&reg_z_eos_half_body<T, J, E>, &reg_z_eos_half<T, J, E>, eos_waves<T, E>(), false>>(
+            gr, bl, lds, ea, g, P, tab);
+    }
+};
+
 template <typename T, int J, int E>
 struct RegImpl {
     static_assert(J * E <= kScalarRep, "a scalar map's buffer must cover a row (SMap::lane)");
@@ -125,10 +165,28 @@ struct RegImpl {
         else em.go<MidB<NTR / 2>>(gr, bl, lds, a, tab, g);
     }
 
+    // Length-512 FP32 rows also run in the half form (256-point transforms, 16 x 16): E = 32 row
+    // pairs hold 64 complex registers per lane for the two transforms of the EOS kernel.
+    static constexpr bool kHalf = sizeof(T) == 4 && J == 16 && E == 32;
+    using H = RegHalfImpl<T, J, (kHalf ? E / 2 : E)>;
+    static constexpr decltype(RegOps<T>::z_velocity) half_vel() {
+        if constexpr (kHalf) return &H::z_velocity;
+        else return nullptr;
+    }
+    static constexpr decltype(RegOps<T>::z_density) half_den() {
+        if constexpr (kHalf) return &H::z_density;
+        else return nullptr;
+    }
+    static constexpr decltype(RegOps<T>::z_eos) half_eos() {
+        if constexpr (kHalf) return &H::z_eos;
+        else return nullptr;
+    }
+
     static const RegOps<T>* ops() {
         static const RegOps<T> o{J, E, NTR, &z_forward_real, &z_inverse_real, &z_colloc_rows, &z_velocity,
                                  &z_density, &z_eos, density_rhot_split<T, E, 1>(), &z_rhot, &adj_fwd, &adj_inv,
-                                 &strided, &middle};
+                                 &strided, &middle, kHalf ? J : 0, kHalf ? E / 2 : 0, half_vel(), half_den(),
+                                 half_eos()};
         return &o;
     }
 };

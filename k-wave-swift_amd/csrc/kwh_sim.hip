@@ -412,6 +412,7 @@ struct SimT final : kwh_sim {
         int strided_rev = 1;
         int grad_dual = 1;   // KWH_GRAD_DUAL=0: three x-pass outputs for the gradient
         int corr_fuse = 1;   // KWH_CORR_FUSE=0: the plane source correction as a real field
+        int row_half = 1;    // KWH_ROW_HALF=0: length-512 rows as row pairs in every row kernel
     };
     Schedule sched;
     bool use_sq = true;
@@ -684,6 +685,23 @@ struct SimT final : kwh_sim {
         return z;
     }
 
+    // The velocity, density and EOS row kernels in the half form (RegOps::half_J): one real row per
+    // lane group, P rows per workgroup.
+    bool zhalf() const { return rx[2] && rx[2]->half_J > 0 && sched.row_half; }
+    RegZ regz_half() const {
+        const RegOps<T>* o = rx[2];
+        const int L = o->half_J * o->half_E;   // complex points per transform (rows of 2L)
+        RegZ z{};
+        z.P = ((sizeof(T) == 8 ? 128 : 256) / o->half_J) >> sched.p_shift;
+        const size_t per_row = (size_t)(L + o->half_E + 1) * sizeof(C);
+        const size_t twb = (size_t)L * sizeof(C);
+        while (z.P > 1 && z.P * per_row + twb > 65536) z.P /= 2;
+        z.blocks = (g.nrows + z.P - 1) / z.P;
+        z.threads = z.P * o->half_J;
+        z.lds = (size_t)z.P * per_row + twb;
+        return z;
+    }
+
     // Full forward transform of a real field into `out` (z, then plain passes): leaves the
     // spectrum ready for a middle pass.
     void forward_to_middle(const T* src, C* out, T scale) {
@@ -749,6 +767,7 @@ struct SimT final : kwh_sim {
         if (const char* e = std::getenv("KWH_STRIDED_REV")) sched.strided_rev = std::atoi(e) != 0 ? 1 : 0;
         if (const char* e = std::getenv("KWH_GRAD_DUAL")) sched.grad_dual = std::atoi(e) != 0 ? 1 : 0;
         if (const char* e = std::getenv("KWH_CORR_FUSE")) sched.corr_fuse = std::atoi(e) != 0 ? 1 : 0;
+        if (const char* e = std::getenv("KWH_ROW_HALF")) sched.row_half = std::atoi(e) != 0 ? 1 : 0;
         scal_pool.alloc((size_t)64 * kScalarRep * sizeof(T));
         zero_map = mk_scalar(T(0));
         taus = etas = bonas = zero_map;
@@ -1596,7 +1615,12 @@ struct SimT final : kwh_sim {
                 bytes += 2 * spec_bytes() + 2 * real_bytes() + mapb(dtrs[a].p) + (cu[a] ? real_bytes() : 0);
             bool anycorr = false;
             for (int a = 0; a < dim; ++a) anycorr |= cu[a] != nullptr;
-            if (rx[2]) {
+            if (zhalf()) {
+                const RegZ z = regz_half();
+                timed(K_ZV, bytes, [&] {
+                    rx[2]->z_velocity_h(em(), dim3(z.blocks), dim3(z.threads), z.lds, va, g, z.P, tab, anycorr);
+                });
+            } else if (rx[2]) {
                 const RegZ z = regz();
                 timed(K_ZV, bytes, [&] {
                     rx[2]->z_velocity(em(), dim3(z.blocks), dim3(z.threads), z.lds, va, g, z.P, tab, anycorr);
@@ -1686,14 +1710,19 @@ struct SimT final : kwh_sim {
             if (fft_abs) bytes += real_bytes() + spec_bytes() * (dispersion ? 2 : 1);
             else bytes += real_bytes() + mapb(c0s.p) + spec_bytes() + (stokes ? mapb(taus.p) : 0);
             if (rx[2]) {
-                const RegZ z = regz();
+                const bool half = zhalf();
+                const RegZ z = half ? regz_half() : regz();
                 const int mode = fft_abs ? 1 : (stokes ? 2 : 0);
                 // long rows: rhoT and F2 in a pass of their own (reg_z_rhot), after the density kernel
-                const bool split = mode == 1 && rx[2]->rhot_split;
+                const bool split = mode == 1 && rx[2]->rhot_split && !half;
                 const double rt_bytes = real_bytes() + (dispersion ? spec_bytes() : 0);
                 timed(K_ZD, split ? bytes - rt_bytes : bytes, [&] {
-                    rx[2]->z_density(em(), dim3(z.blocks), dim3(z.threads), z.lds, ra, g, z.P, tab, nonlinear, mode,
-                                     cp != nullptr);
+                    if (half)
+                        rx[2]->z_density_h(em(), dim3(z.blocks), dim3(z.threads), z.lds, ra, g, z.P, tab, nonlinear,
+                                           mode, cp != nullptr);
+                    else
+                        rx[2]->z_density(em(), dim3(z.blocks), dim3(z.threads), z.lds, ra, g, z.P, tab, nonlinear,
+                                         mode, cp != nullptr);
                 });
                 if (split)
                     timed(K_ZRT, dim * real_bytes() + rt_bytes, [&] {
@@ -1741,7 +1770,12 @@ struct SimT final : kwh_sim {
             ea.outP = spec[0].as<C>();
             double bytes = spec_bytes() * (dispersion ? 3 : 2) + 2 * real_bytes() + mapb(c0s.p) + mapb(taus.p) +
                            (dispersion ? mapb(etas.p) : 0) + (nonlinear ? mapb(bonas.p) + mapb(rho0s.p) : 0);
-            if (rx[2]) {
+            if (zhalf()) {
+                const RegZ z = regz_half();
+                timed(K_ZE, bytes, [&] {
+                    rx[2]->z_eos_h(em(), dim3(z.blocks), dim3(z.threads), z.lds, ea, g, z.P, tab);
+                });
+            } else if (rx[2]) {
                 const RegZ z = regz();
                 timed(K_ZE, bytes, [&] {
                     rx[2]->z_eos(em(), dim3(z.blocks), dim3(z.threads), z.lds, ea, g, z.P, tab);

@@ -72,9 +72,9 @@ def test_2d_register_density_variants(absorbing, mode, nonlinear):
 @pytest.mark.parametrize("mode,nonlinear", [(AbsorptionMode.powerLaw, False), (AbsorptionMode.noDispersion, False),
                                             (AbsorptionMode.powerLaw, True), (AbsorptionMode.noDispersion, True)])
 def test_2d_512_rows_fft_absorption(mode, nonlinear):
-    """Length-512 rows with FFT power-law absorption: the density kernel leaves rhoT and, with
-    dispersion, R2C_z(rhoT) to a pass of their own (reg_z_rhot); linear and B/A, with and without
-    dispersion, against the oracle (pFinal and the p series of a line sensor)."""
+    """Length-512 rows with FFT power-law absorption, linear and B/A, with and without dispersion,
+    against the oracle (pFinal and the p series of a line sensor): the half-form density kernel
+    forms rhoT and R2C_z(rhoT) itself (the row-pair form leaves them to reg_z_rhot)."""
     shape = (48, 512)
     med = _medium(shape, seed=75, absorbing=True, nonlinear=nonlinear, mode=mode, y=1.3)
     grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4)
@@ -83,6 +83,77 @@ def test_2d_512_rows_fft_absorption(mode, nonlinear):
     sens[30, :] = 1
     check(f"2d_512_rows_{mode.value}_{nonlinear}", grid, med, KWaveSource(p0=makeDisc(*shape, radius=6)),
           KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal), ["p", "pFinal"], pmlSize=10)
+
+
+@pytest.mark.parametrize("absorbing,mode,nonlinear", [
+    (False, AbsorptionMode.powerLaw, False), (False, AbsorptionMode.powerLaw, True),
+    (True, AbsorptionMode.stokes, False), (True, AbsorptionMode.stokes, True)])
+def test_2d_512_rows_density_variants(absorbing, mode, nonlinear):
+    """Length-512 rows in the half form (one real row per 256-point transform) through the EOS and
+    Stokes density kernels, linear and B/A, with an additive (k-space corrected) pressure source."""
+    shape = (48, 512)
+    med = _medium(shape, seed=73, absorbing=absorbing, nonlinear=nonlinear, mode=mode, y=1.4)
+    grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4)
+    grid.setTime(100, 0.3 * 1e-4 / float(np.max(med.soundSpeed)))
+    pmask = np.zeros(shape, np.float32)
+    pmask[20, 200:260] = 1
+    sig = (1e5 * toneBurst(1 / grid.dt, 2e6, 2)).astype(np.float32)
+    sens = np.zeros(shape, np.float32)
+    sens[30, :] = 1
+    check(f"2d_512_density_{absorbing}_{mode.value}_{nonlinear}", grid, med,
+          KWaveSource(p0=makeDisc(*shape, radius=6), pMask=pmask, p=sig, pMode=SourceMode.additive),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal), ["p", "pFinal"], pmlSize=10)
+
+
+@pytest.mark.parametrize("mode", [SourceMode.dirichlet, SourceMode.additiveNoCorrection])
+def test_2d_512_rows_point_sources(mode):
+    """Length-512 rows in the half form with per-point pressure and velocity sources (applied in the
+    row kernels' LDS, sample n of a row in slot n / 2): points at even and odd positions along the
+    row, against the oracle."""
+    shape = (40, 512)
+    grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4)
+    grid.setTime(80, 2e-8)
+    pmask = np.zeros(shape, np.float32)
+    pmask[15, 250:254] = 1
+    umask = np.zeros(shape, np.float32)
+    umask[25, 101:106] = 1
+    sig = toneBurst(1 / grid.dt, 1e6, 3).astype(np.float32)
+    sens = np.zeros(shape, np.float32)
+    sens[20, :] = 1
+    src = KWaveSource(pMask=pmask, p=1e4 * sig, pMode=mode, uMask=umask, ux=0.01 * sig, uy=0.005 * sig, uMode=mode)
+    check(f"2d_512_points_{mode.value}", grid, KWaveMedium(random_field(shape, 83, 4.0, 1450.0, 1550.0), 1000.0),
+          src, KWaveSensor(mask=sens, record=RecordField.p | RecordField.ux | RecordField.pFinal),
+          ["p", "ux", "pFinal"], pmlSize=8, smoothP0=False)
+
+
+@pytest.mark.parametrize("shape", [(48, 512), (12, 16, 512)])
+def test_512_rows_half_form_matches_row_pairs(shape):
+    """KWH_ROW_HALF=0 runs the velocity, density and EOS kernels of length-512 rows as row pairs
+    (512-point transforms, 16 x 32) with the summed-density pass: the two forms differ only in FP32
+    rounding (power-law medium with B/A, p0, line sensor)."""
+    dim = len(shape)
+    args = [v for n in shape for v in (n, 1e-4)]
+    grid = KWaveGrid(*args)
+    grid.setTime(40, 2e-8)
+    p0 = makeDisc(*shape, radius=5) if dim == 2 else makeBall(*shape, radius=3)
+    c = random_field(shape, 87, 4.0, 1400.0, 1600.0)
+    med = KWaveMedium(c, 1000.0, alphaCoeff=0.6, alphaPower=1.2, alphaMode=AbsorptionMode.powerLaw, bOnA=5.0)
+    sens = np.zeros(shape, np.float32)
+    sens[shape[0] // 2 + 2] = 1
+    run = lambda: kspaceFirstOrder(grid, med, KWaveSource(p0=p0),  # noqa: E731
+                                   KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal),
+                                   SimulationOptions(pmlSize=4, dtype=F32))
+    print(f"\n[case] half form vs row pairs {shape}", flush=True)
+    a = run()
+    os.environ["KWH_ROW_HALF"] = "0"
+    try:
+        b = run()
+    finally:
+        del os.environ["KWH_ROW_HALF"]
+    for f in ("p", "pFinal"):
+        rl2, rmax = metrics(getattr(a, f), getattr(b, f))
+        print(f"  {f} relL2={rl2:.3e} relMax={rmax:.3e}", flush=True)
+        assert rl2 < 1e-5 and rmax < 1e-4
 
 
 def test_3d_1024_rows_fft_absorption():
