@@ -126,6 +126,65 @@ def test_2d_512_rows_point_sources(mode):
           ["p", "ux", "pFinal"], pmlSize=8, smoothP0=False)
 
 
+def test_3d_512_columns():
+    """Length-512 y columns (the strided passes as two 256-point halves, reg_strided2, with the
+    two-output gradient inverse) on an 8 x 512 x 32 heterogeneous power-law B/A medium with a p0
+    ball and a line sensor along y (p, uy, pFinal), against the oracle."""
+    shape = (8, 512, 32)
+    c = random_field(shape, 93, 4.0, 1400.0, 1600.0)
+    rho = random_field(shape, 94, 4.0, 900.0, 1100.0)
+    med = KWaveMedium(c, rho, alphaCoeff=0.6, alphaPower=1.2, alphaMode=AbsorptionMode.powerLaw, bOnA=5.0)
+    grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4, shape[2], 1e-4)
+    grid.setTime(40, 0.3 * 1e-4 / float(c.max()))
+    sens = np.zeros(shape, np.float32)
+    sens[4, :, 16] = 1
+    check("3d_512_columns", grid, med, KWaveSource(p0=makeBall(*shape, radius=3)),
+          KWaveSensor(mask=sens, record=RecordField.p | RecordField.uy | RecordField.pFinal), ["p", "uy", "pFinal"],
+          pmlSize=3)
+
+
+def test_512_columns_split_matches_full():
+    """KWH_COL_SPLIT=0 runs length-512 columns as one 512-point transform (16 x 32): the two forms
+    differ only in FP32 rounding."""
+    shape = (8, 512, 32)
+    grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4, shape[2], 1e-4)
+    grid.setTime(40, 2e-8)
+    med = KWaveMedium(random_field(shape, 95, 4.0, 1400.0, 1600.0), 1000.0, alphaCoeff=0.6, alphaPower=1.2,
+                      alphaMode=AbsorptionMode.powerLaw)
+    sens = np.zeros(shape, np.float32)
+    sens[5] = 1
+    run = lambda: kspaceFirstOrder(grid, med, KWaveSource(p0=makeBall(*shape, radius=3)),  # noqa: E731
+                                   KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal),
+                                   SimulationOptions(pmlSize=3, dtype=F32))
+    print("\n[case] 512 columns split vs full", flush=True)
+    a = run()
+    os.environ["KWH_COL_SPLIT"] = "0"
+    try:
+        b = run()
+    finally:
+        del os.environ["KWH_COL_SPLIT"]
+    for f in ("p", "pFinal"):
+        rl2, rmax = metrics(getattr(a, f), getattr(b, f))
+        print(f"  {f} relL2={rl2:.3e} relMax={rmax:.3e}", flush=True)
+        assert rl2 < 1e-5 and rmax < 1e-4
+
+
+def test_1d_512_half_form():
+    """1-D grid of 512 points (one row, in a workgroup of 16 rows with 15 masked) with power-law
+    absorption and a dirichlet point source at an odd position, against the oracle."""
+    n = 512
+    grid = KWaveGrid(n, 1e-4)
+    grid.setTime(300, 2e-8)
+    mask = np.zeros(n, np.float32)
+    mask[201] = 1
+    sens = np.zeros(n, np.float32)
+    sens[[170, 240]] = 1   # within reach of the source's wave in 300 steps (0.3 cells per step)
+    sig = toneBurst(1 / grid.dt, 1e6, 3).astype(np.float32)
+    med = KWaveMedium(1500.0, 1000.0, alphaCoeff=0.75, alphaPower=1.5, alphaMode=AbsorptionMode.powerLaw)
+    check("1d_512_half", grid, med, KWaveSource(pMask=mask, p=sig, pMode=SourceMode.dirichlet),
+          KWaveSensor(mask=sens), ["p", "pFinal"], pmlSize=20, smoothP0=False)
+
+
 @pytest.mark.parametrize("shape", [(48, 512), (12, 16, 512)])
 def test_512_rows_half_form_matches_row_pairs(shape):
     """KWH_ROW_HALF=0 runs the velocity, density and EOS kernels of length-512 rows as row pairs
