@@ -97,7 +97,6 @@ struct StridedArgs {
     void* dual_out;
     const void* d1;
     const void* d2;
-    int col_split;   // register pass of length 2L built as two halves: use it (reg_strided2)
 };
 
 template <typename T>

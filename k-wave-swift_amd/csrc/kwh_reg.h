@@ -1283,95 +1283,6 @@ __global__ void __launch_bounds__(256) reg_strided(StridedArgs a, const cx<T>* _
     reg_strided_body<T, J, E, NTR, INV, DUAL>(bidx_here(), a, tw);
 }
 
-// Column pass of length 2L as two length-L halves (decimation in time): lanes of half h take the
-// samples 2n + h, n = j + J m; each half transforms in registers (RegCol, L = J * E), then
-// X[k] = Y0[k] + W_2L^(+-k) Y1[k] and X[k + L] = Y0[k] - W_2L^(+-k) Y1[k] through an LDS exchange.
-// Workgroup: 2 * J * NTR lanes (half-major), the LDS of one length-2L RegCol tile. tw: W_2L^i.
-// Against one length-2L transform (E' = 2E per lane) this holds half the elements per lane.
-template <typename T, int J, int E, int NTR, bool INV, bool DUAL = false>
-__device__ __forceinline__ void reg_strided2_body(const BIdx& bi, const StridedArgs& a, const cx<T>* __restrict__ tw) {
-    using S = RegCol<T, J, E, NTR>;
-    constexpr int L = S::L;
-    extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* const ldsb = reinterpret_cast<cx<T>*>(smem);
-    const int t = threadIdx.x;
-    const int h = t / (J * NTR);
-    const int tl = t - h * (J * NTR);
-    const int w = tl % NTR, j = tl / NTR;
-    LaneTw<T, J, E> lt;
-    lt.template load<2>(ldsb, tw, j);
-    cx<T>* lds = ldsb + L + h * (L * NTR + J);
-    const int tile = a.walk_rev ? bi.tile_rev() : bi.tile();
-    const int64_t c = (int64_t)tile * NTR + w;
-    const bool valid = c < a.ncols;
-    const int64_t cc = valid ? c : 0;
-    const int64_t base = (cc / a.inner) * (int64_t)(2 * L) * a.inner + cc % a.inner;
-    cx<T>* const col = reinterpret_cast<cx<T>*>(a.fields[bi.y]) + base;
-    const int64_t step = (int64_t)(2 * J) * a.stride;
-    cx<T> v[E];
-    {
-        const cx<T>* q = col + (int64_t)(2 * j + h) * a.stride;
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            v[m] = *q;
-            q += step;
-        }
-    }
-    // y0[k] + / - W^k y1[k] for k = j + J m, through the LDS slab: half 0 writes y0, half 1 W^k y1
-    cx<T>* const xch = ldsb + L;
-    auto combine_store = [&](cx<T>* u, cx<T>* dst) {
-        if (h == 1) {
-            const cx<T>* wk = tw + j;
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                const cx<T> c2 = wk[J * m];
-                u[m] = cmul(u[m], INV ? conjv(c2) : c2);
-            }
-        }
-        cx<T>* mine = xch + h * (L * NTR) + (int64_t)j * NTR + w;
-#pragma unroll
-        for (int m = 0; m < E; ++m) mine[J * m * NTR] = u[m];
-        __syncthreads();
-        const cx<T>* other = xch + (1 - h) * (L * NTR) + (int64_t)j * NTR + w;
-        const T sg = h == 0 ? T(1) : T(-1);
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const cx<T> o = other[J * m * NTR];
-            const cx<T> y0 = h == 0 ? u[m] : o, t1 = h == 0 ? o : u[m];
-            u[m] = cx<T>{y0.x + sg * t1.x, y0.y + sg * t1.y};
-        }
-        __syncthreads();
-        if (valid) {
-            cx<T>* q = dst + base + (int64_t)(j + h * L) * a.stride;
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                *q = u[m];
-                q += (int64_t)J * a.stride;
-            }
-        }
-    };
-    if constexpr (DUAL) {
-        if (bi.y == a.dual - 1) {   // uniform over the workgroup
-            cx<T> v2[E];
-            const cx<T> d2 = reinterpret_cast<const cx<T>*>(a.d2)[cc % a.inner];
-            const cx<T>* d1 = reinterpret_cast<const cx<T>*>(a.d1) + (2 * j + h);
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                v2[m] = cmul(v[m], d2);
-                v[m] = cmul(v[m], d1[2 * J * m]);
-            }
-            S::template transform<INV>(v2, lds, j, w, lt);
-            combine_store(v2, reinterpret_cast<cx<T>*>(a.dual_out));
-        }
-    }
-    S::template transform<INV>(v, lds, j, w, lt);
-    combine_store(v, reinterpret_cast<cx<T>*>(a.fields[bi.y]));
-}
-template <typename T, int J, int E, int NTR, bool INV, bool DUAL = false>
-__global__ void __launch_bounds__(256) reg_strided2(StridedArgs a, const cx<T>* __restrict__ tw) {
-    reg_strided2_body<T, J, E, NTR, INV, DUAL>(bidx_here(), a, tw);
-}
-
 // Middle pass: forward along the middle axis, operators, inverse, store. Every table-using
 // operator of one input reads the same real k-space table (checked on the host), loaded once.
 // XL = false: lane (w, j) with w fastest, the inner radix through an LDS transpose (RegCol);

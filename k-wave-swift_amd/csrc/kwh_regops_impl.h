@@ -136,29 +136,8 @@ struct RegImpl {
         em.go<Body<&reg_adj_inv_body<T, J, E>, &reg_adj_inv<T, J, E>, 1, false>>(gr, bl, lds, spec, dst, s, ia, mode, g,
                                                                                  P, tab);
     }
-    // Length-512 FP32 columns as two 256-point halves (reg_strided2, 16 x 16 per lane group, twice
-    // the lanes per tile; the LDS of the length-512 tile covers both halves) when the caller asks
-    // for it (StridedArgs::col_split); otherwise the 512-point column kernel (16 x 32).
-    static constexpr bool kSplit = sizeof(T) == 4 && J == 16 && E == 32;
-    template <int N, bool INV, bool DUAL>
-    using Str2B = Body<&reg_strided2_body<T, J, E / 2, N, INV, DUAL>, &reg_strided2<T, J, E / 2, N, INV, DUAL>, 1, false>;
-    template <bool INV, bool DUAL>
-    static void strided2(const Emit& em, dim3 gr, dim3 bl, size_t lds, const StridedArgs& a, const cx<T>* tw,
-                         int ntr) {
-        const dim3 b2(bl.x * 2, bl.y, bl.z);
-        if (ntr == NTR) em.go<Str2B<NTR, INV, DUAL>>(gr, b2, lds, a, tw);
-        else em.go<Str2B<NTR / 2, INV, DUAL>>(gr, b2, lds, a, tw);
-    }
     static void strided(const Emit& em, dim3 gr, dim3 bl, size_t lds, const StridedArgs& a, const cx<T>* tw, bool inv,
                         int ntr) {
-        if constexpr (kSplit) {
-            if (a.col_split) {
-                if (inv && a.dual > 0) strided2<true, true>(em, gr, bl, lds, a, tw, ntr);
-                else if (inv) strided2<true, false>(em, gr, bl, lds, a, tw, ntr);
-                else strided2<false, false>(em, gr, bl, lds, a, tw, ntr);
-                return;
-            }
-        }
         if (inv && a.dual > 0) {
             if (ntr == NTR) em.go<StrDualB<NTR>>(gr, bl, lds, a, tw);
             else em.go<StrDualB<NTR / 2>>(gr, bl, lds, a, tw);

@@ -413,7 +413,6 @@ struct SimT final : kwh_sim {
         int grad_dual = 1;   // KWH_GRAD_DUAL=0: three x-pass outputs for the gradient
         int corr_fuse = 1;   // KWH_CORR_FUSE=0: the plane source correction as a real field
         int row_half = 1;    // KWH_ROW_HALF=0: length-512 rows as row pairs in every row kernel
-        int col_split = 1;   // KWH_COL_SPLIT=0: length-512 columns as one 512-point transform
     };
     Schedule sched;
     bool use_sq = true;
@@ -591,7 +590,6 @@ struct SimT final : kwh_sim {
             bytes += spec_bytes();
         }
         a.walk_rev = sched.strided_rev;
-        a.col_split = sched.col_split;
         if (const RegOps<T>* o = rx[axis]) {
             const int ntr = reg_ntr(o);
             a.lg_ntr = ilog2(ntr);
@@ -770,7 +768,6 @@ struct SimT final : kwh_sim {
         if (const char* e = std::getenv("KWH_GRAD_DUAL")) sched.grad_dual = std::atoi(e) != 0 ? 1 : 0;
         if (const char* e = std::getenv("KWH_CORR_FUSE")) sched.corr_fuse = std::atoi(e) != 0 ? 1 : 0;
         if (const char* e = std::getenv("KWH_ROW_HALF")) sched.row_half = std::atoi(e) != 0 ? 1 : 0;
-        if (const char* e = std::getenv("KWH_COL_SPLIT")) sched.col_split = std::atoi(e) != 0 ? 1 : 0;
         scal_pool.alloc((size_t)64 * kScalarRep * sizeof(T));
         zero_map = mk_scalar(T(0));
         taus = etas = bonas = zero_map;

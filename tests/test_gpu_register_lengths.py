@@ -127,9 +127,9 @@ def test_2d_512_rows_point_sources(mode):
 
 
 def test_3d_512_columns():
-    """Length-512 y columns (the strided passes as two 256-point halves, reg_strided2, with the
-    two-output gradient inverse) on an 8 x 512 x 32 heterogeneous power-law B/A medium with a p0
-    ball and a line sensor along y (p, uy, pFinal), against the oracle."""
+    """Length-512 y columns (the strided passes, 16 x 32, with the two-output gradient inverse) on
+    an 8 x 512 x 32 heterogeneous power-law B/A medium with a p0 ball and a line sensor along y
+    (p, uy, pFinal), against the oracle."""
     shape = (8, 512, 32)
     c = random_field(shape, 93, 4.0, 1400.0, 1600.0)
     rho = random_field(shape, 94, 4.0, 900.0, 1100.0)
@@ -141,32 +141,6 @@ def test_3d_512_columns():
     check("3d_512_columns", grid, med, KWaveSource(p0=makeBall(*shape, radius=3)),
           KWaveSensor(mask=sens, record=RecordField.p | RecordField.uy | RecordField.pFinal), ["p", "uy", "pFinal"],
           pmlSize=3)
-
-
-def test_512_columns_split_matches_full():
-    """KWH_COL_SPLIT=0 runs length-512 columns as one 512-point transform (16 x 32): the two forms
-    differ only in FP32 rounding."""
-    shape = (8, 512, 32)
-    grid = KWaveGrid(shape[0], 1e-4, shape[1], 1e-4, shape[2], 1e-4)
-    grid.setTime(40, 2e-8)
-    med = KWaveMedium(random_field(shape, 95, 4.0, 1400.0, 1600.0), 1000.0, alphaCoeff=0.6, alphaPower=1.2,
-                      alphaMode=AbsorptionMode.powerLaw)
-    sens = np.zeros(shape, np.float32)
-    sens[5] = 1
-    run = lambda: kspaceFirstOrder(grid, med, KWaveSource(p0=makeBall(*shape, radius=3)),  # noqa: E731
-                                   KWaveSensor(mask=sens, record=RecordField.p | RecordField.pFinal),
-                                   SimulationOptions(pmlSize=3, dtype=F32))
-    print("\n[case] 512 columns split vs full", flush=True)
-    a = run()
-    os.environ["KWH_COL_SPLIT"] = "0"
-    try:
-        b = run()
-    finally:
-        del os.environ["KWH_COL_SPLIT"]
-    for f in ("p", "pFinal"):
-        rl2, rmax = metrics(getattr(a, f), getattr(b, f))
-        print(f"  {f} relL2={rl2:.3e} relMax={rmax:.3e}", flush=True)
-        assert rl2 < 1e-5 and rmax < 1e-4
 
 
 def test_1d_512_half_form():
